@@ -1,0 +1,50 @@
+"""ctypes binding of libdplr_hip.so (the C-ABI of include/dplr.h and include/dplrx.h).
+
+The shared library holds the GPU implementation; there is no Python or CPU
+fallback.  Importing this module raises when the library has not been built.
+"""
+import ctypes
+import os
+from ctypes import POINTER, c_char, c_double, c_int, c_long, c_size_t, c_uint, c_ulonglong, c_void_p
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libdplr_hip.so")
+
+
+from .abi import PLR_DMTX, PLR_HMTX, PLR_LRMTX, PLR_UMTX, PD, PM, DPlrMtx, declare  # noqa: F401
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(make -C dplr_amd/csrc).  There is no CPU fallback.")
+    lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
+    declare(lib)
+    P = c_void_p
+    x = {
+        "dplrx_plan_create": (c_int, [c_int, PM, POINTER(P)]),
+        "dplrx_plan_destroy": (None, [P]),
+        "dplrx_plan_info": (c_int, [P, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
+        "dplrx_plan_bytes": (c_size_t, [P]),
+        "dplrx_gepmv": (c_int, [P, c_char, c_double, P, c_double, P, P]),
+        "dplrx_gepdmm": (c_int, [P, c_char, c_int, c_double, P, c_int, c_double, P, c_int, P]),
+        "dplrx_last_error": (ctypes.c_char_p, []),
+        "dplrx_gen_random": (c_int, [c_int, c_int, c_int, c_double, c_ulonglong, PM]),
+        "dplrx_gen_normal": (c_int, [c_size_t, c_ulonglong, c_uint, PD]),
+    }
+    for name, (res, args) in x.items():
+        if hasattr(lib, name):
+            f = getattr(lib, name)
+            f.restype = res
+            f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def check(rc, what="dplrx call"):
+    if rc != 0:
+        msg = lib.dplrx_last_error()
+        raise RuntimeError(f"{what} failed ({rc}): {msg.decode() if msg else ''}")

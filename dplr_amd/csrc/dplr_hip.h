@@ -1,0 +1,86 @@
+/*
+ * Internal interface between the C host layer (csrc/host) and the HIP layer
+ * (csrc/hip).  Plain C; no HIP or torch types.
+ *
+ * Level-wise layout.  A uniform PLR tree of order N = b * 2^L has L levels
+ * of low-rank blocks.  At depth d there are 2^(d+1) row segments of length
+ * len_d = N >> (d+1); segment s covers rows [s*len_d, (s+1)*len_d).  Segment
+ * s = 2n is the upper half of node n, s = 2n+1 its lower half.  Because the
+ * U factor of node n's ne block lives on its upper half and the U factor of
+ * its sw block on its lower half, the U factors of one level tile all N rows
+ * exactly once; so do the V factors (sw.V on the upper half, ne.V on the
+ * lower half).  Each level is therefore stored as two column-major N x R_d
+ * arrays (column k of level d starts at uoff[d] + k*N) plus sig values
+ * S[soff[d] + s*R_d + k] of the block whose U lives on segment s.  Ranks
+ * below R_d are zero-padded (zero columns, zero sig).  Leaves are N/b
+ * consecutive b x b column-major tiles.
+ *
+ * With t_s = V|_s^T x|_s, the product is
+ *     (A x)|_s = D x|_s + sum_d U_d|_s (S_d[s] .* t_{s^1})
+ * and with t'_s = U|_s^T x|_s the transpose is
+ *     (A^T x)|_s = D^T x|_s + sum_d V_d|_s (S_d[s^1] .* t'_{s^1}).
+ */
+#ifndef DPLR_AMD_HIP_INTERNAL_H
+#define DPLR_AMD_HIP_INTERNAL_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define DPLRH_MAXL 24
+
+typedef struct
+{
+    int N, L, b;                 /* order, depth, leaf order: N == b << L     */
+    int R[DPLRH_MAXL];           /* stored (padded) rank of each level         */
+    size_t uoff[DPLRH_MAXL + 1]; /* element offset of level d in U and in V    */
+    size_t soff[DPLRH_MAXL + 1]; /* element offset of level d in S             */
+    int *rank;                   /* true rank of the block whose U is on
+                                    segment s of level d: rank[(2<<d)-2+s]     */
+    double *U, *V, *S, *D;       /* host arrays in the layout above            */
+} plr_flat;
+
+/* ---- host side (C) ---- */
+int plr_flat_shape(int m, const void *A, plr_flat *f, int verbose);
+int plr_flat_fill(const void *A, plr_flat *f);
+void plr_flat_free(plr_flat *f);
+
+/* ---- HIP side (extern "C" from .hip files) ---- */
+typedef struct dplrh_plan dplrh_plan;
+
+/* Uploads the FP64 level arrays; returns 0 or a negative error code. */
+int dplrh_plan_create(const plr_flat *f, dplrh_plan **out);
+void dplrh_plan_destroy(dplrh_plan *p);
+int dplrh_plan_dims(const dplrh_plan *p, int *N, int *L, int *b, int *R);
+size_t dplrh_plan_bytes(const dplrh_plan *p);
+
+/* y := alpha*op(A)*x + beta*y on device vectors (unit stride). */
+int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
+               const double *dx, double *dy, void *stream);
+/* C := alpha*op(A)*B + beta*C on device column-major blocks. */
+int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
+               const double *dB, int ldb, double *dC, int ldc, void *stream);
+
+/* Device memory helpers for the drop-in entry points. */
+void *dplrh_dev_alloc(size_t bytes);
+void dplrh_dev_free(void *p);
+int dplrh_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int dplrh_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int dplrh_sync(void *stream);
+/* Strided gather/scatter and transpose on device. */
+int dplrh_gather(const double *src, int inc, int n, double *dst, void *stream);
+int dplrh_transpose(int rows, int cols, const double *src, int lds, double *dst,
+                    int ldd, void *stream);
+/* y := beta*beta*y (or 0 when beta == 0): the alpha == 0 quirk of gepmv. */
+int dplrh_scale2(int n, int cols, double beta, double *dy, int ldy, void *stream);
+/* C := beta*C (or 0 when beta == 0). */
+int dplrh_scale(int n, int cols, double beta, double *dy, int ldy, void *stream);
+const char *dplrh_last_error(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

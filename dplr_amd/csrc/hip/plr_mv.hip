@@ -1,0 +1,206 @@
+// PLR matvec and its column-batched form: y := alpha*op(A)*x + beta*y.
+//
+// The reference walks the tree depth-first and issues two GEMVs per
+// low-rank block and one per leaf (src/dplrpmv.c:19-180).  Here the whole
+// tree is two launches over the level-wise arrays:
+//
+//   mv_reduce  one wavefront per (level, rank group, segment, tile) reads a
+//              tile of P = V (or U for A^T) and x, reduces the tile's
+//              contribution to t = P|_s^T x|_s across the wavefront, and the
+//              last tile to arrive for a segment sums the tiles' partials in
+//              a fixed order and writes w_{s^1} = alpha * sig .* t_s.
+//   mv_expand  one thread per row and rank quarter: leaf product plus
+//              sum over levels of Q|_s w_s (Q = U, or V for A^T).
+//
+// Both kernels take a column count: blockIdx.y selects the column, so the
+// same code serves the multi-RHS product for any shape (each column reads
+// the factors once).
+#include "plr_common.h"
+
+static constexpr int KG = 16; // rank group; matches plr_plan.hip
+
+// Sum over the 64 lanes of a wavefront; every lane receives the total.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off, 64);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_mv_reduce(LevelTab a, const double *__restrict__ x,
+                                                   int ldx, double alpha, int trans,
+                                                   double *__restrict__ part, size_t part_stride,
+                                                   int *__restrict__ cnt, size_t cnt_stride,
+                                                   double *__restrict__ w, size_t w_stride)
+{
+    const int lane = threadIdx.x & 63;
+    const int task = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (task >= a.task_off[a.L]) return;
+    const int col = blockIdx.y;
+    x += (size_t)col * ldx;
+    part += (size_t)col * part_stride;
+    cnt += (size_t)col * cnt_stride;
+    w += (size_t)col * w_stride;
+
+    int d = 0;
+    while (task >= a.task_off[d + 1]) ++d;
+    const int loc = task - a.task_off[d];
+    const int nkg = a.nkg[d], tps = a.tps[d];
+    const int kg = loc % nkg;
+    const int st = loc / nkg; // segment * tps + tile
+    const int s = st / tps, tile = st - s * tps;
+    const int R = a.R[d], k0 = kg * KG;
+    const int kn = min(KG, R - k0);
+    const int N = a.N, seg = a.seglen[d], T = a.T[d];
+    const int rbeg = s * seg + tile * T;
+    const int rend = min(rbeg + T, (s + 1) * seg);
+    const double *__restrict__ P = a.P[d] + (size_t)k0 * N;
+
+    double acc[KG];
+#pragma unroll
+    for (int kk = 0; kk < KG; ++kk) acc[kk] = 0.0;
+#pragma unroll 2
+    for (int i = rbeg + lane; i < rend; i += 64) {
+        const double xv = x[i];
+#pragma unroll
+        for (int kk = 0; kk < KG; ++kk)
+            if (kk < kn) acc[kk] += P[(size_t)kk * N + i] * xv;
+    }
+#pragma unroll
+    for (int kk = 0; kk < KG; ++kk) acc[kk] = wave_sum(acc[kk]);
+
+    // lane kk owns rank column k0 + kk from here on
+    double mine = 0.0;
+#pragma unroll
+    for (int kk = 0; kk < KG; ++kk)
+        if (lane == kk) mine = acc[kk];
+
+    const int sp = s ^ 1;                     // segment the coefficients act on
+    const int sidx = trans ? s : sp;          // whose sig scales them
+    const double *S = a.S[d] + (size_t)sidx * R;
+    double *wd = w + a.w_off[d] + (size_t)sp * R;
+
+    if (tps == 1) {
+        if (lane < kn) wd[k0 + lane] = alpha * S[k0 + lane] * mine;
+        return;
+    }
+    double *pd = part + a.part_off[d];
+    if (lane < kn) pd[(size_t)st * R + k0 + lane] = mine;
+    __threadfence();
+    int old = 0;
+    int *c = cnt + a.cnt_off[d] + s * nkg + kg;
+    if (lane == 0) old = atomicAdd(c, 1);
+    old = __shfl(old, 0, 64);
+    if (old != tps - 1) return;
+    // Last tile of this segment: every other tile's partial is visible.
+    __threadfence();
+    if (lane < kn) {
+        double t = 0.0;
+        const double *q = pd + (size_t)s * tps * R + k0 + lane;
+        for (int j = 0; j < tps; ++j) t += __builtin_nontemporal_load(q + (size_t)j * R);
+        wd[k0 + lane] = alpha * S[k0 + lane] * t;
+    }
+    if (lane == 0) atomicExch(c, 0);
+}
+
+__global__ __launch_bounds__(256) void k_mv_expand(LevelTab a, const double *__restrict__ D,
+                                                   const double *__restrict__ x, int ldx,
+                                                   double alpha, double beta, int trans,
+                                                   const double *__restrict__ w, size_t w_stride,
+                                                   double *__restrict__ y, int ldy)
+{
+    __shared__ double red[3][64];
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = blockIdx.y;
+    x += (size_t)col * ldx;
+    y += (size_t)col * ldy;
+    w += (size_t)col * w_stride;
+    const int N = a.N, b = a.b;
+    const int row0 = blockIdx.x * 64;
+    const int i = row0 + lane;
+    double acc = 0.0;
+    if (i < N) {
+        const int leaf = i / b, il = i - leaf * b;
+        const double *__restrict__ Dl = D + (size_t)leaf * b * b;
+        const double *__restrict__ xl = x + (size_t)leaf * b;
+        double sl = 0.0;
+        if (!trans) {
+            for (int j = g; j < b; j += 4) sl += Dl[(size_t)j * b + il] * xl[j];
+        } else {
+            for (int j = g; j < b; j += 4) sl += Dl[(size_t)il * b + j] * xl[j];
+        }
+        acc = alpha * sl;
+        for (int d = 0; d < a.L; ++d) {
+            const int R = a.R[d], seg = a.seglen[d];
+            const double *__restrict__ Q = a.Q[d];
+            if ((seg & 63) == 0) {
+                // the wavefront's 64 rows share one segment: uniform coefficients
+                const int s = __builtin_amdgcn_readfirstlane(row0 / seg);
+                const double *__restrict__ wd = w + a.w_off[d] + (size_t)s * R;
+#pragma unroll 4
+                for (int k = g; k < R; k += 4) acc += Q[(size_t)k * N + i] * wd[k];
+            } else {
+                const int s = i / seg;
+                const double *__restrict__ wd = w + a.w_off[d] + (size_t)s * R;
+                for (int k = g; k < R; k += 4) acc += Q[(size_t)k * N + i] * wd[k];
+            }
+        }
+    }
+    if (g > 0) red[g - 1][lane] = acc;
+    __syncthreads();
+    if (g == 0 && i < N) {
+        const double r = acc + red[0][lane] + red[1][lane] + red[2][lane];
+        y[i] = (beta == 0.0 ? 0.0 : beta * y[i]) + r;
+    }
+}
+
+// y := alpha*op(A)*x + beta*y for ncols columns (alpha != 0), in chunks of
+// at most kColChunk columns so the workspace and gridDim.y stay bounded.
+static constexpr int kColChunk = 64;
+
+int plr_mv_launch(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
+                  const double *dx, int ldx, double *dy, int ldy, hipStream_t st)
+{
+    const int chunk = ncols < kColChunk ? ncols : kColChunk;
+    int rc = plr_workspace(p, chunk);
+    if (rc) return rc;
+    LevelTab t;
+    plr_level_tab(p, trans, &t);
+    const int ntask = t.task_off[t.L];
+    for (int c0 = 0; c0 < ncols; c0 += chunk) {
+        const int nc = ncols - c0 < chunk ? ncols - c0 : chunk;
+        const double *x = dx + (size_t)c0 * ldx;
+        double *y = dy + (size_t)c0 * ldy;
+        if (ntask > 0) {
+            dim3 grid((ntask + 3) / 4, nc);
+            hipLaunchKernelGGL(k_mv_reduce, grid, dim3(256), 0, st, t, x, ldx, alpha, trans,
+                               p->part, p->part_per_col, p->cnt, p->cnt_per_col, p->w,
+                               p->w_per_col);
+            PLR_HIP(hipGetLastError());
+        }
+        dim3 grid2((p->N + 63) / 64, nc);
+        hipLaunchKernelGGL(k_mv_expand, grid2, dim3(256), 0, st, t, p->D, x, ldx, alpha, beta,
+                           trans, p->w, p->w_per_col, y, ldy);
+        PLR_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+extern "C" int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
+                          const double *dx, double *dy, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (p->N == 0) return 0;
+    if (alpha == 0.0) return dplrh_scale2(p->N, 1, beta, dy, p->N, stream);
+    return plr_mv_launch(p, trans, 1, alpha, beta, dx, p->N, dy, p->N, st);
+}
+
+extern "C" int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
+                          const double *dB, int ldb, double *dC, int ldc, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (p->N == 0 || n == 0) return 0;
+    if (alpha == 0.0) return dplrh_scale(p->N, n, beta, dC, ldc, stream);
+    return plr_mv_launch(p, trans, n, alpha, beta, dB, ldb, dC, ldc, st);
+}

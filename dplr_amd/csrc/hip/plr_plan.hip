@@ -1,0 +1,265 @@
+// Device plan: upload of the level-wise FP64 arrays, kernel workspace, small
+// utility kernels and the C entry points of the HIP layer.
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+
+#include "plr_common.h"
+
+static char g_err[512];
+
+void plr_set_error(const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+}
+
+extern "C" const char *dplrh_last_error(void) { return g_err; }
+
+static constexpr int kMaxTile = 512; // rows per reduction tile (8 per lane)
+static constexpr int kKG = 16;       // rank group of the reduction kernel
+
+void plr_level_tab(const dplrh_plan *p, int trans, LevelTab *t)
+{
+    memset(t, 0, sizeof(*t));
+    t->L = p->L;
+    t->N = p->N;
+    t->b = p->b;
+    int task = 0, part = 0, cnt = 0;
+    for (int d = 0; d < p->L; ++d) {
+        const int seg = p->N >> (d + 1);
+        const int nseg = 2 << d;
+        const int R = p->R[d];
+        const int tps = (seg + kMaxTile - 1) / kMaxTile;
+        int T = (seg + tps - 1) / tps;
+        T = (T + 63) & ~63;
+        const int nkg = (R + kKG - 1) / kKG;
+        t->R[d] = R;
+        t->seglen[d] = seg;
+        t->T[d] = T;
+        t->tps[d] = tps;
+        t->nkg[d] = nkg;
+        t->task_off[d] = task;
+        task += nkg * nseg * tps;
+        t->part_off[d] = part;
+        if (tps > 1) part += nseg * tps * R;
+        t->cnt_off[d] = cnt;
+        if (tps > 1) cnt += nseg * nkg;
+        t->w_off[d] = (int)p->soff[d];
+        const double *Ud = p->U + p->uoff[d], *Vd = p->V + p->uoff[d];
+        t->P[d] = trans ? Ud : Vd;
+        t->Q[d] = trans ? Vd : Ud;
+        t->S[d] = p->S + p->soff[d];
+    }
+    t->task_off[p->L] = task;
+    t->cnt_off[p->L] = cnt;
+    t->w_off[p->L] = (int)p->soff[p->L];
+}
+
+int plr_workspace(dplrh_plan *p, int ncols)
+{
+    if (ncols <= p->wk_cols) return 0;
+    LevelTab t;
+    plr_level_tab(p, 0, &t);
+    size_t part = 0;
+    for (int d = 0; d < p->L; ++d)
+        if (t.tps[d] > 1) part += (size_t)(2 << d) * t.tps[d] * t.R[d];
+    const size_t w = p->soff[p->L];
+    const size_t cnt = (size_t)t.cnt_off[p->L];
+    if (p->part) (void)hipFree(p->part);
+    if (p->w) (void)hipFree(p->w);
+    if (p->cnt) (void)hipFree(p->cnt);
+    p->part = nullptr;
+    p->w = nullptr;
+    p->cnt = nullptr;
+    p->wk_cols = 0;
+    p->part_per_col = part ? part : 1;
+    p->w_per_col = w ? w : 1;
+    p->cnt_per_col = cnt ? cnt : 1;
+    PLR_HIP(hipMalloc(&p->part, p->part_per_col * ncols * sizeof(double)));
+    PLR_HIP(hipMalloc(&p->w, p->w_per_col * ncols * sizeof(double)));
+    PLR_HIP(hipMalloc(&p->cnt, p->cnt_per_col * ncols * sizeof(int)));
+    // Arrival counters start at zero; every kernel that uses one returns it
+    // to zero, so this is the only clear they need.
+    PLR_HIP(hipMemset(p->cnt, 0, p->cnt_per_col * ncols * sizeof(int)));
+    PLR_HIP(hipDeviceSynchronize());
+    p->wk_cols = ncols;
+    return 0;
+}
+
+extern "C" int dplrh_plan_create(const plr_flat *f, dplrh_plan **out)
+{
+    *out = nullptr;
+    dplrh_plan *p = new dplrh_plan;
+    memset(p, 0, sizeof(*p));
+    p->N = f->N;
+    p->L = f->L;
+    p->b = f->b;
+    for (int d = 0; d < f->L; ++d) p->R[d] = f->R[d];
+    for (int d = 0; d <= f->L; ++d) {
+        p->uoff[d] = f->uoff[d];
+        p->soff[d] = f->soff[d];
+    }
+    const size_t nuv = f->uoff[f->L], ns = f->soff[f->L];
+    const size_t nd = (size_t)f->N * f->b;
+    const size_t nr = (size_t)2 << f->L;
+    int rc = 0;
+    auto up = [&](void **dst, const void *src, size_t bytes) -> int {
+        PLR_HIP(hipMalloc(dst, bytes ? bytes : 8));
+        if (bytes) PLR_HIP(hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    if ((rc = up((void **)&p->U, f->U, nuv * sizeof(double))) ||
+        (rc = up((void **)&p->V, f->V, nuv * sizeof(double))) ||
+        (rc = up((void **)&p->S, f->S, ns * sizeof(double))) ||
+        (rc = up((void **)&p->D, f->D, nd * sizeof(double))) ||
+        (rc = up((void **)&p->rank, f->rank, nr * sizeof(int))) ||
+        (rc = plr_workspace(p, 1))) {
+        dplrh_plan_destroy(p);
+        return rc;
+    }
+    *out = p;
+    return 0;
+}
+
+extern "C" void dplrh_plan_destroy(dplrh_plan *p)
+{
+    if (!p) return;
+    void *bufs[] = {p->U, p->V, p->S, p->D, p->rank, p->part, p->w, p->cnt, p->tmp};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    delete p;
+}
+
+extern "C" int dplrh_plan_dims(const dplrh_plan *p, int *N, int *L, int *b, int *R)
+{
+    *N = p->N;
+    *L = p->L;
+    *b = p->b;
+    for (int d = 0; d < p->L; ++d) R[d] = p->R[d];
+    return 0;
+}
+
+extern "C" size_t dplrh_plan_bytes(const dplrh_plan *p)
+{
+    return (2 * p->uoff[p->L] + p->soff[p->L] + (size_t)p->N * p->b) * sizeof(double);
+}
+
+// ---------------------------------------------------------------------------
+// utilities
+
+__global__ void k_gather(const double *__restrict__ src, int inc, int n, double *__restrict__ dst)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = src[(size_t)i * inc];
+}
+
+// dst[c*ldd + r] = src[r*lds + c]: dst is rows x cols, src is cols x rows.
+__global__ void k_transpose(int rows, int cols, const double *__restrict__ src, int lds,
+                            double *__restrict__ dst, int ldd)
+{
+    __shared__ double tile[32][33];
+    const int r0 = blockIdx.x * 32, c0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5; // 256 threads: 8 rows of 32
+    for (int j = ty; j < 32; j += 8) {
+        int c = c0 + j, r = r0 + tx;
+        if (c < cols && r < rows) tile[j][tx] = src[(size_t)r * lds + c];
+    }
+    __syncthreads();
+    for (int j = ty; j < 32; j += 8) {
+        int r = r0 + j, c = c0 + tx;
+        if (c < cols && r < rows) dst[(size_t)c * ldd + r] = tile[tx][j];
+    }
+}
+
+__global__ void k_scale(int n, int cols, double beta, int twice, double *__restrict__ y, int ldy)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    int c = blockIdx.y;
+    if (i >= n || c >= cols) return;
+    double *p = y + (size_t)c * ldy + i;
+    if (beta == 0.0) {
+        *p = 0.0;
+    } else {
+        double v = beta * *p;
+        *p = twice ? beta * v : v;
+    }
+}
+
+extern "C" int dplrh_gather(const double *src, int inc, int n, double *dst, void *stream)
+{
+    if (n <= 0) return 0;
+    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                       src, inc, n, dst);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int dplrh_transpose(int rows, int cols, const double *src, int lds, double *dst,
+                               int ldd, void *stream)
+{
+    if (rows <= 0 || cols <= 0) return 0;
+    dim3 grid((rows + 31) / 32, (cols + 31) / 32);
+    hipLaunchKernelGGL(k_transpose, grid, dim3(256), 0, (hipStream_t)stream, rows, cols, src,
+                       lds, dst, ldd);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+static int launch_scale(int n, int cols, double beta, int twice, double *dy, int ldy,
+                        void *stream)
+{
+    if (n <= 0 || cols <= 0) return 0;
+    hipLaunchKernelGGL(k_scale, dim3((n + 255) / 256, cols), dim3(256), 0,
+                       (hipStream_t)stream, n, cols, beta, twice, dy, ldy);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+extern "C" int dplrh_scale2(int n, int cols, double beta, double *dy, int ldy, void *stream)
+{
+    return launch_scale(n, cols, beta, 1, dy, ldy, stream);
+}
+
+extern "C" int dplrh_scale(int n, int cols, double beta, double *dy, int ldy, void *stream)
+{
+    return launch_scale(n, cols, beta, 0, dy, ldy, stream);
+}
+
+extern "C" void *dplrh_dev_alloc(size_t bytes)
+{
+    void *p = nullptr;
+    if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) {
+        plr_set_error("hipMalloc(%zu) failed", bytes);
+        return nullptr;
+    }
+    return p;
+}
+
+extern "C" void dplrh_dev_free(void *p)
+{
+    if (p) (void)hipFree(p);
+}
+
+extern "C" int dplrh_h2d(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    PLR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int dplrh_d2h(void *dst, const void *src, size_t bytes, void *stream)
+{
+    if (!bytes) return 0;
+    PLR_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream));
+    PLR_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}
+
+extern "C" int dplrh_sync(void *stream)
+{
+    PLR_HIP(hipStreamSynchronize((hipStream_t)stream));
+    return 0;
+}

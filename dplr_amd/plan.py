@@ -1,0 +1,62 @@
+"""Device-resident PLR plans (include/dplrx.h) driven with torch device tensors.
+
+torch only provides device memory and the stream; every operation is a call
+into libdplr_hip.so.
+"""
+import ctypes
+from ctypes import byref, c_int, c_void_p
+
+from ._lib import check, lib
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream") else int(stream))
+
+
+def _dptr(t):
+    return c_void_p(t.data_ptr())
+
+
+class Plan:
+    """An uploaded PLR matrix (level-wise FP64 arrays in HBM)."""
+
+    def __init__(self, tree):
+        p = c_void_p()
+        check(lib.dplrx_plan_create(tree.m, tree.ref, byref(p)), "dplrx_plan_create")
+        self.p = p
+        N, L, b = c_int(), c_int(), c_int()
+        R = (c_int * 32)()
+        lib.dplrx_plan_info(self.p, byref(N), byref(L), byref(b), R)
+        self.m, self.nlvl, self.leaf = N.value, L.value, b.value
+        self.ranks = [R[d] for d in range(self.nlvl)]
+
+    @property
+    def nbytes(self):
+        return int(lib.dplrx_plan_bytes(self.p))
+
+    def close(self):
+        if getattr(self, "p", None):
+            lib.dplrx_plan_destroy(self.p)
+            self.p = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def gepmv(self, trans, alpha, x, beta, y, stream=None):
+        check(lib.dplrx_gepmv(self.p, trans.encode(), alpha, _dptr(x), beta, _dptr(y),
+                              _stream(stream)), "dplrx_gepmv")
+        return y
+
+    def gepdmm(self, trans, alpha, B, beta, C, stream=None):
+        """C := alpha*op(A)*B + beta*C; B, C are (n, m) row-major torch tensors,
+        i.e. column-major m x n blocks with leading dimension m."""
+        n, m = B.shape
+        check(lib.dplrx_gepdmm(self.p, trans.encode(), n, alpha, _dptr(B), B.stride(0), beta,
+                               _dptr(C), C.stride(0), _stream(stream)), "dplrx_gepdmm")
+        return C

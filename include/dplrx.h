@@ -1,0 +1,63 @@
+/*
+ * dplrx.h -- device-resident extension API of the MI355X PLR library.
+ *
+ * The drop-in calls of dplr.h take host memory and re-upload the matrix on
+ * every call, because the caller may change the tree between calls.  These
+ * entry points have no counterpart in the reference: they keep a matrix in
+ * HBM as a "plan" (level-wise arrays, see dplr_hip.h) and take device
+ * pointers, so a caller can apply and solve many right-hand sides without
+ * host traffic.  Every function returns 0 on success and a negative code on
+ * failure, with a message from dplrx_last_error().  `stream` is a
+ * hipStream_t (NULL = the null stream).  Calls on one plan must not overlap
+ * in time: the plan's workspace is shared.
+ */
+#ifndef DPLR_AMD_DPLRX_H
+#define DPLR_AMD_DPLRX_H
+
+#include <stddef.h>
+
+#include "dplr.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dplrh_plan dplrx_plan;
+
+/* Upload the m x m uniform PLR tree A. */
+int dplrx_plan_create(int m, DPlrMtx *A, dplrx_plan **plan);
+void dplrx_plan_destroy(dplrx_plan *plan);
+/* Order, depth, leaf order and the stored rank of each level (ranks must
+ * hold depth entries; pass NULL to skip). */
+int dplrx_plan_info(const dplrx_plan *plan, int *m, int *nlvl, int *leaf, int *ranks);
+/* Bytes of the FP64 level arrays (U, V, sig, leaves) held in HBM. */
+size_t dplrx_plan_bytes(const dplrx_plan *plan);
+
+/* y := alpha*op(A)*x + beta*y with device vectors of length m.  trans as in
+ * dplrgepmv, including the alpha == 0 quirk. */
+int dplrx_gepmv(dplrx_plan *plan, char trans, double alpha, const double *dx,
+                double beta, double *dy, void *stream);
+
+/* C := alpha*op(A)*B + beta*C with device column-major m x n blocks
+ * (ldb, ldc >= m). */
+int dplrx_gepdmm(dplrx_plan *plan, char transa, int n, double alpha, const double *dB,
+                 int ldb, double beta, double *dC, int ldc, void *stream);
+
+const char *dplrx_last_error(void);
+
+/* ---- synthetic inputs (deterministic, counter-based RNG) --------------- */
+
+/* Random uniform PLR tree of order m = leaf * 2^nlvl with every off-diagonal
+ * block U*diag(s)*V^T, U and V orthonormal (Householder QR of N(0,1)
+ * matrices), s_j = 10^(-j/4), and leaves N(0,1)/sqrt(leaf) + shift*I.
+ * A must be dplrini'd; it is reshaped. */
+int dplrx_gen_random(int m, int nlvl, int rank, double shift, unsigned long long seed,
+                     DPlrMtx *A);
+/* n samples of N(0,1) from stream `stream` of `seed`. */
+int dplrx_gen_normal(size_t n, unsigned long long seed, unsigned int stream, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

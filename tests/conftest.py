@@ -1,0 +1,36 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs the HIP kernels)")
+
+
+@pytest.fixture(scope="session")
+def orc():
+    import oracle
+    return oracle.load_oracle()
+
+
+@pytest.fixture(scope="session")
+def ref():
+    """The reference sources built into oracle/_ref, or skip when absent."""
+    import oracle
+    api = oracle.load_ref()
+    if api is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    return api
+
+
+@pytest.fixture(scope="session")
+def cuda():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    return torch.device("cuda:0")
