@@ -19,6 +19,15 @@ def _load():
         raise ImportError(
             f"{LIB_PATH} is missing: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
             "(make -C dplr_amd/csrc).  There is no CPU fallback.")
+    # One HIP runtime per process: torch ships its own libamdhip64 (soname
+    # libamdhip64.so.7, the same soname as /opt/rocm's).  Loading torch first
+    # makes the dynamic loader resolve this library's libamdhip64.so.7 to
+    # torch's copy, so device pointers from torch tensors are valid here and
+    # no second HSA runtime competes for the device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH, mode=ctypes.RTLD_LOCAL)
     declare(lib)
     P = c_void_p
@@ -30,6 +39,10 @@ def _load():
         "dplrx_gepmv": (c_int, [P, c_char, c_double, P, c_double, P, P]),
         "dplrx_gepdmm": (c_int, [P, c_char, c_int, c_double, P, c_int, c_double, P, c_int, P]),
         "dplrx_last_error": (ctypes.c_char_p, []),
+        "dplrx_factor": (c_int, [P, POINTER(c_int), P]),
+        "dplrx_solve": (c_int, [P, c_char, c_int, P, c_int, P, c_int, P]),
+        "dplrx_factor_mixed": (c_int, [P, c_int, POINTER(c_int), P]),
+        "dplrx_solve_refined": (c_int, [P, c_int, P, c_int, P, c_int, c_int, PD, P]),
         "dplrx_gen_random": (c_int, [c_int, c_int, c_int, c_double, c_ulonglong, PM]),
         "dplrx_gen_normal": (c_int, [c_size_t, c_ulonglong, c_uint, PD]),
     }

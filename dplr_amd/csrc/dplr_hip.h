@@ -63,6 +63,34 @@ int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
 int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
                const double *dB, int ldb, double *dC, int ldc, void *stream);
 
+/* FP64 factored inverse (recursive Woodbury, product form); *info > 0 when
+ * a leaf or capacitance matrix is singular. */
+int dplrh_factor(dplrh_plan *p, int *info, void *stream);
+/* X := op(A)^{-1} B on device columns, after dplrh_factor. */
+int dplrh_solve(dplrh_plan *p, int trans, int n, const double *dB, int ldb, double *dX,
+                int ldx, void *stream);
+
+/* Mixed-precision factors (type 1 = f16, 2 = bf16) and refined solve:
+ * `iters` low-precision solves with FP64 residuals between them; res[c]
+ * (optional, host) receives max|b - A x| of column c afterwards. */
+int dplrh_lp_factor(dplrh_plan *p, int type, int *info, void *stream);
+int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, double *dX, int ldx,
+                   int iters, double *res, void *stream);
+
+/* Explicit inverse of a factored plan, downloaded per level d' into
+ * hBU[d'], hBV[d'] (N*Rt[d'] doubles, [segment][column][row]) and hBsig[d']
+ * (segments*Rt[d']), plus the leaves hBD.  Level d' buffers hold the own
+ * columns first (R[d']) and the corrections from level d at column
+ * coff[d'][d]. */
+int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][DPLRH_MAXL],
+                double **hBU, double **hBV, double **hBsig, double *hBD, void *stream);
+/* Rank-k update data: per level d the k appended columns of every block
+ * ([segment][column][row], scaled as dplrgeprk stores them) and their sig;
+ * the leaves hD are updated in place. */
+int dplrh_geprk(int N, int L, int b, int k, double alpha, double eps, const double *hu,
+                const double *hv, const double *hs, double *hD, double **hUn, double **hVn,
+                double **hsig, void *stream);
+
 /* Device memory helpers for the drop-in entry points. */
 void *dplrh_dev_alloc(size_t bytes);
 void dplrh_dev_free(void *p);

@@ -28,6 +28,33 @@ struct LevelTab
     const double *S[PLR_MAXL];
 };
 
+// Factored inverse (recursive Woodbury in product form), FP64:
+//   inv(A) = F_0 F_1 ... F_{L-1} inv(D_leaf),  F_d = I - Ut_d S_d inv(K_d) Z_d^T
+// with Ut_d = (block-diagonal inverse below level d) * U_d, and for node n of
+// level d the capacitance  K = [I, M_bot S_bot; M_top S_top, I],
+// M_s = V_d|_s^T Ut_d|_s.  Kinv holds inv(K) per node, (2R_d)^2 each.
+struct plr_fact
+{
+    int done;
+    int info;
+    double *Dinv;                  // leaves' inverses, N/b tiles of b x b
+    double *Ut;                    // Ut, layout of U
+    double *Kinv;                  // capacitance inverses
+    size_t koff[PLR_MAXL + 1];     // element offset of level d in Kinv
+    int *dinfo;                    // per-matrix LU status
+    double *work;                  // LU scratch (leaves, capacitances)
+};
+
+// Workspace of the per-level solve kernels, sized for `cols` columns.
+struct plr_solve_ws
+{
+    int cols;
+    size_t t_per_col, part_per_col, cnt_per_col;
+    double *T, *W, *part, *tmp;
+    int *cnt;
+    size_t tmp_cap;
+};
+
 struct dplrh_plan
 {
     int N, L, b;
@@ -42,6 +69,9 @@ struct dplrh_plan
     size_t part_per_col, w_per_col, cnt_per_col;
     double *tmp; // op(B) staging for transb
     size_t tmp_cap;
+    plr_fact f;
+    plr_solve_ws sw;
+    int force_valu; // testing aid: use the general-shape kernels everywhere
 };
 
 void plr_set_error(const char *fmt, ...);
@@ -59,3 +89,44 @@ void plr_set_error(const char *fmt, ...);
 void plr_level_tab(const dplrh_plan *p, int trans, LevelTab *t);
 // Make sure the workspace holds ncols columns.
 int plr_workspace(dplrh_plan *p, int ncols);
+// Level table for level d alone, with explicit operands (P reduced, Q expanded).
+void plr_level_tab1(const dplrh_plan *p, int d, const double *P, const double *Q,
+                    const double *S, LevelTab *t);
+// A table with no levels (leaf term only).
+void plr_level_tab0(const dplrh_plan *p, LevelTab *t);
+
+int plr_launch_reduce(const LevelTab &t, const double *x, int ldx, double alpha, int trans,
+                      int raw, int ncols, double *part, size_t pstride, int *cnt, size_t cstride,
+                      double *w, size_t wstride, hipStream_t st);
+int plr_launch_expand(const LevelTab &t, const double *D, const double *x, int ldx, double alpha,
+                      double beta, int trans, const double *w, size_t wstride, double *y, int ldy,
+                      int ncols, hipStream_t st);
+void plr_free_fact(dplrh_plan *p);
+
+// Batched FP64 LU with partial pivoting and inverse of `count` n x n
+// column-major matrices (K is overwritten by the LU factors); *info = the
+// first nonzero dgetrf-style status in matrix order.
+int dplrh_lp_lu_inverse(double *K, int n, int count, double *Kinv, int *info, hipStream_t st);
+// Single-level building blocks (plr_solve.hip), MFMA when possible.
+int plr_lvl_reduce(dplrh_plan *p, int d, const double *P, const double *X, int ldx, int ncols,
+                   double *T, size_t tstride, hipStream_t st);
+int plr_lvl_expand(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,
+                   const double *D, int trans, const double *X, int ldx, double alpha,
+                   double beta, double *Y, int ldy, int ncols, hipStream_t st);
+int plr_lvl_cap(dplrh_plan *p, int d, const double *T, size_t tstride, double *W, int ncols,
+                int trans, hipStream_t st);
+int plr_solve_reserve(dplrh_plan *p, int cols);
+
+// Mixed-precision factors of a plan (plr_lp.hip).
+void plr_lp_release(const dplrh_plan *p);
+
+// FP64 MFMA kernels (plr_mm.hip); usable when plr_mm_ok(p).
+int plr_mm_ok(const dplrh_plan *p);
+int plr_mm_apply(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
+                 const double *X, int ldx, double *Y, int ldy, double *w, size_t wstride,
+                 hipStream_t st);
+int plr_mm_reduce1(dplrh_plan *p, int d, const double *P, const double *X, int ldx, int ncols,
+                   double *T, size_t tstride, hipStream_t st);
+int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,
+                   const double *D, int trans, const double *X, int ldx, double alpha, double beta,
+                   double *Y, int ldy, int ncols, hipStream_t st);

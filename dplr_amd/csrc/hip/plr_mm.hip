@@ -1,0 +1,381 @@
+// Multi-RHS PLR products on FP64 MFMA (v_mfma_f64_16x16x4_f64).
+//
+// Lane layout of the instruction on gfx950 (measured): with l the lane,
+//   A (16x4):  lane l holds A[l%16][l/16]
+//   B (4x16):  lane l holds B[l/16][l%16]
+//   D (16x16): lane l holds D[l/16 + 4v][l%16], v = 0..3.
+//
+// mm_reduce: per (level, segment, row tile, 32-rank group, 64-column group)
+//   one wavefront computes T = P|_tile^T X|_tile.  The MFMA K dimension runs
+//   over rows: in step t lane group g = l/16 contributes row rb + 16g + t, so
+//   each lane streams 16 consecutive doubles of one column per 64-row block.
+//   A tile that covers its whole segment writes the finished coefficients;
+//   otherwise it writes a partial that mm_finish sums in a fixed order.
+// mm_expand: one wavefront per 16 rows x 64 columns computes
+//   Y^T = beta Y^T + alpha X_leaf^T D^T + sum_d W_d^T Q_d^T, so the D output
+//   rows are RHS columns and its columns are matrix rows: lane l%16 walks 16
+//   consecutive matrix rows and the stores are contiguous.
+#include <cstring>
+
+#include "plr_common.h"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+static constexpr int MM_KG = 32;   // ranks per task (2 MFMA tiles)
+static constexpr int MM_CG = 64;   // columns per task (4 MFMA tiles)
+static constexpr int MM_TMAX = 256; // rows per reduction task
+static constexpr int MM_BLK = MM_KG * MM_CG;
+
+struct MmTab
+{
+    int L, N, b, ncols, ncg;
+    int R[PLR_MAXL], seglen[PLR_MAXL], T[PLR_MAXL], tps[PLR_MAXL], nkg[PLR_MAXL];
+    int task_off[PLR_MAXL + 1];
+    int part_off[PLR_MAXL + 1]; // in blocks of MM_BLK doubles
+    int w_off[PLR_MAXL + 1];
+    int fin_off[PLR_MAXL + 1];  // finishing tasks (segments x groups of levels with tps > 1)
+    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+};
+
+// ---------------------------------------------------------------------------
+
+__global__ __launch_bounds__(64) void k_mm_reduce(MmTab a, const double *__restrict__ X, int ldx,
+                                                  double alpha, int trans, int raw,
+                                                  double *__restrict__ part,
+                                                  double *__restrict__ w, size_t wstride)
+{
+    const int lane = threadIdx.x, m = lane & 15, g = lane >> 4;
+    const int task = blockIdx.x;
+    int d = 0;
+    while (task >= a.task_off[d + 1]) ++d;
+    const int loc = task - a.task_off[d];
+    const int nkg = a.nkg[d], ncg = a.ncg, tps = a.tps[d];
+    const int kg = loc % nkg;
+    const int cg = (loc / nkg) % ncg;
+    const int st = loc / (nkg * ncg); // segment * tps + tile
+    const int s = st / tps, tile = st - s * tps;
+    const int R = a.R[d], N = a.N, seg = a.seglen[d], T = a.T[d];
+    const int k0 = kg * MM_KG, c0 = cg * MM_CG;
+    const int rbeg = s * seg + tile * T;
+    const int rend = min(rbeg + T, (s + 1) * seg);
+
+    // per-lane column pointers (zero operand for columns past the end)
+    const double *pa[2];
+    const double *pb[4];
+    bool va[2], vb[4];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) {
+        const int k = k0 + kt * 16 + m;
+        va[kt] = k < R;
+        pa[kt] = a.P[d] + (size_t)(va[kt] ? k : 0) * N;
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int c = c0 + ct * 16 + m;
+        vb[ct] = c < a.ncols;
+        pb[ct] = X + (size_t)(vb[ct] ? c : 0) * ldx;
+    }
+    d4 acc[2][4];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+
+    for (int rb = rbeg; rb < rend; rb += 64) {
+        const int r = rb + 16 * g;
+#pragma unroll
+        for (int t0 = 0; t0 < 16; t0 += 4) {
+            double av[2][4], bv[4][4];
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt) {
+                const double2 *q = (const double2 *)(pa[kt] + r + t0);
+                double2 u0 = va[kt] ? q[0] : double2{0.0, 0.0};
+                double2 u1 = va[kt] ? q[1] : double2{0.0, 0.0};
+                av[kt][0] = u0.x; av[kt][1] = u0.y; av[kt][2] = u1.x; av[kt][3] = u1.y;
+            }
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const double2 *q = (const double2 *)(pb[ct] + r + t0);
+                double2 u0 = vb[ct] ? q[0] : double2{0.0, 0.0};
+                double2 u1 = vb[ct] ? q[1] : double2{0.0, 0.0};
+                bv[ct][0] = u0.x; bv[ct][1] = u0.y; bv[ct][2] = u1.x; bv[ct][3] = u1.y;
+            }
+#pragma unroll
+            for (int tt = 0; tt < 4; ++tt)
+#pragma unroll
+                for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct)
+                        acc[kt][ct] = mfma64(av[kt][tt], bv[ct][tt], acc[kt][ct]);
+        }
+    }
+
+    // lane holds T[k0 + 16kt + g + 4v][c0 + 16ct + m]
+    if (tps == 1) {
+        const int sp = raw ? s : (s ^ 1);
+        const int sidx = trans ? s : (s ^ 1);
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const int c = c0 + ct * 16 + m;
+                if (c >= a.ncols) continue;
+                double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = k0 + kt * 16 + g + 4 * v;
+                    if (k < R)
+                        wc[k] = raw ? acc[kt][ct][v] : alpha * a.S[d][(size_t)sidx * R + k] * acc[kt][ct][v];
+                }
+            }
+        return;
+    }
+    // partial block: [c_local][k_local], c_local < 64, k_local < 32
+    double *pp = part + ((size_t)a.part_off[d] + ((size_t)st * nkg + kg) * ncg + cg) * MM_BLK;
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                pp[(ct * 16 + m) * MM_KG + kt * 16 + g + 4 * v] = acc[kt][ct][v];
+}
+
+// Sum the partial blocks of every segment (fixed tile order) into the
+// coefficients.  One work-group per (level, segment, rank group, column group).
+__global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int trans, int raw,
+                                                   const double *__restrict__ part,
+                                                   double *__restrict__ w, size_t wstride)
+{
+    const int task = blockIdx.x;
+    int d = 0;
+    while (task >= a.fin_off[d + 1]) ++d;
+    const int loc = task - a.fin_off[d];
+    const int nkg = a.nkg[d], ncg = a.ncg, tps = a.tps[d], R = a.R[d];
+    const int kg = loc % nkg, cg = (loc / nkg) % ncg, s = loc / (nkg * ncg);
+    const int sp = raw ? s : (s ^ 1);
+    const int sidx = trans ? s : (s ^ 1);
+    for (int e = threadIdx.x; e < MM_BLK; e += blockDim.x) {
+        const int kl = e % MM_KG, cl = e / MM_KG;
+        const int k = kg * MM_KG + kl, c = cg * MM_CG + cl;
+        if (k >= R || c >= a.ncols) continue;
+        double t = 0.0;
+        for (int j = 0; j < tps; ++j) {
+            const size_t st = (size_t)s * tps + j;
+            t += part[((size_t)a.part_off[d] + (st * nkg + kg) * ncg + cg) * MM_BLK + e];
+        }
+        double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+        wc[k] = raw ? t : alpha * a.S[d][(size_t)sidx * R + k] * t;
+    }
+}
+
+// Y^T (64 columns x 16 rows per wavefront) += W^T Q^T over all levels, plus
+// the leaf term; four wavefronts per work-group cover 64 rows.
+__global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__restrict__ D,
+                                                   const double *__restrict__ X, int ldx,
+                                                   double alpha, double beta, int trans,
+                                                   const double *__restrict__ w, size_t wstride,
+                                                   double *__restrict__ Y, int ldy)
+{
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const int wv = threadIdx.x >> 6;
+    const int r0 = blockIdx.x * 64 + wv * 16;
+    const int c0 = blockIdx.y * MM_CG;
+    const int N = a.N, b = a.b;
+    if (r0 >= N) return;
+    // A operand (16 columns c of W^T or X^T, 4 k): lane holds [c0 + 16ct + m][k + g]
+    // B operand (4 k x 16 rows of Q^T or D^T):     lane holds [k + g][r0 + m]
+    bool vc[4];
+    int cc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        cc[ct] = c0 + ct * 16 + m;
+        vc[ct] = cc[ct] < a.ncols;
+        if (!vc[ct]) cc[ct] = 0;
+    }
+    d4 acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
+
+    if (D) {
+        const int leaf = r0 / b, il = r0 - leaf * b;
+        const double *Dl = D + (size_t)leaf * b * b;
+        const double *xl = X + (size_t)leaf * b;
+        for (int j = 0; j < b; j += 4) {
+            // B[k][n] = D^T[j+k][il+n] = D[il+n][j+k]  ('N'),  D[j+k][il+n] ('T')
+            const double bq = trans ? Dl[(size_t)(il + m) * b + j + g] : Dl[(size_t)(j + g) * b + il + m];
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const double av = vc[ct] ? xl[(size_t)cc[ct] * ldx + j + g] : 0.0;
+                acc[ct] = mfma64(av, bq, acc[ct]);
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] *= alpha;
+    }
+    for (int d = 0; d < a.L; ++d) {
+        const int R = a.R[d];
+        const int s = r0 / a.seglen[d];
+        const double *Qd = a.Q[d];
+        const double *wd = w + a.w_off[d] + (size_t)s * R;
+        for (int k = 0; k < R; k += 4) {
+            const double bq = Qd[(size_t)(k + g) * N + r0 + m];
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const double av = vc[ct] ? wd[(size_t)cc[ct] * wstride + k + g] : 0.0;
+                acc[ct] = mfma64(av, bq, acc[ct]);
+            }
+        }
+    }
+    // lane holds Y^T[c0 + 16ct + g + 4v][r0 + m]
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = c0 + ct * 16 + g + 4 * v;
+            if (c >= a.ncols) continue;
+            double *y = Y + (size_t)c * ldy + r0 + m;
+            *y = (beta == 0.0 ? 0.0 : beta * *y) + acc[ct][v];
+        }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+// Table for levels [d0, d1) with operands P (reduced), Q (expanded) and S.
+static void mm_tab(const dplrh_plan *p, int d0, int d1, const double *const *P,
+                   const double *const *Q, const double *const *S, int ncols, int w_local,
+                   MmTab *t)
+{
+    memset(t, 0, sizeof(*t));
+    t->L = d1 - d0;
+    t->N = p->N;
+    t->b = p->b;
+    t->ncols = ncols;
+    t->ncg = (ncols + MM_CG - 1) / MM_CG;
+    int task = 0, part = 0, fin = 0, woff = 0;
+    for (int i = 0; i < t->L; ++i) {
+        const int d = d0 + i;
+        const int seg = p->N >> (d + 1), nseg = 2 << d, R = p->R[d];
+        const int T = seg < MM_TMAX ? seg : MM_TMAX;
+        const int tps = seg / T;
+        const int nkg = (R + MM_KG - 1) / MM_KG;
+        t->R[i] = R;
+        t->seglen[i] = seg;
+        t->T[i] = T;
+        t->tps[i] = tps;
+        t->nkg[i] = nkg;
+        t->task_off[i] = task;
+        task += nkg * t->ncg * nseg * tps;
+        t->part_off[i] = part;
+        if (tps > 1) part += nkg * t->ncg * nseg * tps;
+        t->fin_off[i] = fin;
+        if (tps > 1) fin += nkg * t->ncg * nseg;
+        t->w_off[i] = w_local ? woff : (int)p->soff[d];
+        woff += nseg * R;
+        t->P[i] = P[i];
+        t->Q[i] = Q[i];
+        t->S[i] = S[i];
+    }
+    t->task_off[t->L] = task;
+    t->part_off[t->L] = part;
+    t->fin_off[t->L] = fin;
+    t->w_off[t->L] = w_local ? woff : (int)p->soff[d1];
+}
+
+// MFMA path needs every segment to be a whole number of 64-row blocks.
+int plr_mm_ok(const dplrh_plan *p) { return p->b % 64 == 0 && p->N >= 128; }
+
+static int mm_part(dplrh_plan *p, size_t blocks)
+{
+    const size_t need = blocks * MM_BLK;
+    if (need <= p->tmp_cap) return 0;
+    if (p->tmp) (void)hipFree(p->tmp);
+    p->tmp = nullptr;
+    p->tmp_cap = 0;
+    PLR_HIP(hipMalloc(&p->tmp, need * sizeof(double)));
+    p->tmp_cap = need;
+    return 0;
+}
+
+// Reduce-finish-expand over all levels for ncols columns (the multi-RHS
+// product): Y := alpha*op(A)*X + beta*Y.  w must hold soff[L] doubles per
+// column (stride wstride).
+int plr_mm_apply(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
+                 const double *X, int ldx, double *Y, int ldy, double *w, size_t wstride,
+                 hipStream_t st)
+{
+    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+    for (int d = 0; d < p->L; ++d) {
+        const double *Ud = p->U + p->uoff[d], *Vd = p->V + p->uoff[d];
+        P[d] = trans ? Ud : Vd;
+        Q[d] = trans ? Vd : Ud;
+        S[d] = p->S + p->soff[d];
+    }
+    MmTab t;
+    mm_tab(p, 0, p->L, P, Q, S, ncols, 0, &t);
+    int rc = mm_part(p, (size_t)t.part_off[t.L]);
+    if (rc) return rc;
+    if (t.task_off[t.L] > 0) {
+        hipLaunchKernelGGL(k_mm_reduce, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx, alpha,
+                           trans, 0, p->tmp, w, wstride);
+        PLR_HIP(hipGetLastError());
+    }
+    if (t.fin_off[t.L] > 0) {
+        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[t.L]), dim3(256), 0, st, t, alpha, trans, 0,
+                           p->tmp, w, wstride);
+        PLR_HIP(hipGetLastError());
+    }
+    dim3 grid((p->N + 63) / 64, t.ncg);
+    hipLaunchKernelGGL(k_mm_expand, grid, dim3(256), 0, st, t, p->D, X, ldx, alpha, beta, trans, w,
+                       wstride, Y, ldy);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+// Raw reduction T = P_d^T X per segment of level d (T layout: column c at
+// T + c*tstride, segment s at s*R).
+int plr_mm_reduce1(dplrh_plan *p, int d, const double *P, const double *X, int ldx, int ncols,
+                   double *T, size_t tstride, hipStream_t st)
+{
+    const double *Q = nullptr, *S = nullptr;
+    MmTab t;
+    mm_tab(p, d, d + 1, &P, &Q, &S, ncols, 1, &t);
+    int rc = mm_part(p, (size_t)t.part_off[1]);
+    if (rc) return rc;
+    if (t.task_off[1] > 0) {
+        hipLaunchKernelGGL(k_mm_reduce, dim3(t.task_off[1]), dim3(64), 0, st, t, X, ldx, 1.0, 0, 1,
+                           p->tmp, T, tstride);
+        PLR_HIP(hipGetLastError());
+    }
+    if (t.fin_off[1] > 0) {
+        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[1]), dim3(256), 0, st, t, 1.0, 0, 1, p->tmp,
+                           T, tstride);
+        PLR_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// Y += Q_d W per segment of level d (no leaf term), or, with d < 0, the leaf
+// term alone: Y := alpha*op(D) X + beta*Y with D given.
+int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,
+                   const double *D, int trans, const double *X, int ldx, double alpha, double beta,
+                   double *Y, int ldy, int ncols, hipStream_t st)
+{
+    const double *P = nullptr, *S = nullptr;
+    MmTab t;
+    if (d >= 0)
+        mm_tab(p, d, d + 1, &P, &Q, &S, ncols, 1, &t);
+    else
+        mm_tab(p, 0, 0, &P, &Q, &S, ncols, 1, &t);
+    dim3 grid((p->N + 63) / 64, t.ncg);
+    hipLaunchKernelGGL(k_mm_expand, grid, dim3(256), 0, st, t, D, X, ldx, alpha, beta, trans, W,
+                       wstride, Y, ldy);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}

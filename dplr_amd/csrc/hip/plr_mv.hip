@@ -28,7 +28,7 @@ __device__ __forceinline__ double wave_sum(double v)
 }
 
 __global__ __launch_bounds__(256) void k_mv_reduce(LevelTab a, const double *__restrict__ x,
-                                                   int ldx, double alpha, int trans,
+                                                   int ldx, double alpha, int trans, int raw,
                                                    double *__restrict__ part, size_t part_stride,
                                                    int *__restrict__ cnt, size_t cnt_stride,
                                                    double *__restrict__ w, size_t w_stride)
@@ -75,13 +75,15 @@ __global__ __launch_bounds__(256) void k_mv_reduce(LevelTab a, const double *__r
     for (int kk = 0; kk < KG; ++kk)
         if (lane == kk) mine = acc[kk];
 
-    const int sp = s ^ 1;                     // segment the coefficients act on
-    const int sidx = trans ? s : sp;          // whose sig scales them
-    const double *S = a.S[d] + (size_t)sidx * R;
+    // raw: t_s itself, stored at segment s.  Otherwise the coefficients
+    // alpha*sig.*t_s that act on the partner segment s^1.
+    const int sp = raw ? s : (s ^ 1);
+    const int sidx = trans ? s : (s ^ 1); // whose sig scales them
+    const double *S = raw ? nullptr : a.S[d] + (size_t)sidx * R;
     double *wd = w + a.w_off[d] + (size_t)sp * R;
 
     if (tps == 1) {
-        if (lane < kn) wd[k0 + lane] = alpha * S[k0 + lane] * mine;
+        if (lane < kn) wd[k0 + lane] = raw ? mine : alpha * S[k0 + lane] * mine;
         return;
     }
     double *pd = part + a.part_off[d];
@@ -98,7 +100,7 @@ __global__ __launch_bounds__(256) void k_mv_reduce(LevelTab a, const double *__r
         double t = 0.0;
         const double *q = pd + (size_t)s * tps * R + k0 + lane;
         for (int j = 0; j < tps; ++j) t += __builtin_nontemporal_load(q + (size_t)j * R);
-        wd[k0 + lane] = alpha * S[k0 + lane] * t;
+        wd[k0 + lane] = raw ? t : alpha * S[k0 + lane] * t;
     }
     if (lane == 0) atomicExch(c, 0);
 }
@@ -121,16 +123,18 @@ __global__ __launch_bounds__(256) void k_mv_expand(LevelTab a, const double *__r
     const int i = row0 + lane;
     double acc = 0.0;
     if (i < N) {
-        const int leaf = i / b, il = i - leaf * b;
-        const double *__restrict__ Dl = D + (size_t)leaf * b * b;
-        const double *__restrict__ xl = x + (size_t)leaf * b;
-        double sl = 0.0;
-        if (!trans) {
-            for (int j = g; j < b; j += 4) sl += Dl[(size_t)j * b + il] * xl[j];
-        } else {
-            for (int j = g; j < b; j += 4) sl += Dl[(size_t)il * b + j] * xl[j];
+        if (D) { // leaf term (skipped when the caller passes no leaves)
+            const int leaf = i / b, il = i - leaf * b;
+            const double *__restrict__ Dl = D + (size_t)leaf * b * b;
+            const double *__restrict__ xl = x + (size_t)leaf * b;
+            double sl = 0.0;
+            if (!trans) {
+                for (int j = g; j < b; j += 4) sl += Dl[(size_t)j * b + il] * xl[j];
+            } else {
+                for (int j = g; j < b; j += 4) sl += Dl[(size_t)il * b + j] * xl[j];
+            }
+            acc = alpha * sl;
         }
-        acc = alpha * sl;
         for (int d = 0; d < a.L; ++d) {
             const int R = a.R[d], seg = a.seglen[d];
             const double *__restrict__ Q = a.Q[d];
@@ -155,6 +159,31 @@ __global__ __launch_bounds__(256) void k_mv_expand(LevelTab a, const double *__r
     }
 }
 
+int plr_launch_reduce(const LevelTab &t, const double *x, int ldx, double alpha, int trans,
+                      int raw, int ncols, double *part, size_t pstride, int *cnt, size_t cstride,
+                      double *w, size_t wstride, hipStream_t st)
+{
+    const int ntask = t.task_off[t.L];
+    if (ntask <= 0 || ncols <= 0) return 0;
+    dim3 grid((ntask + 3) / 4, ncols);
+    hipLaunchKernelGGL(k_mv_reduce, grid, dim3(256), 0, st, t, x, ldx, alpha, trans, raw, part,
+                       pstride, cnt, cstride, w, wstride);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+int plr_launch_expand(const LevelTab &t, const double *D, const double *x, int ldx, double alpha,
+                      double beta, int trans, const double *w, size_t wstride, double *y, int ldy,
+                      int ncols, hipStream_t st)
+{
+    if (t.N <= 0 || ncols <= 0) return 0;
+    dim3 grid((t.N + 63) / 64, ncols);
+    hipLaunchKernelGGL(k_mv_expand, grid, dim3(256), 0, st, t, D, x, ldx, alpha, beta, trans, w,
+                       wstride, y, ldy);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
 // y := alpha*op(A)*x + beta*y for ncols columns (alpha != 0), in chunks of
 // at most kColChunk columns so the workspace and gridDim.y stay bounded.
 static constexpr int kColChunk = 64;
@@ -174,7 +203,7 @@ int plr_mv_launch(dplrh_plan *p, int trans, int ncols, double alpha, double beta
         double *y = dy + (size_t)c0 * ldy;
         if (ntask > 0) {
             dim3 grid((ntask + 3) / 4, nc);
-            hipLaunchKernelGGL(k_mv_reduce, grid, dim3(256), 0, st, t, x, ldx, alpha, trans,
+            hipLaunchKernelGGL(k_mv_reduce, grid, dim3(256), 0, st, t, x, ldx, alpha, trans, 0,
                                p->part, p->part_per_col, p->cnt, p->cnt_per_col, p->w,
                                p->w_per_col);
             PLR_HIP(hipGetLastError());
@@ -196,11 +225,25 @@ extern "C" int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
     return plr_mv_launch(p, trans, 1, alpha, beta, dx, p->N, dy, p->N, st);
 }
 
+static constexpr int kMmChunk = 128;
+
 extern "C" int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
                           const double *dB, int ldb, double *dC, int ldc, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (p->N == 0 || n == 0) return 0;
     if (alpha == 0.0) return dplrh_scale(p->N, n, beta, dC, ldc, stream);
+    if (n >= 2 && plr_mm_ok(p) && ldb % 2 == 0 && !p->force_valu) {
+        const int chunk = n < kMmChunk ? n : kMmChunk;
+        int rc = plr_workspace(p, chunk);
+        if (rc) return rc;
+        for (int c0 = 0; c0 < n; c0 += chunk) {
+            const int nc = n - c0 < chunk ? n - c0 : chunk;
+            rc = plr_mm_apply(p, trans, nc, alpha, beta, dB + (size_t)c0 * ldb, ldb,
+                              dC + (size_t)c0 * ldc, ldc, p->w, p->w_per_col, st);
+            if (rc) return rc;
+        }
+        return 0;
+    }
     return plr_mv_launch(p, trans, n, alpha, beta, dB, ldb, dC, ldc, st);
 }

@@ -58,6 +58,40 @@ void plr_level_tab(const dplrh_plan *p, int trans, LevelTab *t)
     t->w_off[p->L] = (int)p->soff[p->L];
 }
 
+void plr_level_tab1(const dplrh_plan *p, int d, const double *P, const double *Q,
+                    const double *S, LevelTab *t)
+{
+    LevelTab all;
+    plr_level_tab(p, 0, &all);
+    memset(t, 0, sizeof(*t));
+    t->L = 1;
+    t->N = p->N;
+    t->b = p->b;
+    t->R[0] = all.R[d];
+    t->seglen[0] = all.seglen[d];
+    t->T[0] = all.T[d];
+    t->tps[0] = all.tps[d];
+    t->nkg[0] = all.nkg[d];
+    t->task_off[0] = 0;
+    t->task_off[1] = all.task_off[d + 1] - all.task_off[d];
+    t->part_off[0] = 0;
+    t->cnt_off[0] = 0;
+    t->cnt_off[1] = all.cnt_off[d + 1] - all.cnt_off[d];
+    t->w_off[0] = 0;
+    t->w_off[1] = (2 << d) * all.R[d];
+    t->P[0] = P;
+    t->Q[0] = Q;
+    t->S[0] = S;
+}
+
+void plr_level_tab0(const dplrh_plan *p, LevelTab *t)
+{
+    memset(t, 0, sizeof(*t));
+    t->L = 0;
+    t->N = p->N;
+    t->b = p->b;
+}
+
 int plr_workspace(dplrh_plan *p, int ncols)
 {
     if (ncols <= p->wk_cols) return 0;
@@ -130,6 +164,8 @@ extern "C" void dplrh_plan_destroy(dplrh_plan *p)
     void *bufs[] = {p->U, p->V, p->S, p->D, p->rank, p->part, p->w, p->cnt, p->tmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    plr_free_fact(p);
+    plr_lp_release(p);
     delete p;
 }
 

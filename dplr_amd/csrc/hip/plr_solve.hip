@@ -1,0 +1,563 @@
+// Factored FP64 inverse of a PLR matrix and its application to right-hand
+// sides (the solve that replaces "dplrgeinv, then apply the explicit
+// inverse", reference src/dplrinv.c:93 with src/dplrpmv.c / dplrpdmm.c).
+//
+// The reference forms B = inv(A) explicitly, node by node in post-order,
+// using X = [inv(S1) V1^T u2; V2^T u1 inv(S2)] (src/dplrinv.c:283-315).  The
+// same Woodbury identity is used here in product form, level by level:
+//
+//   inv(A) = F_0 F_1 ... F_{L-1} inv(D_leaf),   F_d = I - Ut_d S_d inv(K_d) Z_d^T
+//
+// where Ut_d is U_d with every level below d already inverted, and per node
+// K = I + Z^T inv(D) W S = [I, M_bot S_bot; M_top S_top, I] with
+// M_s = V_d|_s^T Ut_d|_s.  K needs no 1/sig, so sig = 0 or tiny sig cannot
+// overflow it (the reference's X holds 1/sig).  Factorization, bottom-up:
+//   1. LU with partial pivoting of every leaf, then its inverse (Dinv);
+//   2. Ut = Dinv applied to the U columns of all levels;
+//   3. for d = L-1 .. 0: M = V_d^T [Ut_{<d} | Ut_d] per segment, build and
+//      invert K per node, then Ut_{<d} -= Ut_d S_d inv(K) Z_d^T Ut_{<d}.
+// Solve: z = Dinv b, then for d = L-1 .. 0: z -= Ut_d S_d inv(K) Z_d^T z.
+// Transpose solve: the factors in reverse order, transposed.
+#include <cstdio>
+#include <cstring>
+#include <vector>
+
+#include "plr_common.h"
+
+// ---------------------------------------------------------------------------
+// batched small dense kernels
+
+// In-place LU with partial pivoting of n x n column-major matrices
+// A + k*stride, one work-group per matrix.  piv: 0-based pivot rows.
+// info[k] = first j (1-based) with an exactly zero pivot, else 0.
+__global__ __launch_bounds__(256) void k_lu_batched(double *A, int n, size_t stride, int *piv,
+                                                    int *info)
+{
+    __shared__ double sval[256];
+    __shared__ int sidx[256];
+    __shared__ int s_p, s_info;
+    const int tid = threadIdx.x;
+    double *a = A + (size_t)blockIdx.x * stride;
+    int *pv = piv + (size_t)blockIdx.x * n;
+    if (tid == 0) s_info = 0;
+    __syncthreads();
+    for (int j = 0; j < n; ++j) {
+        double best = -1.0;
+        int bi = n;
+        for (int i = j + tid; i < n; i += blockDim.x) {
+            const double v = fabs(a[(size_t)j * n + i]);
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        sval[tid] = best;
+        sidx[tid] = bi;
+        __syncthreads();
+        for (int off = 128; off > 0; off >>= 1) {
+            if (tid < off) {
+                const double v2 = sval[tid + off];
+                const int i2 = sidx[tid + off];
+                if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
+                    sval[tid] = v2;
+                    sidx[tid] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        if (tid == 0) {
+            int p = sidx[0];
+            if (p >= n) p = j; // column of NaNs: keep the diagonal
+            s_p = p;
+            pv[j] = p;
+            if (a[(size_t)j * n + p] == 0.0 && s_info == 0) s_info = j + 1;
+        }
+        __syncthreads();
+        const int p = s_p;
+        const double pval = a[(size_t)j * n + p];
+        if (pval != 0.0) {
+            if (p != j)
+                for (int c = tid; c < n; c += blockDim.x) {
+                    const double t = a[(size_t)c * n + j];
+                    a[(size_t)c * n + j] = a[(size_t)c * n + p];
+                    a[(size_t)c * n + p] = t;
+                }
+            __syncthreads();
+            const double inv = 1.0 / a[(size_t)j * n + j];
+            for (int i = j + 1 + tid; i < n; i += blockDim.x) a[(size_t)j * n + i] *= inv;
+            __syncthreads();
+            const int m = n - j - 1;
+            for (int e = tid; e < m * m; e += blockDim.x) {
+                const int ii = j + 1 + e % m, cc = j + 1 + e / m;
+                a[(size_t)cc * n + ii] -= a[(size_t)j * n + ii] * a[(size_t)cc * n + j];
+            }
+        }
+        __syncthreads();
+    }
+    if (tid == 0) info[blockIdx.x] = s_info;
+}
+
+// inv(A) from its LU factors, one work-group per matrix, one thread per
+// column of the identity: P e_c, then unit-lower and upper substitution.
+__global__ __launch_bounds__(256) void k_lu_inverse(const double *LU, const int *piv, int n,
+                                                    size_t stride, double *Inv, size_t istride)
+{
+    const double *a = LU + (size_t)blockIdx.x * stride;
+    const int *pv = piv + (size_t)blockIdx.x * n;
+    double *out = Inv + (size_t)blockIdx.x * istride;
+    for (int c = threadIdx.x; c < n; c += blockDim.x) {
+        double *x = out + (size_t)c * n;
+        for (int i = 0; i < n; ++i) x[i] = (i == c) ? 1.0 : 0.0;
+        for (int j = 0; j < n; ++j) {
+            const int p = pv[j];
+            if (p != j) {
+                const double t = x[j];
+                x[j] = x[p];
+                x[p] = t;
+            }
+        }
+        for (int i = 0; i < n; ++i) {
+            double s = x[i];
+            for (int k = 0; k < i; ++k) s -= a[(size_t)k * n + i] * x[k];
+            x[i] = s;
+        }
+        for (int i = n - 1; i >= 0; --i) {
+            double s = x[i];
+            for (int k = i + 1; k < n; ++k) s -= a[(size_t)k * n + i] * x[k];
+            x[i] = s / a[(size_t)i * n + i];
+        }
+    }
+}
+
+// Build K = [I, M_bot S_bot; M_top S_top, I] for every node of a level.
+// M_s[k][c] = T[(off + c)*tstride + s*R + k] (output of the raw reduction
+// over the level's own Ut columns).
+__global__ void k_capf_build(const double *T, size_t tstride, int off, int R,
+                             const double *S, double *K)
+{
+    const int node = blockIdx.x, n2 = 2 * R;
+    double *k = K + (size_t)node * n2 * n2;
+    const int top = 2 * node, bot = 2 * node + 1;
+    for (int e = threadIdx.x; e < n2 * n2; e += blockDim.x) {
+        const int i = e % n2, j = e / n2;
+        double v = (i == j) ? 1.0 : 0.0;
+        if (i < R && j >= R)
+            v += T[(size_t)(off + j - R) * tstride + (size_t)bot * R + i] * S[(size_t)bot * R + j - R];
+        else if (i >= R && j < R)
+            v += T[(size_t)(off + j) * tstride + (size_t)top * R + i - R] * S[(size_t)top * R + j];
+        k[e] = v;
+    }
+}
+
+// Capacitance step of the solve for one level: per node and column,
+// rhs from the raw reduction T, c = inv(K) rhs (or inv(K)^T rhs), then the
+// negated expansion coefficients W (layout of T).
+//   trans == 0: rhs = [t_bot; t_top],           W_top = -S_top.*c1, W_bot = -S_bot.*c2
+//   trans == 1: rhs = [S_top.*t_top; S_bot.*t_bot], W_top = -c2,     W_bot = -c1
+__global__ __launch_bounds__(256) void k_cap_apply(const double *Kinv, int R, const double *S,
+                                                   const double *T, size_t tstride, double *W,
+                                                   size_t wstride, int trans)
+{
+    extern __shared__ double rhs[];
+    const int node = blockIdx.x, col = blockIdx.y, n2 = 2 * R;
+    const int top = 2 * node, bot = 2 * node + 1;
+    const double *t = T + (size_t)col * tstride;
+    double *w = W + (size_t)col * wstride;
+    const double *k = Kinv + (size_t)node * n2 * n2;
+    for (int j = threadIdx.x; j < n2; j += blockDim.x) {
+        double v;
+        if (!trans)
+            v = (j < R) ? t[(size_t)bot * R + j] : t[(size_t)top * R + j - R];
+        else
+            v = (j < R) ? S[(size_t)top * R + j] * t[(size_t)top * R + j]
+                        : S[(size_t)bot * R + j - R] * t[(size_t)bot * R + j - R];
+        rhs[j] = v;
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+        double acc = 0.0;
+        if (!trans)
+            for (int j = 0; j < n2; ++j) acc += k[(size_t)j * n2 + i] * rhs[j];
+        else
+            for (int j = 0; j < n2; ++j) acc += k[(size_t)i * n2 + j] * rhs[j];
+        if (!trans) {
+            if (i < R)
+                w[(size_t)top * R + i] = -S[(size_t)top * R + i] * acc;
+            else
+                w[(size_t)bot * R + i - R] = -S[(size_t)bot * R + i - R] * acc;
+        } else {
+            if (i < R)
+                w[(size_t)bot * R + i] = -acc;
+            else
+                w[(size_t)top * R + i - R] = -acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+static size_t col_off(const dplrh_plan *p, int d) { return p->N ? p->uoff[d] / p->N : 0; }
+
+void plr_free_fact(dplrh_plan *p)
+{
+    void *bufs[] = {p->f.Dinv, p->f.Ut, p->f.Kinv, p->f.dinfo, p->f.work,
+                    p->sw.T, p->sw.W, p->sw.part, p->sw.tmp, p->sw.cnt};
+    for (void *b : bufs)
+        if (b) (void)hipFree(b);
+    memset(&p->f, 0, sizeof(p->f));
+    memset(&p->sw, 0, sizeof(p->sw));
+}
+
+// Per-column sizes of the single-level reduction workspace (max over levels).
+static void solve_ws_sizes(const dplrh_plan *p, size_t *t, size_t *part, size_t *cnt)
+{
+    LevelTab all;
+    plr_level_tab(p, 0, &all);
+    *t = *part = *cnt = 1;
+    for (int d = 0; d < p->L; ++d) {
+        const size_t nseg = (size_t)2 << d;
+        if (nseg * all.R[d] > *t) *t = nseg * all.R[d];
+        if (all.tps[d] > 1) {
+            if (nseg * all.tps[d] * all.R[d] > *part) *part = nseg * all.tps[d] * all.R[d];
+            if (nseg * all.nkg[d] > *cnt) *cnt = nseg * all.nkg[d];
+        }
+    }
+}
+
+static int solve_ws(dplrh_plan *p, int cols)
+{
+    plr_solve_ws &w = p->sw;
+    if (cols <= w.cols) return 0;
+    void *old[] = {w.T, w.W, w.part, w.cnt};
+    for (void *b : old)
+        if (b) (void)hipFree(b);
+    w.T = w.W = w.part = nullptr;
+    w.cnt = nullptr;
+    w.cols = 0;
+    solve_ws_sizes(p, &w.t_per_col, &w.part_per_col, &w.cnt_per_col);
+    PLR_HIP(hipMalloc(&w.T, w.t_per_col * cols * sizeof(double)));
+    PLR_HIP(hipMalloc(&w.W, w.t_per_col * cols * sizeof(double)));
+    PLR_HIP(hipMalloc(&w.part, w.part_per_col * cols * sizeof(double)));
+    PLR_HIP(hipMalloc(&w.cnt, w.cnt_per_col * cols * sizeof(int)));
+    PLR_HIP(hipMemset(w.cnt, 0, w.cnt_per_col * cols * sizeof(int)));
+    PLR_HIP(hipDeviceSynchronize());
+    w.cols = cols;
+    return 0;
+}
+
+static int solve_tmp(dplrh_plan *p, size_t elems)
+{
+    if (elems <= p->sw.tmp_cap) return 0;
+    if (p->sw.tmp) (void)hipFree(p->sw.tmp);
+    p->sw.tmp = nullptr;
+    p->sw.tmp_cap = 0;
+    PLR_HIP(hipMalloc(&p->sw.tmp, elems * sizeof(double)));
+    p->sw.tmp_cap = elems;
+    return 0;
+}
+
+// Post-order position of a failure: a node is visited after its last leaf
+// and after every deeper node that ends at the same leaf.
+struct fail_key
+{
+    long last_leaf;
+    int height;
+    int info;
+};
+
+static bool key_less(const fail_key &a, const fail_key &b)
+{
+    return a.last_leaf != b.last_leaf ? a.last_leaf < b.last_leaf : a.height < b.height;
+}
+
+static int read_info(int count, const int *dinfo, std::vector<int> &h, hipStream_t st)
+{
+    h.resize(count > 0 ? count : 1);
+    PLR_HIP(hipStreamSynchronize(st));
+    if (count > 0) PLR_HIP(hipMemcpy(h.data(), dinfo, count * sizeof(int), hipMemcpyDeviceToHost));
+    return 0;
+}
+
+// FP64 factorization; *info as dgetrf reports it for the first singular
+// matrix in the reference's post-order (0 if none).
+extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    *info = 0;
+    plr_fact &f = p->f;
+    const int N = p->N, L = p->L, b = p->b;
+    const int nleaf = b ? N / b : 0;
+    if (N == 0) {
+        f.done = 1;
+        return 0;
+    }
+    if (!f.Dinv) {
+        size_t kk = 0, work = (size_t)N * b, piv = (size_t)nleaf * b;
+        int maxnodes = nleaf;
+        for (int d = 0; d < L; ++d) {
+            f.koff[d] = kk;
+            const size_t n2 = 2 * (size_t)p->R[d];
+            kk += ((size_t)1 << d) * n2 * n2;
+            if (((size_t)1 << d) * n2 * n2 > work) work = ((size_t)1 << d) * n2 * n2;
+            if (((size_t)1 << d) * n2 > piv) piv = ((size_t)1 << d) * n2;
+            if ((1 << d) > maxnodes) maxnodes = 1 << d;
+        }
+        f.koff[L] = kk;
+        PLR_HIP(hipMalloc(&f.Dinv, (size_t)N * b * sizeof(double)));
+        PLR_HIP(hipMalloc(&f.Ut, (p->uoff[L] ? p->uoff[L] : 1) * sizeof(double)));
+        PLR_HIP(hipMalloc(&f.Kinv, (kk ? kk : 1) * sizeof(double)));
+        PLR_HIP(hipMalloc(&f.work, (work + piv / 2 + 1) * sizeof(double) + piv * sizeof(int)));
+        PLR_HIP(hipMalloc(&f.dinfo, (size_t)maxnodes * sizeof(int)));
+    }
+    f.done = 0;
+    double *work = f.work;
+    // pivots live behind the largest LU scratch
+    size_t worksz = (size_t)N * b;
+    for (int d = 0; d < L; ++d) {
+        const size_t n2 = 2 * (size_t)p->R[d];
+        if (((size_t)1 << d) * n2 * n2 > worksz) worksz = ((size_t)1 << d) * n2 * n2;
+    }
+    int *piv = (int *)(work + worksz);
+
+    std::vector<fail_key> fails;
+    std::vector<int> h;
+
+    // 1. leaves
+    PLR_HIP(hipMemcpyAsync(work, p->D, (size_t)N * b * sizeof(double), hipMemcpyDeviceToDevice, st));
+    hipLaunchKernelGGL(k_lu_batched, dim3(nleaf), dim3(256), 0, st, work, b, (size_t)b * b, piv,
+                       f.dinfo);
+    PLR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_lu_inverse, dim3(nleaf), dim3(b < 256 ? ((b + 63) & ~63) : 256), 0, st,
+                       work, piv, b, (size_t)b * b, f.Dinv, (size_t)b * b);
+    PLR_HIP(hipGetLastError());
+    if (read_info(nleaf, f.dinfo, h, st)) return -10;
+    for (int l = 0; l < nleaf; ++l)
+        if (h[l]) fails.push_back({l, 0, h[l]});
+
+    // 2. Ut = Dinv U (all levels' columns)
+    const int ctot = (int)col_off(p, L);
+    if (ctot > 0) {
+        LevelTab t0;
+        plr_level_tab0(p, &t0);
+        int rc;
+        if (plr_mm_ok(p) && !p->force_valu)
+            rc = plr_mm_expand1(p, -1, nullptr, nullptr, 0, f.Dinv, 0, p->U, N, 1.0, 0.0, f.Ut, N,
+                                ctot, st);
+        else
+            rc = plr_launch_expand(t0, f.Dinv, p->U, N, 1.0, 0.0, 0, nullptr, 0, f.Ut, N, ctot, st);
+        if (rc) return rc;
+    }
+
+    // 3. levels, bottom-up
+    for (int d = L - 1; d >= 0; --d) {
+        const int R = p->R[d];
+        if (R == 0) continue;
+        const int off = (int)col_off(p, d);
+        const int ncol = off + R;
+        int rc = solve_ws(p, ncol);
+        if (rc) return rc;
+        const double *Ud = f.Ut + p->uoff[d], *Vd = p->V + p->uoff[d], *Sd = p->S + p->soff[d];
+        LevelTab t1;
+        plr_level_tab1(p, d, Vd, Ud, Sd, &t1);
+        const bool mm = plr_mm_ok(p) && !p->force_valu;
+        if (mm)
+            rc = plr_mm_reduce1(p, d, Vd, f.Ut, N, ncol, p->sw.T, p->sw.t_per_col, st);
+        else
+            rc = plr_launch_reduce(t1, f.Ut, N, 1.0, 0, 1, ncol, p->sw.part, p->sw.part_per_col,
+                                   p->sw.cnt, p->sw.cnt_per_col, p->sw.T, p->sw.t_per_col, st);
+        if (rc) return rc;
+        const int nodes = 1 << d, n2 = 2 * R;
+        hipLaunchKernelGGL(k_capf_build, dim3(nodes), dim3(256), 0, st, p->sw.T, p->sw.t_per_col,
+                           off, R, Sd, work);
+        PLR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lu_batched, dim3(nodes), dim3(256), 0, st, work, n2,
+                           (size_t)n2 * n2, piv, f.dinfo);
+        PLR_HIP(hipGetLastError());
+        hipLaunchKernelGGL(k_lu_inverse, dim3(nodes), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256), 0,
+                           st, work, piv, n2, (size_t)n2 * n2, f.Kinv + f.koff[d], (size_t)n2 * n2);
+        PLR_HIP(hipGetLastError());
+        if (read_info(nodes, f.dinfo, h, st)) return -10;
+        for (int n = 0; n < nodes; ++n)
+            if (h[n]) fails.push_back({((long)(n + 1) << (L - d)) - 1, L - d, h[n]});
+        if (off > 0) {
+            hipLaunchKernelGGL(k_cap_apply, dim3(nodes, off), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256),
+                               n2 * sizeof(double), st, f.Kinv + f.koff[d], R, Sd, p->sw.T,
+                               p->sw.t_per_col, p->sw.W, p->sw.t_per_col, 0);
+            PLR_HIP(hipGetLastError());
+            if (mm)
+                rc = plr_mm_expand1(p, d, Ud, p->sw.W, p->sw.t_per_col, nullptr, 0, nullptr, N,
+                                    1.0, 1.0, f.Ut, N, off, st);
+            else
+                rc = plr_launch_expand(t1, nullptr, nullptr, N, 1.0, 1.0, 0, p->sw.W,
+                                       p->sw.t_per_col, f.Ut, N, off, st);
+            if (rc) return rc;
+        }
+    }
+    PLR_HIP(hipStreamSynchronize(st));
+    if (!fails.empty()) {
+        fail_key best = fails[0];
+        for (const fail_key &k : fails)
+            if (key_less(k, best)) best = k;
+        *info = best.info;
+        f.info = best.info;
+        return 0;
+    }
+    f.info = 0;
+    f.done = 1;
+    return 0;
+}
+
+// X := op(A)^{-1} B for n device columns (ldb, ldx >= N), after dplrh_factor.
+extern "C" int dplrh_solve(dplrh_plan *p, int trans, int n, const double *dB, int ldb,
+                           double *dX, int ldx, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (!p->f.done) {
+        plr_set_error("solve before a successful factorization");
+        return -4;
+    }
+    const int N = p->N, L = p->L;
+    if (N == 0 || n == 0) return 0;
+    const int chunk = n < 1024 ? n : 1024;
+    int rc = solve_ws(p, chunk);
+    if (rc) return rc;
+    LevelTab t0;
+    plr_level_tab0(p, &t0);
+    const bool mm = plr_mm_ok(p) && !p->force_valu && ldb % 2 == 0 && ldx % 2 == 0 && n >= 2;
+    for (int c0 = 0; c0 < n; c0 += chunk) {
+        const int nc = n - c0 < chunk ? n - c0 : chunk;
+        const double *B = dB + (size_t)c0 * ldb;
+        double *X = dX + (size_t)c0 * ldx;
+        double *Z = X;
+        int ldz = ldx;
+        if (!trans) {
+            if (mm)
+                rc = plr_mm_expand1(p, -1, nullptr, nullptr, 0, p->f.Dinv, 0, B, ldb, 1.0, 0.0, X,
+                                    ldx, nc, st);
+            else
+                rc = plr_launch_expand(t0, p->f.Dinv, B, ldb, 1.0, 0.0, 0, nullptr, 0, X, ldx, nc,
+                                       st);
+            if (rc) return rc;
+        } else {
+            if ((rc = solve_tmp(p, (size_t)N * nc))) return rc;
+            Z = p->sw.tmp;
+            ldz = N;
+            PLR_HIP(hipMemcpy2DAsync(Z, (size_t)N * sizeof(double), B, (size_t)ldb * sizeof(double),
+                                     (size_t)N * sizeof(double), nc, hipMemcpyDeviceToDevice, st));
+        }
+        for (int k = 0; k < L; ++k) {
+            const int d = trans ? k : L - 1 - k;
+            const int R = p->R[d];
+            if (R == 0) continue;
+            const double *Ud = p->f.Ut + p->uoff[d], *Vd = p->V + p->uoff[d];
+            const double *Sd = p->S + p->soff[d];
+            LevelTab t1;
+            plr_level_tab1(p, d, trans ? Ud : Vd, trans ? Vd : Ud, Sd, &t1);
+            if (mm)
+                rc = plr_mm_reduce1(p, d, trans ? Ud : Vd, Z, ldz, nc, p->sw.T, p->sw.t_per_col, st);
+            else
+                rc = plr_launch_reduce(t1, Z, ldz, 1.0, 0, 1, nc, p->sw.part, p->sw.part_per_col,
+                                       p->sw.cnt, p->sw.cnt_per_col, p->sw.T, p->sw.t_per_col, st);
+            if (rc) return rc;
+            const int n2 = 2 * R;
+            hipLaunchKernelGGL(k_cap_apply, dim3(1 << d, nc), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256),
+                               n2 * sizeof(double), st, p->f.Kinv + p->f.koff[d], R, Sd, p->sw.T,
+                               p->sw.t_per_col, p->sw.W, p->sw.t_per_col, trans);
+            PLR_HIP(hipGetLastError());
+            if (mm)
+                rc = plr_mm_expand1(p, d, trans ? Vd : Ud, p->sw.W, p->sw.t_per_col, nullptr, 0,
+                                    nullptr, ldz, 1.0, 1.0, Z, ldz, nc, st);
+            else
+                rc = plr_launch_expand(t1, nullptr, nullptr, ldz, 1.0, 1.0, 0, p->sw.W,
+                                       p->sw.t_per_col, Z, ldz, nc, st);
+            if (rc) return rc;
+        }
+        if (trans) {
+            if (mm)
+                rc = plr_mm_expand1(p, -1, nullptr, nullptr, 0, p->f.Dinv, 1, Z, ldz, 1.0, 0.0, X,
+                                    ldx, nc, st);
+            else
+                rc = plr_launch_expand(t0, p->f.Dinv, Z, ldz, 1.0, 0.0, 1, nullptr, 0, X, ldx, nc,
+                                       st);
+            if (rc) return rc;
+        }
+    }
+    return 0;
+}
+
+int dplrh_lp_lu_inverse(double *K, int n, int count, double *Kinv, int *info, hipStream_t st)
+{
+    *info = 0;
+    if (count <= 0 || n <= 0) return 0;
+    int *piv = nullptr, *dinfo = nullptr;
+    PLR_HIP(hipMalloc(&piv, (size_t)count * n * sizeof(int)));
+    PLR_HIP(hipMalloc(&dinfo, (size_t)count * sizeof(int)));
+    hipLaunchKernelGGL(k_lu_batched, dim3(count), dim3(256), 0, st, K, n, (size_t)n * n, piv, dinfo);
+    PLR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_lu_inverse, dim3(count), dim3(n < 256 ? ((n + 63) & ~63) : 256), 0, st, K,
+                       piv, n, (size_t)n * n, Kinv, (size_t)n * n);
+    PLR_HIP(hipGetLastError());
+    std::vector<int> h;
+    int rc = read_info(count, dinfo, h, st);
+    (void)hipFree(piv);
+    (void)hipFree(dinfo);
+    if (rc) return rc;
+    for (int i = 0; i < count; ++i)
+        if (h[i]) {
+            *info = h[i];
+            break;
+        }
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// Single-level building blocks with MFMA / general-shape dispatch.
+
+// T[c][s*R_d + k] = P_d|_s^T X|_s for ncols columns (raw, FP64).
+int plr_lvl_reduce(dplrh_plan *p, int d, const double *P, const double *X, int ldx, int ncols,
+                   double *T, size_t tstride, hipStream_t st)
+{
+    if (ncols <= 0 || p->R[d] == 0) return 0;
+    if (plr_mm_ok(p) && !p->force_valu && ldx % 2 == 0)
+        return plr_mm_reduce1(p, d, P, X, ldx, ncols, T, tstride, st);
+    int rc = solve_ws(p, ncols);
+    if (rc) return rc;
+    LevelTab t1;
+    plr_level_tab1(p, d, P, nullptr, nullptr, &t1);
+    return plr_launch_reduce(t1, X, ldx, 1.0, 0, 1, ncols, p->sw.part, p->sw.part_per_col,
+                             p->sw.cnt, p->sw.cnt_per_col, T, tstride, st);
+}
+
+// Y := beta*Y + Q_d W (per segment of level d), or with d < 0 the leaf term
+// Y := alpha*op(D) X + beta*Y.
+int plr_lvl_expand(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,
+                   const double *D, int trans, const double *X, int ldx, double alpha,
+                   double beta, double *Y, int ldy, int ncols, hipStream_t st)
+{
+    if (ncols <= 0) return 0;
+    if (plr_mm_ok(p) && !p->force_valu && (d >= 0 || ldx % 2 == 0))
+        return plr_mm_expand1(p, d, Q, W, wstride, D, trans, X, ldx, alpha, beta, Y, ldy, ncols,
+                              st);
+    LevelTab t;
+    if (d >= 0)
+        plr_level_tab1(p, d, nullptr, Q, nullptr, &t);
+    else
+        plr_level_tab0(p, &t);
+    return plr_launch_expand(t, D, X, ldx, alpha, beta, trans, W, wstride, Y, ldy, ncols, st);
+}
+
+// W = capacitance coefficients for one level (see k_cap_apply).
+int plr_lvl_cap(dplrh_plan *p, int d, const double *T, size_t tstride, double *W, int ncols,
+                int trans, hipStream_t st)
+{
+    const int R = p->R[d], n2 = 2 * R;
+    if (ncols <= 0 || R == 0) return 0;
+    hipLaunchKernelGGL(k_cap_apply, dim3(1 << d, ncols), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256),
+                       n2 * sizeof(double), st, p->f.Kinv + p->f.koff[d], R, p->S + p->soff[d], T,
+                       tstride, W, tstride, trans);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+int plr_solve_reserve(dplrh_plan *p, int cols) { return solve_ws(p, cols); }

@@ -193,4 +193,32 @@ int dplrx_gepdmm(dplrx_plan *plan, char transa, int n, double alpha, const doubl
     return dplrh_gemm(plan, !is_n(transa), n, alpha, beta, dB, ldb, dC, ldc, stream);
 }
 
+int dplrx_factor(dplrx_plan *plan, int *info, void *stream)
+{
+    return dplrh_factor(plan, info, stream);
+}
+
+int dplrx_solve(dplrx_plan *plan, char trans, int n, const double *dB, int ldb, double *dX,
+                int ldx, void *stream)
+{
+    int N, L, b, R[DPLRH_MAXL];
+    dplrh_plan_dims(plan, &N, &L, &b, R);
+    if (n < 0 || ldb < N || ldx < N || (n > 0 && dB == dX)) return -1;
+    return dplrh_solve(plan, !is_n(trans), n, dB, ldb, dX, ldx, stream);
+}
+
+int dplrx_factor_mixed(dplrx_plan *plan, int type, int *info, void *stream)
+{
+    return dplrh_lp_factor(plan, type, info, stream);
+}
+
+int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
+                        int ldx, int iters, double *res, void *stream)
+{
+    int N, L, b, R[DPLRH_MAXL];
+    dplrh_plan_dims(plan, &N, &L, &b, R);
+    if (n < 0 || iters < 1 || ldb < N || ldx < N || (n > 0 && dB == dX)) return -1;
+    return dplrh_lp_solve(plan, n, dB, ldb, dX, ldx, iters, res, stream);
+}
+
 const char *dplrx_last_error(void) { return dplrh_last_error(); }

@@ -1,0 +1,310 @@
+/*
+ * Drop-in dplrgeinv and dplrgeprk: argument checks, the structural reshaping
+ * of the output tree and the copy of GPU-computed values into it.
+ *
+ * The tree shape of B = inv(A) follows the reference's post-order walk
+ * (src/dplrinv.c:124-229 and the dplrgeprk appends at :428-429): a node's
+ * own Woodbury blocks get A's ranks, then every ancestor's correction
+ * appends k = (rank of the ancestor's other off-diagonal block) columns to
+ * every low-rank block of the subtree, growing capacity with dplrresizelr
+ * exactly when the reference does.  Column order per block is therefore
+ * [own | correction from the parent | from the grandparent | ...].
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "dplr.h"
+#include "../dplr_hip.h"
+
+typedef struct
+{
+    const plr_flat *f;
+    int Rt[DPLRH_MAXL];
+    int coff[DPLRH_MAXL][DPLRH_MAXL];
+    double *BU[DPLRH_MAXL], *BV[DPLRH_MAXL], *Bsig[DPLRH_MAXL], *BD;
+} inv_ctx;
+
+static int rank_at(const plr_flat *f, int d, int s) { return f->rank[(2 << d) - 2 + s]; }
+
+/* dplrgeprk's structural effect: k more columns in every block of the subtree */
+static void grow_subtree(DPlrMtx *X, int k)
+{
+    if (k == 0 || X->type != PLR_HMTX) return;
+    DPlrMtx *lr[2] = {X->dh.ne, X->dh.sw};
+    for (int i = 0; i < 2; ++i) {
+        if (lr[i]->dlr.capacity < lr[i]->dlr.rank + k) dplrresizelr(lr[i], lr[i]->dlr.rank + k);
+        lr[i]->dlr.rank += k;
+    }
+    grow_subtree(X->dh.nw, k);
+    grow_subtree(X->dh.se, k);
+}
+
+static void shape_inverse(const DPlrMtx *A, DPlrMtx *B, int m, int d)
+{
+    if (A->type == PLR_DMTX) {
+        if (B->type != PLR_DMTX || B->size != m) {
+            dplrclr(B);
+            dplrinid(m, B);
+        }
+        return;
+    }
+    if (B->type != PLR_HMTX || B->size != m) {
+        dplrclr(B);
+        dplrinih(m, B);
+    }
+    shape_inverse(A->dh.nw, B->dh.nw, m / 2, d + 1);
+    shape_inverse(A->dh.se, B->dh.se, m / 2, d + 1);
+    DPlrMtx *Bx[2] = {B->dh.ne, B->dh.sw};
+    const DPlrMtx *Ax[2] = {A->dh.ne, A->dh.sw};
+    for (int i = 0; i < 2; ++i) {
+        if (Bx[i]->type != PLR_LRMTX || Bx[i]->size != m / 2) {
+            dplrclr(Bx[i]);
+            dplrinilr(m / 2, Bx[i], Ax[i]->dlr.rank);
+        }
+        if (Bx[i]->dlr.capacity < Ax[i]->dlr.rank) dplrresizelr(Bx[i], Ax[i]->dlr.rank);
+    }
+    const int r1 = A->dh.ne->dlr.rank, r2 = A->dh.sw->dlr.rank;
+    if (r1 == 0 && r2 == 0) return;
+    B->dh.ne->dlr.rank = r1;
+    B->dh.sw->dlr.rank = r2;
+    grow_subtree(B->dh.nw, r2);
+    grow_subtree(B->dh.se, r1);
+}
+
+/* Copy the level-d' block whose U lives on segment q: own columns, then the
+ * corrections from levels d'-1 .. 0. */
+static void fill_block(DPlrMtx *X, const inv_ctx *c, int dp, int q)
+{
+    const plr_flat *f = c->f;
+    const int len = X->size, Rt = c->Rt[dp];
+    const double *bu = c->BU[dp] + (size_t)q * Rt * len;
+    const double *bv = c->BV[dp] + (size_t)q * Rt * len;
+    const double *bs = c->Bsig[dp] + (size_t)q * Rt;
+    int pos = 0;
+    const int rown = rank_at(f, dp, q);
+    for (int j = 0; j < rown; ++j, ++pos) {
+        memcpy(X->dlr.u + (size_t)pos * len, bu + (size_t)j * len, len * sizeof(double));
+        memcpy(X->dlr.v + (size_t)pos * len, bv + (size_t)j * len, len * sizeof(double));
+        X->dlr.sig[pos] = bs[j];
+    }
+    for (int d = dp - 1; d >= 0; --d) {
+        const int s = q >> (dp - d);
+        const int k = rank_at(f, d, s ^ 1);
+        for (int j = 0; j < k; ++j, ++pos) {
+            const int col = c->coff[dp][d] + j;
+            memcpy(X->dlr.u + (size_t)pos * len, bu + (size_t)col * len, len * sizeof(double));
+            memcpy(X->dlr.v + (size_t)pos * len, bv + (size_t)col * len, len * sizeof(double));
+            X->dlr.sig[pos] = bs[col];
+        }
+    }
+    if (pos != X->dlr.rank)
+        fprintf(stderr, "dplrgeinv: internal rank mismatch (%d vs %d)\n", pos, X->dlr.rank);
+}
+
+static void fill_inverse(DPlrMtx *B, const inv_ctx *c, int d, int n)
+{
+    if (B->type == PLR_DMTX) {
+        const size_t bb = (size_t)c->f->b * c->f->b;
+        if (bb) memcpy(B->dd.d, c->BD + (size_t)n * bb, bb * sizeof(double));
+        return;
+    }
+    fill_block(B->dh.ne, c, d, 2 * n);
+    fill_block(B->dh.sw, c, d, 2 * n + 1);
+    fill_inverse(B->dh.nw, c, d + 1, 2 * n);
+    fill_inverse(B->dh.se, c, d + 1, 2 * n + 1);
+}
+
+/* reference src/dplrinv.c:93 */
+void dplrgeinv(int m, DPlrMtx *A, DPlrMtx *B, double eps, int *info)
+{
+    if (m < 0) {
+        fprintf(stderr, "%s: invalid matrix size: %d\n", __func__, m);
+        return;
+    }
+    if (A->type != PLR_DMTX && A->type != PLR_HMTX) {
+        fprintf(stderr, "%s: invalid A type\n", __func__);
+        return;
+    }
+    if (A->size != m) {
+        fprintf(stderr, "%s: invalid A size\n", __func__);
+        return;
+    }
+    *info = -1;
+    plr_flat f;
+    if (plr_flat_shape(m, A, &f, 1) || plr_flat_fill(A, &f)) return;
+    dplrh_plan *p = NULL;
+    inv_ctx c;
+    memset(&c, 0, sizeof(c));
+    c.f = &f;
+    int rc = dplrh_plan_create(&f, &p), finfo = 0;
+    if (!rc && m > 0) rc = dplrh_factor(p, &finfo, NULL);
+    if (rc) {
+        fprintf(stderr, "%s: GPU execution failed (%d): %s\n", __func__, rc, dplrh_last_error());
+        goto out;
+    }
+    if (finfo) {
+        fprintf(stderr, "%s: failed to invert a dense matrix: %d\n", __func__, finfo);
+        *info = finfo;
+        goto out;
+    }
+    for (int dp = 0; dp < f.L; ++dp) {
+        int acc = f.R[dp];
+        for (int d = dp - 1; d >= 0; --d) {
+            c.coff[dp][d] = acc;
+            acc += f.R[d];
+        }
+        c.Rt[dp] = acc;
+        const size_t nseg = (size_t)2 << dp;
+        c.BU[dp] = malloc(((size_t)f.N * acc + 1) * sizeof(double));
+        c.BV[dp] = malloc(((size_t)f.N * acc + 1) * sizeof(double));
+        c.Bsig[dp] = malloc((nseg * acc + 1) * sizeof(double));
+        if (!c.BU[dp] || !c.BV[dp] || !c.Bsig[dp]) {
+            fprintf(stderr, "%s: host memory allocation failed\n", __func__);
+            goto out;
+        }
+    }
+    c.BD = malloc(((size_t)f.N * f.b + 1) * sizeof(double));
+    if (!c.BD) {
+        fprintf(stderr, "%s: host memory allocation failed\n", __func__);
+        goto out;
+    }
+    if (m > 0) {
+        rc = dplrh_geinv(p, eps, c.Rt, c.coff, c.BU, c.BV, c.Bsig, c.BD, NULL);
+        if (rc) {
+            fprintf(stderr, "%s: GPU execution failed (%d): %s\n", __func__, rc, dplrh_last_error());
+            goto out;
+        }
+    }
+    shape_inverse(A, B, m, 0);
+    if (m > 0) fill_inverse(B, &c, 0, 0);
+    *info = 0;
+out:
+    for (int d = 0; d < DPLRH_MAXL; ++d) {
+        free(c.BU[d]);
+        free(c.BV[d]);
+        free(c.Bsig[d]);
+    }
+    free(c.BD);
+    dplrh_plan_destroy(p);
+    plr_flat_free(&f);
+}
+
+/* ---- rank-k update ------------------------------------------------------ */
+
+static void gather_leaves(const DPlrMtx *A, int b, int n, double *D)
+{
+    if (A->type == PLR_DMTX) {
+        if (b) memcpy(D + (size_t)n * b * b, A->dd.d, (size_t)b * b * sizeof(double));
+        return;
+    }
+    gather_leaves(A->dh.nw, b, 2 * n, D);
+    gather_leaves(A->dh.se, b, 2 * n + 1, D);
+}
+
+static void scatter_leaves(DPlrMtx *A, int b, int n, const double *D)
+{
+    if (A->type == PLR_DMTX) {
+        if (b) memcpy(A->dd.d, D + (size_t)n * b * b, (size_t)b * b * sizeof(double));
+        return;
+    }
+    scatter_leaves(A->dh.nw, b, 2 * n, D);
+    scatter_leaves(A->dh.se, b, 2 * n + 1, D);
+}
+
+typedef struct
+{
+    int k;
+    double **Un, **Vn, **sig;
+} prk_ctx;
+
+static void append_block(DPlrMtx *X, const prk_ctx *c, int d, int q)
+{
+    const int len = X->size, r0 = X->dlr.rank, k = c->k;
+    if (X->dlr.capacity < r0 + k) dplrresizelr(X, r0 + k);
+    const double *bu = c->Un[d] + (size_t)q * k * len;
+    const double *bv = c->Vn[d] + (size_t)q * k * len;
+    memcpy(X->dlr.u + (size_t)r0 * len, bu, (size_t)k * len * sizeof(double));
+    memcpy(X->dlr.v + (size_t)r0 * len, bv, (size_t)k * len * sizeof(double));
+    memcpy(X->dlr.sig + r0, c->sig[d] + (size_t)q * k, (size_t)k * sizeof(double));
+    X->dlr.rank = r0 + k;
+}
+
+static void append_walk(DPlrMtx *A, const prk_ctx *c, int d, int n)
+{
+    if (A->type != PLR_HMTX) return;
+    append_block(A->dh.ne, c, d, 2 * n);
+    append_block(A->dh.sw, c, d, 2 * n + 1);
+    append_walk(A->dh.nw, c, d + 1, 2 * n);
+    append_walk(A->dh.se, c, d + 1, 2 * n + 1);
+}
+
+/* reference src/dplrprk.c:21 */
+void dplrgeprk(int m, int k, double *alpha, double *u, int ldu, double *v, int ldv, double *s,
+               int incs, DPlrMtx *A, double eps)
+{
+    if (m < 0) {
+        fprintf(stderr, "%s: invalid matrix size %d\n", __func__, m);
+        return;
+    }
+    if (A->type != PLR_DMTX && A->type != PLR_HMTX) {
+        fprintf(stderr, "%s: invalid A type\n", __func__);
+        return;
+    }
+    if (A->size != m) {
+        fprintf(stderr, "%s: invalid A size\n", __func__);
+        return;
+    }
+    if (ldu < m || ldv < m) {
+        fprintf(stderr, "%s: invalid leading dimensions\n", __func__);
+        return;
+    }
+    if (*alpha == 0.0 || k == 0) return;
+    if (k < 0) {
+        fprintf(stderr, "%s: invalid rank %d\n", __func__, k);
+        return;
+    }
+    plr_flat f;
+    if (plr_flat_shape(m, A, &f, 1)) return;
+    const int N = f.N, L = f.L, b = f.b;
+    if (N == 0) return;
+    const size_t nk = (size_t)N * k;
+    double *hu = malloc(nk * sizeof(double)), *hv = malloc(nk * sizeof(double));
+    double *hs = malloc((size_t)k * sizeof(double));
+    double *hD = malloc((size_t)N * b * sizeof(double));
+    double *Un[DPLRH_MAXL] = {0}, *Vn[DPLRH_MAXL] = {0}, *sig[DPLRH_MAXL] = {0};
+    int ok = hu && hv && hs && hD;
+    for (int d = 0; d < L && ok; ++d) {
+        Un[d] = malloc(nk * sizeof(double));
+        Vn[d] = malloc(nk * sizeof(double));
+        sig[d] = malloc(((size_t)2 << d) * k * sizeof(double));
+        ok = Un[d] && Vn[d] && sig[d];
+    }
+    if (!ok) {
+        fprintf(stderr, "%s: host memory allocation failed\n", __func__);
+    } else {
+        for (int j = 0; j < k; ++j) {
+            memcpy(hu + (size_t)j * N, u + (size_t)j * ldu, (size_t)N * sizeof(double));
+            memcpy(hv + (size_t)j * N, v + (size_t)j * ldv, (size_t)N * sizeof(double));
+            hs[j] = s[(size_t)j * incs];
+        }
+        gather_leaves(A, b, 0, hD);
+        int rc = dplrh_geprk(N, L, b, k, *alpha, eps, hu, hv, hs, hD, Un, Vn, sig, NULL);
+        if (rc) {
+            fprintf(stderr, "%s: GPU execution failed (%d): %s\n", __func__, rc, dplrh_last_error());
+        } else {
+            prk_ctx c = {k, Un, Vn, sig};
+            append_walk(A, &c, 0, 0);
+            scatter_leaves(A, b, 0, hD);
+        }
+    }
+    for (int d = 0; d < DPLRH_MAXL; ++d) {
+        free(Un[d]);
+        free(Vn[d]);
+        free(sig[d]);
+    }
+    free(hu);
+    free(hv);
+    free(hs);
+    free(hD);
+}

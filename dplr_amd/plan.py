@@ -60,3 +60,36 @@ class Plan:
         check(lib.dplrx_gepdmm(self.p, trans.encode(), n, alpha, _dptr(B), B.stride(0), beta,
                                _dptr(C), C.stride(0), _stream(stream)), "dplrx_gepdmm")
         return C
+
+    def factor(self, stream=None):
+        """FP64 factorization in HBM; returns info (0, or > 0 when singular)."""
+        info = c_int(0)
+        check(lib.dplrx_factor(self.p, byref(info), _stream(stream)), "dplrx_factor")
+        return info.value
+
+    def solve(self, trans, B, X, stream=None):
+        """X := op(A)^{-1} B; B, X are (n, m) row-major torch tensors."""
+        n, m = B.shape
+        check(lib.dplrx_solve(self.p, trans.encode(), n, _dptr(B), B.stride(0), _dptr(X),
+                              X.stride(0), _stream(stream)), "dplrx_solve")
+        return X
+
+    LP_TYPES = {"f16": 1, "bf16": 2}
+
+    def factor_mixed(self, dtype="f16", stream=None):
+        """Mixed-precision factors (f16 or bf16); returns info."""
+        info = c_int(0)
+        check(lib.dplrx_factor_mixed(self.p, self.LP_TYPES[dtype], byref(info), _stream(stream)),
+              "dplrx_factor_mixed")
+        return info.value
+
+    def solve_refined(self, B, X, iters, with_residual=False, stream=None):
+        """X := A^{-1} B with `iters` refined mixed-precision solves.  Returns the
+        per-column max|B - A X| when with_residual (synchronises)."""
+        import numpy as np
+        n, m = B.shape
+        res = np.zeros(n) if with_residual else None
+        rp = res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if with_residual else None
+        check(lib.dplrx_solve_refined(self.p, n, _dptr(B), B.stride(0), _dptr(X), X.stride(0),
+                                      iters, rp, _stream(stream)), "dplrx_solve_refined")
+        return res

@@ -43,6 +43,28 @@ int dplrx_gepmv(dplrx_plan *plan, char trans, double alpha, const double *dx,
 int dplrx_gepdmm(dplrx_plan *plan, char transa, int n, double alpha, const double *dB,
                  int ldb, double beta, double *dC, int ldc, void *stream);
 
+/* Factor A in HBM for solves: FP64 leaf LU and inverses, Dinv-applied U
+ * factors and one inverted capacitance matrix per node.  *info as in
+ * dplrgeinv (> 0: singular leaf or capacitance). */
+int dplrx_factor(dplrx_plan *plan, int *info, void *stream);
+
+/* X := op(A)^{-1} B for n device columns (ldb, ldx >= m), after
+ * dplrx_factor.  X must not alias B. */
+int dplrx_solve(dplrx_plan *plan, char trans, int n, const double *dB, int ldb, double *dX,
+                int ldx, void *stream);
+
+/* Mixed-precision factors for refined solves: f16 (type 1) or bf16
+ * (type 2) U/V factors and leaf inverses with FP32 accumulation on MFMA,
+ * FP64 leaf LU and capacitance LU, FP64 capacitance inverses on the top two
+ * levels.  Needs the leaf order to be a multiple of 64. */
+int dplrx_factor_mixed(dplrx_plan *plan, int type, int *info, void *stream);
+
+/* X := A^{-1} B by `iters` mixed-precision solves with FP64 residuals
+ * r = B - A X between them (iterative refinement).  res (host, n entries,
+ * optional) receives max|B - A X| per column after the last solve. */
+int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
+                        int ldx, int iters, double *res, void *stream);
+
 const char *dplrx_last_error(void);
 
 /* ---- synthetic inputs (deterministic, counter-based RNG) --------------- */

@@ -1,0 +1,128 @@
+"""CPU tests of the C-ABI library: exports, lifecycle semantics (reference
+src/dplrini.c, src/dplrresizelr.c), host helpers, the synthetic generator,
+and loud failure of the GPU entry points when no device is present."""
+import ctypes
+import os
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import dplr_amd as D
+from dplr_amd.tree import PlrTree
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared(header):
+    text = open(os.path.join(ROOT, "include", header)).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return set(re.findall(r"\b(dplr\w*)\s*\(", text))
+
+
+def test_exports_every_declared_symbol():
+    out = subprocess.run(["nm", "-D", "--defined-only", D.LIB_PATH], capture_output=True,
+                         text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
+    for header in ("dplr.h", "dplrx.h"):
+        missing = declared(header) - exported
+        assert not missing, f"{header}: not exported: {sorted(missing)}"
+
+
+def test_node_layout():
+    from dplr_amd.abi import DPlrMtx
+    assert ctypes.sizeof(DPlrMtx) == 40
+    assert DPlrMtx.type.offset == 0 and DPlrMtx.size.offset == 4
+    lr = D.lib.dplrinilr  # noqa: F841  (layout is what the C side writes)
+    t = PlrTree()
+    D.lib.dplrinilr(64, t.ref, 3)
+    raw = ctypes.string_at(ctypes.addressof(t.root), 40)
+    assert int.from_bytes(raw[0:4], "little") == 3          # PLR_LRMTX
+    assert int.from_bytes(raw[4:8], "little") == 64         # size
+    assert int.from_bytes(raw[8:12], "little") == 0         # rank
+    assert int.from_bytes(raw[12:16], "little") == 3        # capacity
+
+
+def test_lifecycle_matches_reference_semantics(orc):
+    t = PlrTree()
+    D.lib.dplrinilr(10, t.ref, 2)
+    lr = t.root.dlr
+    assert lr.rank == 0 and lr.capacity == 2
+    assert np.all(np.isnan(np.ctypeslib.as_array(lr.u, shape=(20,))))
+    lr.u[0] = 1.5
+    lr.sig[0] = 2.0
+    lr.rank = 1
+    # equal request still doubles (reference src/dplrresizelr.c:42,54)
+    D.lib.dplrresizelr(t.ref, 2)
+    assert t.root.dlr.capacity == 4
+    D.lib.dplrresizelr(t.ref, 9)
+    assert t.root.dlr.capacity == 9
+    D.lib.dplrresizelr(t.ref, 3)
+    assert t.root.dlr.capacity == 9
+    assert t.root.dlr.u[0] == 1.5 and t.root.dlr.sig[0] == 2.0
+    assert np.isnan(t.root.dlr.u[10])
+    D.lib.dplrclr(t.ref)
+    assert t.root.type == 0
+    D.lib.dplrclr(t.ref)  # clearing twice is harmless
+    D.lib.dplrinih(8, t.ref)
+    assert t.root.type == 1 and t.root.dh.nw.contents.type == 0
+    D.lib.dplrinih(7, t.ref)  # odd order: diagnostic, node untouched
+    assert t.root.size == 8
+
+
+def test_dense_expansion_and_queries(orc, ref):
+    rng = np.random.default_rng(0)
+    A = PlrTree.from_blocks(256, 3, lambda d, n, w: (d + 2 * n + (w == "sw")) % 5, rng)
+    assert np.array_equal(A.dense(), A.dense(api=orc))
+    r1 = (ctypes.c_int * 3)(-1, -1, -1)
+    r2 = (ctypes.c_int * 3)(-1, -1, -1)
+    D.lib.dplrrnk(256, A.ref, r1)
+    ref["dplrrnk"](256, A.ref, r2)
+    assert list(r1) == list(r2)
+    assert D.lib.dplrnz(256, A.ref) == ref["dplrnz"](256, A.ref)
+
+
+def test_generator_is_deterministic_and_orthonormal():
+    A1 = PlrTree.random(1024, 4, 16, seed=5)
+    A2 = PlrTree.random(1024, 4, 16, seed=5)
+    A3 = PlrTree.random(1024, 4, 16, seed=6)
+    d1, d2, d3 = A1.dense(), A2.dense(), A3.dense()
+    assert np.array_equal(d1, d2) and not np.array_equal(d1, d3)
+    for (d, n, w, node) in A1.blocks():
+        u, s, v = A1.lr_arrays(node)
+        assert u.shape[1] == 16
+        assert np.abs(u.T @ u - np.eye(16)).max() < 1e-13
+        assert np.abs(v.T @ v - np.eye(16)).max() < 1e-13
+        assert np.allclose(s, 10.0 ** (-np.arange(16) / 4.0), rtol=1e-15, atol=0)
+    x1 = np.empty(1000)
+    x2 = np.empty(1000)
+    D.lib.dplrx_gen_normal(1000, 3, 1, x1.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    D.lib.dplrx_gen_normal(1000, 3, 1, x2.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    assert np.array_equal(x1, x2) and abs(x1.mean()) < 0.15 and abs(x1.std() - 1) < 0.1
+
+
+def test_nonuniform_tree_rejected(capfd):
+    t = PlrTree()
+    D.lib.dplrinih(8, t.ref)
+    D.lib.dplrinid(4, t.root.dh.nw)
+    D.lib.dplrinih(4, t.root.dh.se)
+    for c in ("nw", "se"):
+        D.lib.dplrinid(2, getattr(t.root.dh.se.contents.dh, c))
+    for node, h in ((t.root.dh.ne, 4), (t.root.dh.sw, 4), (t.root.dh.se.contents.dh.ne, 2),
+                    (t.root.dh.se.contents.dh.sw, 2)):
+        D.lib.dplrinilr(h, node, 0)
+    t.m = 8
+    y = np.full(8, 7.0)
+    out = D.gepmv("N", t, np.ones(8), 1.0, 0.0, y)
+    assert np.array_equal(out, y)
+    assert "uniform" in capfd.readouterr().err
+
+
+@pytest.mark.skipif(__import__("torch").cuda.is_available(), reason="checks the no-GPU path")
+def test_gpu_entry_fails_loudly_without_device(capfd):
+    A = PlrTree.random(256, 2, 4, seed=1)
+    y = np.full(256, 3.0)
+    out = D.gepmv("N", A, np.ones(256), 1.0, 0.0, y)
+    assert np.array_equal(out, y)          # untouched: no silent CPU fallback
+    assert "failed" in capfd.readouterr().err
