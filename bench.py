@@ -1,11 +1,22 @@
 """Benchmark of the PLR apply/solve hot path on one MI355X.
 
-Workload (N=65536, 10 levels, leaf 64, rank 32, seeded random PLR with
-orthonormal factors, sig_j = 10^(-j/4), leaves N(0,1)/8 + 4I): inputs are
-resident in HBM before timing.  Three independent matrices are rotated so
-that the 256 MB Infinity Cache cannot hold the working set between calls.
+Workload: N=65536, 10 levels, leaf 64, rank 32, 64 right-hand sides; seeded
+random PLR matrices (orthonormal U, V from Householder QR of N(0,1)
+matrices, sig_j = 10^(-j/4), leaves N(0,1)/8 + 4I).  Inputs are resident in
+HBM before timing; three independent matrices are rotated so the 256 MB
+Infinity Cache cannot hold the working set between calls.
 
-Prints ONE JSON line (rank 0).
+One step = apply and solve for the 64 right-hand sides, both to relative
+error <= 1e-10:  Y = A X (FP64 apply) and Z = inv(A) B.
+  * mixed  (the reported value): Z from f16 factors with FP32 accumulation on
+    MFMA, refined with FP64 residuals until the relative error against the
+    FP64 solve is <= 1e-10 (iteration count calibrated before timing);
+  * plain FP64: Z from the FP64 factored solve.
+Factorizations are done once per matrix outside the step and timed
+separately ("factor_ms"); "incl_factor" gives the rates when every step also
+factors its matrix.
+
+Prints ONE JSON line.
 """
 import argparse
 import json
@@ -18,6 +29,7 @@ sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
 FP64_PEAK_TFS = 78.6
+TARGET_ERR = 1e-10
 
 
 def parse():
@@ -30,13 +42,15 @@ def parse():
     ap.add_argument("--nlvl", type=int, default=10)
     ap.add_argument("--rank", type=int, default=32)
     ap.add_argument("--nmat", type=int, default=3)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--lp", default="f16", choices=["f16", "bf16"])
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     return ap.parse_args()
 
 
-def event_time(torch, fn, reps):
-    """Average ms per call of fn() measured with events on the current stream."""
+def event_ms(torch, fn, reps):
+    """Average ms per call of fn(i), HIP events on the current stream (the
+    stream every library call in this script is launched on)."""
     s, e = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
     s.record()
@@ -48,48 +62,49 @@ def event_time(torch, fn, reps):
 
 
 def cpu_baseline(args, log):
-    """The reference sources (oracle/_ref, OpenBLAS) on host cores, bounded sample."""
+    """The reference sources (oracle/_ref, OpenBLAS, one thread) on the host:
+    apply + explicit inverse + inverse applied, once, on the same workload
+    shape (a bounded sample: one 64-RHS batch on one matrix)."""
+    import ctypes
+    import glob
+
     import numpy as np
 
-    import oracle
     import dplr_amd as D
+    import oracle
 
-    ref = oracle.load_ref()
+    api = oracle.load_ref()
     kind = "reference"
-    if ref is None:
-        ref = oracle.load_oracle()
+    if api is None:
+        api = oracle.load_oracle()
         kind = "port"
-    lib = ref["_lib"]
-    threads = 1
-    if kind == "reference":
-        try:
-            lib_ob = None
-            import ctypes
-            import glob
-            so = glob.glob(os.path.join(os.path.dirname(np.__file__), "..", "scipy.libs",
-                                        "libscipy_openblas*.so"))
-            if so:
-                lib_ob = ctypes.CDLL(so[0])
-                lib_ob.scipy_openblas_set_num_threads(1)
-        except Exception:
-            pass
+    else:
+        so = glob.glob(os.path.join(os.path.dirname(np.__file__), "..", "scipy.libs",
+                                    "libscipy_openblas*.so"))
+        if so:
+            ctypes.CDLL(so[0]).scipy_openblas_set_num_threads(1)
     A = D.PlrTree.random(args.m, args.nlvl, args.rank, 4.0, 1)
     rng = np.random.default_rng(0)
-    B = np.asfortranarray(rng.standard_normal((args.m, args.nrhs)))
+    X = np.asfortranarray(rng.standard_normal((args.m, args.nrhs)))
     t0 = time.perf_counter()
-    reps = 0
-    while True:
-        D.gepdmm("N", "N", A, B, args.nrhs, api=ref)
-        reps += 1
-        el = time.perf_counter() - t0
-        if el > args.cpu_seconds or reps >= 50:
-            break
-    rhs_s = reps * args.nrhs / el
-    log(f"cpu baseline ({kind}): {reps} x dplrgepdmm {args.nrhs} RHS in {el:.2f}s")
+    D.gepdmm("N", "N", A, X, args.nrhs, api=api)
+    t_apply = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    B, info = D.geinv(A, 0.0, api=api)
+    t_inv = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    D.gepdmm("N", "N", B, X, args.nrhs, api=api)
+    t_solve = time.perf_counter() - t0
+    total = t_apply + t_inv + t_solve
+    log(f"cpu ({kind}): apply {t_apply:.2f}s, inverse {t_inv:.2f}s, apply inverse {t_solve:.2f}s")
     A.free()
-    return {"value": rhs_s, "unit": "RHS/s", "cores": threads, "kind": kind,
-            "sample": f"{reps} x dplrgepdmm('N','N') with {args.nrhs} RHS at N={args.m} "
-                      f"(reference C sources + OpenBLAS 0.3.28, 1 thread)"}
+    B.free()
+    return {"value": args.nrhs / total, "unit": "RHS/s", "cores": 1, "kind": kind,
+            "sample": f"one {args.nrhs}-RHS batch on one N={args.m} matrix: dplrgepdmm(A) + "
+                      f"dplrgeinv + dplrgepdmm(inv(A)) with the reference C sources and "
+                      f"OpenBLAS 0.3.28, 1 thread ({total:.1f} s)",
+            "apply_only_rhs_s": args.nrhs / t_apply,
+            "solve_excl_inverse_rhs_s": args.nrhs / t_solve}
 
 
 def main():
@@ -119,32 +134,87 @@ def main():
     abytes = plans[0].nbytes
     rng = np.random.default_rng(0)
     X = torch.tensor(rng.standard_normal((nrhs, m)), device=dev)
+    B = torch.tensor(rng.standard_normal((nrhs, m)), device=dev)
     Y = torch.zeros_like(X)
+    Z = torch.zeros_like(X)
+    Z64 = [torch.zeros_like(X) for _ in plans]
     x1 = X[0].contiguous()
     y1 = torch.zeros(m, dtype=torch.float64, device=dev)
 
-    # --- single-RHS matvec (HBM-bound) ------------------------------------
+    # --- factorizations (outside the step) --------------------------------
+    t0 = time.perf_counter()
+    for P in plans:
+        assert P.factor() == 0
+    torch.cuda.synchronize()
+    f64_ms = (time.perf_counter() - t0) * 1e3 / len(plans)
+    t0 = time.perf_counter()
+    for P in plans:
+        assert P.factor_mixed(args.lp) == 0
+    torch.cuda.synchronize()
+    lp_ms = (time.perf_counter() - t0) * 1e3 / len(plans)
+    log(f"factor: fp64 {f64_ms:.2f} ms, {args.lp} {lp_ms:.2f} ms (host-timed, per matrix)")
+
+    # --- single-RHS matvec (HBM-bound kernel pair) -------------------------
     mv = lambda i: plans[i % len(plans)].gepmv("N", 1.0, x1, 0.0, y1)
-    for i in range(args.warmup * 3):
+    for i in range(6):
         mv(i)
-    reps = max(args.steps * 5, 30)
-    ms_mv = event_time(torch, mv, reps)
+    ms_mv = event_ms(torch, mv, 60)
     mv_bytes = abytes + 2 * 8 * m
     mv_gbs = mv_bytes / (ms_mv * 1e-3) / 1e9
-    log(f"matvec 1 RHS: {ms_mv * 1e3:.1f} us  {mv_gbs:.0f} GB/s")
+    log(f"matvec 1 RHS: {ms_mv * 1e3:.1f} us  {mv_gbs:.0f} GB/s ({mv_gbs / HBM_PEAK_GBS:.1%} of HBM)")
 
-    # --- 64-RHS FP64 apply (timed step) ------------------------------------
-    step = lambda i: plans[i % len(plans)].gepdmm("N", 1.0, X, 0.0, Y)
+    # --- legs of the step ---------------------------------------------------
+    apply = lambda i: plans[i % len(plans)].gepdmm("N", 1.0, X, 0.0, Y)
+    solve64 = lambda i: plans[i % len(plans)].solve("N", B, Z)
+    for i in range(3):
+        apply(i)
+        solve64(i)
+    ms_apply = event_ms(torch, apply, 10)
+    ms_solve64 = event_ms(torch, solve64, 10)
+    flops = nrhs * (2.0 * plans[0].leaf * m + 4.0 * m * sum(plans[0].ranks))
+    log(f"apply {nrhs} RHS: {ms_apply * 1e3:.0f} us ({flops / (ms_apply * 1e-3) / 1e12:.1f} TFLOP/s); "
+        f"fp64 solve: {ms_solve64 * 1e3:.0f} us")
+
+    # calibrate the refinement: fewest iterations reaching the target error
+    for k, P in enumerate(plans):
+        P.solve("N", B, Z64[k])
+    torch.cuda.synchronize()
+    iters, err = None, None
+    for it in range(1, 12):
+        errs = []
+        for k, P in enumerate(plans):
+            P.solve_refined(B, Z, it)
+            torch.cuda.synchronize()
+            errs.append(((Z - Z64[k]).norm() / Z64[k].norm()).item())
+        if max(errs) <= TARGET_ERR:
+            iters, err = it, max(errs)
+            break
+        err = max(errs)
+    if iters is None:
+        log(f"refinement did not reach {TARGET_ERR} (last {err:.2e})")
+        iters = 11
+    log(f"mixed ({args.lp}): {iters} refined solves reach rel err {err:.2e}")
+    mixed = lambda i: plans[i % len(plans)].solve_refined(B, Z, iters)
+    for i in range(2):
+        mixed(i)
+    ms_mixed = event_ms(torch, mixed, 10)
+
+    # --- the timed step ------------------------------------------------------
+    def step(i):
+        apply(i)
+        mixed(i)
+
     for i in range(args.warmup):
         step(i)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ms_step = event_time(torch, step, args.steps)
+    ms_step = event_ms(torch, step, args.steps)
     wall = time.perf_counter() - t0
     rhs_s = nrhs / (ms_step * 1e-3)
-    flops = nrhs * (2.0 * plans[0].leaf * m + 4.0 * m * sum(plans[0].ranks))
-    log(f"apply {nrhs} RHS: {ms_step * 1e3:.1f} us/step  {rhs_s:.0f} RHS/s  "
-        f"{flops / (ms_step * 1e-3) / 1e12:.2f} TFLOP/s")
+    plain_ms = ms_apply + ms_solve64
+    plain_rhs_s = nrhs / (plain_ms * 1e-3)
+    log(f"step (apply + mixed solve): {ms_step * 1e3:.0f} us -> {rhs_s:.0f} RHS/s; "
+        f"plain fp64 {plain_ms * 1e3:.0f} us -> {plain_rhs_s:.0f} RHS/s")
 
     cpu = None
     if not args.no_cpu:
@@ -164,14 +234,23 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": f"f64 apply + {args.lp}/f32-accum refined solve",
         "data": "synthetic",
-        "config": {"workload": f"FP64 PLR apply, {nrhs} RHS, N={m}, L={args.nlvl}, leaf "
-                               f"{m >> args.nlvl}, rank {args.rank}",
-                   "global_batch": nrhs, "matrices_rotated": args.nmat},
+        "config": {"workload": f"apply + solve, {nrhs} RHS, N={m}, L={args.nlvl}, leaf "
+                               f"{m >> args.nlvl}, rank {args.rank}, {args.nmat} matrices rotated",
+                   "global_batch": nrhs},
+        "plain_fp64": {"rhs_s": plain_rhs_s, "ms_per_step": plain_ms,
+                       "apply_ms": ms_apply, "solve_ms": ms_solve64},
+        "mixed": {"rhs_s": rhs_s, "solve_ms": ms_mixed, "refine_solves": iters,
+                  "rel_err_vs_fp64_solve": err, "lowprec": args.lp},
+        "speedup_vs_plain_fp64": plain_ms / ms_step,
+        "factor_ms": {"fp64": f64_ms, args.lp: lp_ms},
+        "incl_factor": {"mixed_rhs_s": nrhs / ((ms_step + lp_ms) * 1e-3),
+                        "plain_fp64_rhs_s": nrhs / ((plain_ms + f64_ms) * 1e-3)},
+        "apply_fp64_tflops": flops / (ms_apply * 1e-3) / 1e12,
         "roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": mv_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "single-RHS matvec (mv_reduce + mv_expand)",
+                     "kernel": "single-RHS matvec (k_mv_reduce + k_mv_expand)",
                      "bytes_per_launch": mv_bytes, "us_per_launch": ms_mv * 1e3},
         "cpu_baseline": cpu,
         "wall_s_timed": wall,

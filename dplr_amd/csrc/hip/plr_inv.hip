@@ -117,14 +117,20 @@ __global__ void k_inv_scales(InvLevels a, int dp, const double *sq_ou, const dou
 }
 
 // Leaf factor of the correction columns of level d: G1 * d2 * G2 per
-// (segment s of level d, column).
+// (segment s of level d, column); zero for the padding columns beyond the
+// true rank k of the correction (the rank of the block on the sibling
+// segment).
 __global__ void k_inv_leaf_factor(InvLevels a, int d, const double *sq_cu, const double *sq_vt,
-                                  double *lf)
+                                  const int *rank, double *lf)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int nseg = 2 << d, R = a.R[d];
     if (e >= nseg * R) return;
     const int s = e / R, cp = e % R, col = a.col[d] + cp;
+    if (cp >= rank[(2 << d) - 2 + (s ^ 1)]) {
+        lf[e] = 0.0;
+        return;
+    }
     const double F1 = seg_norm(sq_cu, a.nleaf, a.L, col, d, s);
     const double F2 = seg_norm(sq_vt, a.nleaf, a.L, col, d, s);
     double d2, G1, G2;
@@ -317,7 +323,7 @@ extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][
             const int nl = (2 << d) * p->R[d];
             if (nl == 0) continue;
             hipLaunchKernelGGL(k_inv_leaf_factor, dim3((nl + 255) / 256), dim3(256), 0, st, a, d,
-                               sq_cu, sq_vt, lf + hlfoff[d]);
+                               sq_cu, sq_vt, p->rank, lf + hlfoff[d]);
             PLR_HIP(hipGetLastError());
         }
         // 4. per level: scales, gathered factors, download

@@ -157,9 +157,17 @@ __global__ __launch_bounds__(256) void k_lp_finish(LpTab a, const float *__restr
         const int kl = e % LP_KG, cl = e / LP_KG;
         const int c = cg * LP_CG + cl;
         if (c >= a.ncols) continue;
-        float t = 0.f;
-        for (int j = 0; j < tps; ++j)
-            t += part[((size_t)a.part_off[d] + (((size_t)s * tps + j) * nkg + kg) * ncg + cg) * LP_BLK + e];
+        const float *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * LP_BLK + e;
+        const size_t step = (size_t)nkg * ncg * LP_BLK;
+        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < tps; j += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const float v = pp[(size_t)min(j + u, tps - 1) * step];
+                acc[u] += (j + u < tps) ? v : 0.f;
+            }
+        }
+        const float t = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         out[(size_t)c * ostride + (size_t)s * R + kg * LP_KG + kl] = t;
     }
 }

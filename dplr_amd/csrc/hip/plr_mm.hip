@@ -86,32 +86,39 @@ __global__ __launch_bounds__(64) void k_mm_reduce(MmTab a, const double *__restr
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = d4{0.0, 0.0, 0.0, 0.0};
 
+    // Loads are unconditional (columns past the end read column 0) and are
+    // zeroed after they land, so a whole 8-row chunk is in flight at once.
     for (int rb = rbeg; rb < rend; rb += 64) {
         const int r = rb + 16 * g;
 #pragma unroll
-        for (int t0 = 0; t0 < 16; t0 += 4) {
-            double av[2][4], bv[4][4];
+        for (int t0 = 0; t0 < 16; t0 += 8) {
+            double2 ua[2][4], ub[4][4];
 #pragma unroll
-            for (int kt = 0; kt < 2; ++kt) {
-                const double2 *q = (const double2 *)(pa[kt] + r + t0);
-                double2 u0 = va[kt] ? q[0] : double2{0.0, 0.0};
-                double2 u1 = va[kt] ? q[1] : double2{0.0, 0.0};
-                av[kt][0] = u0.x; av[kt][1] = u0.y; av[kt][2] = u1.x; av[kt][3] = u1.y;
-            }
+            for (int kt = 0; kt < 2; ++kt)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                const double2 *q = (const double2 *)(pb[ct] + r + t0);
-                double2 u0 = vb[ct] ? q[0] : double2{0.0, 0.0};
-                double2 u1 = vb[ct] ? q[1] : double2{0.0, 0.0};
-                bv[ct][0] = u0.x; bv[ct][1] = u0.y; bv[ct][2] = u1.x; bv[ct][3] = u1.y;
-            }
+                for (int h = 0; h < 4; ++h) ua[kt][h] = *(const double2 *)(pa[kt] + r + t0 + 2 * h);
 #pragma unroll
-            for (int tt = 0; tt < 4; ++tt)
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int h = 0; h < 4; ++h) ub[ct][h] = *(const double2 *)(pb[ct] + r + t0 + 2 * h);
+#pragma unroll
+            for (int tt = 0; tt < 8; ++tt) {
+                double av[2], bv[4];
+#pragma unroll
+                for (int kt = 0; kt < 2; ++kt) {
+                    const double v = (tt & 1) ? ua[kt][tt >> 1].y : ua[kt][tt >> 1].x;
+                    av[kt] = va[kt] ? v : 0.0;
+                }
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) {
+                    const double v = (tt & 1) ? ub[ct][tt >> 1].y : ub[ct][tt >> 1].x;
+                    bv[ct] = vb[ct] ? v : 0.0;
+                }
 #pragma unroll
                 for (int kt = 0; kt < 2; ++kt)
 #pragma unroll
-                    for (int ct = 0; ct < 4; ++ct)
-                        acc[kt][ct] = mfma64(av[kt][tt], bv[ct][tt], acc[kt][ct]);
+                    for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = mfma64(av[kt], bv[ct], acc[kt][ct]);
+            }
         }
     }
 
@@ -164,11 +171,18 @@ __global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int tr
         const int kl = e % MM_KG, cl = e / MM_KG;
         const int k = kg * MM_KG + kl, c = cg * MM_CG + cl;
         if (k >= R || c >= a.ncols) continue;
-        double t = 0.0;
-        for (int j = 0; j < tps; ++j) {
-            const size_t st = (size_t)s * tps + j;
-            t += part[((size_t)a.part_off[d] + (st * nkg + kg) * ncg + cg) * MM_BLK + e];
+        // independent accumulators keep eight loads in flight
+        const double *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * MM_BLK + e;
+        const size_t step = (size_t)nkg * ncg * MM_BLK;
+        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+        for (int j = 0; j < tps; j += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double v = pp[(size_t)min(j + u, tps - 1) * step];
+                acc[u] += (j + u < tps) ? v : 0.0;
+            }
         }
+        const double t = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
         double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
         wc[k] = raw ? t : alpha * a.S[d][(size_t)sidx * R + k] * t;
     }
@@ -202,17 +216,27 @@ __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__rest
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
 
+    // Every operand batch (four MFMA k-steps) is loaded before it is used;
+    // loads are unconditional and out-of-range columns are zeroed afterwards.
     if (D) {
         const int leaf = r0 / b, il = r0 - leaf * b;
         const double *Dl = D + (size_t)leaf * b * b;
         const double *xl = X + (size_t)leaf * b;
-        for (int j = 0; j < b; j += 4) {
-            // B[k][n] = D^T[j+k][il+n] = D[il+n][j+k]  ('N'),  D[j+k][il+n] ('T')
-            const double bq = trans ? Dl[(size_t)(il + m) * b + j + g] : Dl[(size_t)(j + g) * b + il + m];
+        for (int j = 0; j < b; j += 16) {
+            double bq[4], av[4][4];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                const double av = vc[ct] ? xl[(size_t)cc[ct] * ldx + j + g] : 0.0;
-                acc[ct] = mfma64(av, bq, acc[ct]);
+            for (int u = 0; u < 4; ++u) {
+                const int jj = min(j + 4 * u, b - 4) + g;
+                bq[u] = trans ? Dl[(size_t)(il + m) * b + jj] : Dl[(size_t)jj * b + il + m];
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) av[u][ct] = xl[(size_t)cc[ct] * ldx + jj];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = j + 4 * u < b;
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+                    acc[ct] = mfma64(vc[ct] && ok ? av[u][ct] : 0.0, bq[u], acc[ct]);
             }
         }
 #pragma unroll
@@ -220,27 +244,46 @@ __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__rest
     }
     for (int d = 0; d < a.L; ++d) {
         const int R = a.R[d];
+        if (R == 0) continue;
         const int s = r0 / a.seglen[d];
         const double *Qd = a.Q[d];
         const double *wd = w + a.w_off[d] + (size_t)s * R;
-        for (int k = 0; k < R; k += 4) {
-            const double bq = Qd[(size_t)(k + g) * N + r0 + m];
+        for (int k = 0; k < R; k += 16) {
+            double bq[4], av[4][4];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                const double av = vc[ct] ? wd[(size_t)cc[ct] * wstride + k + g] : 0.0;
-                acc[ct] = mfma64(av, bq, acc[ct]);
+            for (int u = 0; u < 4; ++u) {
+                const int kk = min(k + 4 * u, R - 4) + g;
+                bq[u] = Qd[(size_t)kk * N + r0 + m];
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) av[u][ct] = wd[(size_t)cc[ct] * wstride + kk];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const bool ok = k + 4 * u < R;
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+                    acc[ct] = mfma64(vc[ct] && ok ? av[u][ct] : 0.0, bq[u], acc[ct]);
             }
         }
     }
     // lane holds Y^T[c0 + 16ct + g + 4v][r0 + m]
+    double yo[4][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
+                yo[ct][v] = Y[(size_t)c * ldy + r0 + m];
+            }
+    }
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = c0 + ct * 16 + g + 4 * v;
             if (c >= a.ncols) continue;
-            double *y = Y + (size_t)c * ldy + r0 + m;
-            *y = (beta == 0.0 ? 0.0 : beta * *y) + acc[ct][v];
+            Y[(size_t)c * ldy + r0 + m] = (beta == 0.0 ? 0.0 : beta * yo[ct][v]) + acc[ct][v];
         }
 }
 

@@ -54,17 +54,24 @@ __global__ __launch_bounds__(256) void k_mv_reduce(LevelTab a, const double *__r
     const int N = a.N, seg = a.seglen[d], T = a.T[d];
     const int rbeg = s * seg + tile * T;
     const int rend = min(rbeg + T, (s + 1) * seg);
+    // Columns past the rank group's end load a duplicate of its last column:
+    // every load is unconditional (no branch, no wait per load) and the
+    // duplicates' sums are never written.
     const double *__restrict__ P = a.P[d] + (size_t)k0 * N;
+    size_t coff[KG];
+#pragma unroll
+    for (int kk = 0; kk < KG; ++kk) coff[kk] = (size_t)(kk < kn ? kk : kn - 1) * N;
 
     double acc[KG];
 #pragma unroll
     for (int kk = 0; kk < KG; ++kk) acc[kk] = 0.0;
-#pragma unroll 2
     for (int i = rbeg + lane; i < rend; i += 64) {
+        double pv[KG];
+#pragma unroll
+        for (int kk = 0; kk < KG; ++kk) pv[kk] = __builtin_nontemporal_load(P + coff[kk] + i);
         const double xv = x[i];
 #pragma unroll
-        for (int kk = 0; kk < KG; ++kk)
-            if (kk < kn) acc[kk] += P[(size_t)kk * N + i] * xv;
+        for (int kk = 0; kk < KG; ++kk) acc[kk] += pv[kk] * xv;
     }
 #pragma unroll
     for (int kk = 0; kk < KG; ++kk) acc[kk] = wave_sum(acc[kk]);
@@ -94,14 +101,27 @@ __global__ __launch_bounds__(256) void k_mv_reduce(LevelTab a, const double *__r
     if (lane == 0) old = atomicAdd(c, 1);
     old = __shfl(old, 0, 64);
     if (old != tps - 1) return;
-    // Last tile of this segment: every other tile's partial is visible.
+    // Last tile of this segment: every other tile's partial is visible.  All
+    // 64 lanes sum: lane = rank (16) x tile phase (4), four independent
+    // accumulators each, so the loads overlap; then the phases combine.
     __threadfence();
-    if (lane < kn) {
-        double t = 0.0;
-        const double *q = pd + (size_t)s * tps * R + k0 + lane;
-        for (int j = 0; j < tps; ++j) t += __builtin_nontemporal_load(q + (size_t)j * R);
-        wd[k0 + lane] = raw ? t : alpha * S[k0 + lane] * t;
+    const int kk = lane & 15, ph = lane >> 4;
+    const int kc = k0 + (kk < kn ? kk : kn - 1);
+    const double *q = pd + (size_t)s * tps * R + kc;
+    double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0;
+    for (int j = ph; j < tps; j += 16) {
+        const int j1 = min(j + 4, tps - 1), j2 = min(j + 8, tps - 1), j3 = min(j + 12, tps - 1);
+        const double v0 = q[(size_t)j * R], v1 = q[(size_t)j1 * R];
+        const double v2 = q[(size_t)j2 * R], v3 = q[(size_t)j3 * R];
+        t0 += v0;
+        t1 += (j + 4 < tps) ? v1 : 0.0;
+        t2 += (j + 8 < tps) ? v2 : 0.0;
+        t3 += (j + 12 < tps) ? v3 : 0.0;
     }
+    double t = (t0 + t1) + (t2 + t3);
+    t += __shfl_xor(t, 16, 64);
+    t += __shfl_xor(t, 32, 64);
+    if (ph == 0 && kk < kn) wd[k0 + kk] = raw ? t : alpha * S[k0 + kk] * t;
     if (lane == 0) atomicExch(c, 0);
 }
 

@@ -20,6 +20,12 @@ def _dptr(t):
     return c_void_p(t.data_ptr())
 
 
+def _ld(t):
+    """Leading dimension of an (n, m) row-major tensor seen as a column-major
+    m x n block (torch reports stride 1 for a leading dimension of size 1)."""
+    return t.stride(0) if t.shape[0] > 1 else t.shape[1]
+
+
 class Plan:
     """An uploaded PLR matrix (level-wise FP64 arrays in HBM)."""
 
@@ -57,8 +63,8 @@ class Plan:
         """C := alpha*op(A)*B + beta*C; B, C are (n, m) row-major torch tensors,
         i.e. column-major m x n blocks with leading dimension m."""
         n, m = B.shape
-        check(lib.dplrx_gepdmm(self.p, trans.encode(), n, alpha, _dptr(B), B.stride(0), beta,
-                               _dptr(C), C.stride(0), _stream(stream)), "dplrx_gepdmm")
+        check(lib.dplrx_gepdmm(self.p, trans.encode(), n, alpha, _dptr(B), _ld(B), beta,
+                               _dptr(C), _ld(C), _stream(stream)), "dplrx_gepdmm")
         return C
 
     def factor(self, stream=None):
@@ -70,8 +76,8 @@ class Plan:
     def solve(self, trans, B, X, stream=None):
         """X := op(A)^{-1} B; B, X are (n, m) row-major torch tensors."""
         n, m = B.shape
-        check(lib.dplrx_solve(self.p, trans.encode(), n, _dptr(B), B.stride(0), _dptr(X),
-                              X.stride(0), _stream(stream)), "dplrx_solve")
+        check(lib.dplrx_solve(self.p, trans.encode(), n, _dptr(B), _ld(B), _dptr(X),
+                              _ld(X), _stream(stream)), "dplrx_solve")
         return X
 
     LP_TYPES = {"f16": 1, "bf16": 2}
@@ -90,6 +96,6 @@ class Plan:
         n, m = B.shape
         res = np.zeros(n) if with_residual else None
         rp = res.ctypes.data_as(ctypes.POINTER(ctypes.c_double)) if with_residual else None
-        check(lib.dplrx_solve_refined(self.p, n, _dptr(B), B.stride(0), _dptr(X), X.stride(0),
+        check(lib.dplrx_solve_refined(self.p, n, _dptr(B), _ld(B), _dptr(X), _ld(X),
                                       iters, rp, _stream(stream)), "dplrx_solve_refined")
         return res

@@ -67,8 +67,9 @@ class PlrTree:
         def fill(node, size, d, n):
             if d == nlvl:
                 lib.dplrinid(size, node)
-                blk = rng.standard_normal((size, size)) / np.sqrt(max(size, 1)) + shift * np.eye(size)
-                ctypes.memmove(node.contents.dd.d, np.asfortranarray(blk).ctypes.data, blk.nbytes)
+                blk = np.asfortranarray(
+                    rng.standard_normal((size, size)) / np.sqrt(max(size, 1)) + shift * np.eye(size))
+                ctypes.memmove(node.contents.dd.d, blk.ctypes.data, blk.nbytes)
                 return
             lib.dplrinih(size, node)
             h = size // 2
@@ -77,13 +78,16 @@ class PlrTree:
                 child = getattr(node.contents.dh, which)
                 lib.dplrinilr(h, child, r)
                 if r > 0:
-                    u = np.linalg.qr(rng.standard_normal((h, r)))[0]
-                    v = np.linalg.qr(rng.standard_normal((h, r)))[0]
-                    s = sig_of(d, n, which, r) if sig_of else 10.0 ** (-np.arange(r) / 4.0)
+                    # keep every array alive while its address is in use
+                    u = np.asfortranarray(np.linalg.qr(rng.standard_normal((h, r)))[0])
+                    v = np.asfortranarray(np.linalg.qr(rng.standard_normal((h, r)))[0])
+                    s = np.ascontiguousarray(
+                        sig_of(d, n, which, r) if sig_of else 10.0 ** (-np.arange(r) / 4.0),
+                        dtype=np.float64)
                     lr = child.contents.dlr
-                    ctypes.memmove(lr.u, np.asfortranarray(u).ctypes.data, u.nbytes)
-                    ctypes.memmove(lr.v, np.asfortranarray(v).ctypes.data, v.nbytes)
-                    ctypes.memmove(lr.sig, np.ascontiguousarray(s, dtype=np.float64).ctypes.data, 8 * r)
+                    ctypes.memmove(lr.u, u.ctypes.data, u.nbytes)
+                    ctypes.memmove(lr.v, v.ctypes.data, v.nbytes)
+                    ctypes.memmove(lr.sig, s.ctypes.data, 8 * r)
                     lr.rank = r
             fill(node.contents.dh.nw, h, d + 1, 2 * n)
             fill(node.contents.dh.se, h, d + 1, 2 * n + 1)
