@@ -72,7 +72,37 @@ struct dplrh_plan
     plr_fact f;
     plr_solve_ws sw;
     int force_valu; // testing aid: use the general-shape kernels everywhere
+    // HIP graphs of repeated calls (see plr_run_cached)
+    unsigned ws_gen;  // bumped whenever plan-owned device memory is reallocated
+    hipStream_t cap;  // private capture stream (default priority)
+    struct
+    {
+        unsigned long long key[10];
+        unsigned gen;
+        int seen;
+        hipGraphExec_t exec;
+        unsigned long long tick;
+    } graphs[16];
+    unsigned long long tick;
 };
+
+// Run op(stream) on `user`, through a cached HIP graph when the same call
+// (key) was made before with the same workspace generation: the first call
+// runs eagerly (and sizes the workspace), the second is captured on a
+// private stream and launched, later calls replay it.  op must neither
+// allocate nor synchronise once the workspace is sized.
+int plr_run_cached_impl(dplrh_plan *p, const unsigned long long *key, hipStream_t user,
+                        int (*op)(void *ctx, hipStream_t), void *ctx);
+template <typename F>
+int plr_run_cached(dplrh_plan *p, const unsigned long long *key, hipStream_t user, F &&f)
+{
+    auto thunk = [](void *c, hipStream_t s) -> int { return (*(F *)c)(s); };
+    return plr_run_cached_impl(p, key, user, thunk, (void *)&f);
+}
+void plr_graphs_clear(dplrh_plan *p);
+// C := alpha*op(A)*B + beta*C without the graph cache (for use inside other ops).
+int plr_gemm_direct(dplrh_plan *p, int trans, int n, double alpha, double beta,
+                    const double *dB, int ldb, double *dC, int ldc, hipStream_t st);
 
 void plr_set_error(const char *fmt, ...);
 

@@ -494,10 +494,11 @@ static void lp_tab(const dplrh_plan *p, const plr_lp *q, int d0, int d1, int nco
     t->fin_off[t->L] = fin;
 }
 
-static int lp_part(plr_lp *q, size_t blocks)
+static int lp_part(const dplrh_plan *p, plr_lp *q, size_t blocks)
 {
     const size_t need = blocks * LP_BLK;
     if (need <= q->part_cap) return 0;
+    ++const_cast<dplrh_plan *>(p)->ws_gen; // buffers move: cached graphs are stale
     if (q->part) (void)hipFree(q->part);
     q->part = nullptr;
     q->part_cap = 0;
@@ -509,6 +510,7 @@ static int lp_part(plr_lp *q, size_t blocks)
 static int lp_ws(const dplrh_plan *p, plr_lp *q, int cols)
 {
     if (cols <= q->cols) return 0;
+    ++const_cast<dplrh_plan *>(p)->ws_gen;
     void *old[] = {q->T, q->W, q->wscale, q->r32, q->z32, q->r, q->scale};
     for (void *x : old)
         if (x) (void)hipFree(x);
@@ -540,7 +542,7 @@ static int lp_reduce(const dplrh_plan *p, plr_lp *q, int d, const float *Z, int 
 {
     LpTab t;
     lp_tab(p, q, d, d + 1, ncols, &t);
-    int rc = lp_part(q, (size_t)t.part_off[1]);
+    int rc = lp_part(p, q, (size_t)t.part_off[1]);
     if (rc) return rc;
     if (t.task_off[1] > 0) {
         hipLaunchKernelGGL(k_lp_reduce<T16>, dim3(t.task_off[1]), dim3(64), 0, st, t, Z, ldz,
@@ -700,6 +702,7 @@ extern "C" int dplrh_lp_factor(dplrh_plan *p, int type, int *info, void *stream)
         return -5;
     }
     plr_lp_release(p);
+    ++p->ws_gen;
     plr_lp *q = new plr_lp;
     memset(q, 0, sizeof(*q));
     q->type = type;
@@ -754,15 +757,9 @@ static int lp_solve_t(const dplrh_plan *p, plr_lp *q, int n, hipStream_t st)
 // X := A^{-1} B by mixed-precision solves refined with FP64 residuals:
 // `iters` solves (iters-1 residuals).  When res != NULL, res[c] receives
 // max|b - A x| of column c after the last solve (one more FP64 residual).
-extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, double *dX,
-                              int ldx, int iters, double *res, void *stream)
+static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int ldb, double *dX,
+                         int ldx, int iters, double *res, hipStream_t st)
 {
-    hipStream_t st = (hipStream_t)stream;
-    plr_lp *q = lp_of(p);
-    if (!q) {
-        plr_set_error("solve before a successful mixed-precision factorization");
-        return -4;
-    }
     const int N = p->N;
     if (N == 0 || n == 0) return 0;
     int rc = 0;
@@ -776,7 +773,7 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
             // r = b - A x (x = 0 on the first pass)
             PLR_HIP(hipMemcpy2DAsync(q->r, (size_t)N * sizeof(double), B, (size_t)ldb * sizeof(double),
                                      (size_t)N * sizeof(double), nc, hipMemcpyDeviceToDevice, st));
-            if (it > 0 && (rc = dplrh_gemm(p, 0, nc, -1.0, 1.0, X, ldx, q->r, N, st))) return rc;
+            if (it > 0 && (rc = plr_gemm_direct(p, 0, nc, -1.0, 1.0, X, ldx, q->r, N, st))) return rc;
             hipLaunchKernelGGL(k_lp_scale_rhs, dim3(nc), dim3(256), 0, st, q->r, N, N, q->scale,
                                q->r32);
             PLR_HIP(hipGetLastError());
@@ -794,4 +791,23 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
         }
     }
     return 0;
+}
+
+extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, double *dX,
+                              int ldx, int iters, double *res, void *stream)
+{
+    plr_lp *q = lp_of(p);
+    if (!q) {
+        plr_set_error("solve before a successful mixed-precision factorization");
+        return -4;
+    }
+    if (res) // reads results back to the host: not capturable
+        return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, res, (hipStream_t)stream);
+    const unsigned long long key[10] = {4, (unsigned long long)n, (unsigned long long)dB,
+                                        (unsigned long long)ldb, (unsigned long long)dX,
+                                        (unsigned long long)ldx, (unsigned long long)iters,
+                                        (unsigned long long)q};
+    return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
+        return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, nullptr, s);
+    });
 }

@@ -338,6 +338,7 @@ static int mm_part(dplrh_plan *p, size_t blocks)
 {
     const size_t need = blocks * MM_BLK;
     if (need <= p->tmp_cap) return 0;
+    ++p->ws_gen;
     if (p->tmp) (void)hipFree(p->tmp);
     p->tmp = nullptr;
     p->tmp_cap = 0;

@@ -15,6 +15,8 @@
 // Both kernels take a column count: blockIdx.y selects the column, so the
 // same code serves the multi-RHS product for any shape (each column reads
 // the factors once).
+#include <cstring>
+
 #include "plr_common.h"
 
 static constexpr int KG = 16; // rank group; matches plr_plan.hip
@@ -179,6 +181,289 @@ __global__ __launch_bounds__(256) void k_mv_expand(LevelTab a, const double *__r
     }
 }
 
+// ---------------------------------------------------------------------------
+// Fast path for segments that are whole multiples of 64 rows (leaf order a
+// multiple of 64).  Reduction: one wavefront per (level, 16 ranks, block of
+// rows); lane l streams rows 4l..4l+3 of each 256-row slab (two 16-byte
+// loads per rank column, all issued before use).  The 64 lane sums of each
+// rank meet once in LDS; a lane quarter is 64 consecutive rows, so segments
+// of 64, 128 or >= 256 rows come out of the same reduction.  Tiles of long
+// segments write partials that k_mv_finish sums (no fences, no counters).
+
+typedef double d2v __attribute__((ext_vector_type(2)));
+
+struct Mv2Tab
+{
+    int L, N;
+    int R[PLR_MAXL], seg[PLR_MAXL], TW[PLR_MAXL], tps[PLR_MAXL], nkg[PLR_MAXL];
+    int task_off[PLR_MAXL + 1], part_off[PLR_MAXL + 1], fin_off[PLR_MAXL + 1];
+    int w_off[PLR_MAXL + 1];
+    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+};
+
+__device__ __forceinline__ double mv_coef(double t, int raw, double alpha, const double *S,
+                                          int sidx, int R, int k)
+{
+    return raw ? t : alpha * S[(size_t)sidx * R + k] * t;
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
+void k_mv_reduce2(Mv2Tab a, const double *__restrict__ x,
+                                                    int ldx, double alpha, int trans, int raw,
+                                                    double *__restrict__ part, size_t pstride,
+                                                    double *__restrict__ w, size_t wstride)
+{
+    __shared__ double red[4][KG][65];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int task = blockIdx.x * 4 + wv;
+    const int col = blockIdx.y;
+    const bool active = task < a.task_off[a.L];
+    int d = 0;
+    if (active)
+        while (task >= a.task_off[d + 1]) ++d;
+    const int loc = active ? task - a.task_off[d] : 0;
+    const int nkg = a.nkg[d], R = a.R[d], seg = a.seg[d], TW = a.TW[d], N = a.N;
+    const int kg = loc % nkg, blk = loc / nkg;
+    const int k0 = kg * KG, kn = min(KG, R - k0);
+    const int r0 = blk * TW;
+    x += (size_t)col * ldx;
+    part += (size_t)col * pstride;
+    w += (size_t)col * wstride;
+    double acc[KG];
+#pragma unroll
+    for (int kk = 0; kk < KG; ++kk) acc[kk] = 0.0;
+    if (active) {
+        const double *__restrict__ P = a.P[d] + (size_t)k0 * N;
+        for (int rs = r0; rs < r0 + TW; rs += 256) {
+            const int i = rs + 4 * lane;
+            d2v pv[KG][2];
+#pragma unroll
+            for (int kk = 0; kk < KG; ++kk) {
+                const d2v *q = (const d2v *)(P + (size_t)(kk < kn ? kk : kn - 1) * N + i);
+                pv[kk][0] = __builtin_nontemporal_load(q);
+                pv[kk][1] = __builtin_nontemporal_load(q + 1);
+            }
+            const d2v x0 = *(const d2v *)(x + i), x1 = *(const d2v *)(x + i + 2);
+#pragma unroll
+            for (int kk = 0; kk < KG; ++kk)
+                acc[kk] += pv[kk][0].x * x0.x + pv[kk][0].y * x0.y + pv[kk][1].x * x1.x +
+                           pv[kk][1].y * x1.y;
+        }
+    }
+#pragma unroll
+    for (int kk = 0; kk < KG; ++kk) red[wv][kk][lane] = acc[kk];
+    __syncthreads();
+    if (!active) return;
+    // lane (kk, quarter) sums the 16 lanes of one 64-row quarter
+    const int kk = lane & 15, qt = lane >> 4;
+    double sq = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) sq += red[wv][kk][qt * 16 + j];
+    const int kc = k0 + kk;
+    const bool ok = kk < kn;
+    const int sp0 = raw ? 0 : 1; // output segment = s ^ sp0
+    if (seg == 64) {
+        const int s = r0 / 64 + qt;
+        if (ok) w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
+            mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
+        return;
+    }
+    sq += __shfl_xor(sq, 16, 64);
+    if (seg == 128) {
+        const int s = r0 / 128 + (qt >> 1);
+        if (ok && (qt & 1) == 0)
+            w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
+                mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
+        return;
+    }
+    sq += __shfl_xor(sq, 32, 64);
+    if (!ok || qt != 0) return;
+    const int s = r0 / seg;
+    if (a.tps[d] == 1) {
+        w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
+            mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
+    } else {
+        const int tile = (r0 - s * seg) / TW;
+        part[a.part_off[d] + ((size_t)s * a.tps[d] + tile) * R + kc] = sq;
+    }
+}
+
+// Sum the tile partials of long segments: one thread per (level, segment, rank).
+__global__ void k_mv_finish(Mv2Tab a, double alpha, int trans, int raw,
+                            const double *__restrict__ part, size_t pstride,
+                            double *__restrict__ w, size_t wstride)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int col = blockIdx.y;
+    if (e >= a.fin_off[a.L]) return;
+    int d = 0;
+    while (e >= a.fin_off[d + 1]) ++d;
+    const int loc = e - a.fin_off[d], R = a.R[d], tps = a.tps[d];
+    const int s = loc / R, k = loc % R;
+    const double *pp = part + (size_t)col * pstride + a.part_off[d] + (size_t)s * tps * R + k;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j = 0; j < tps; j += 8) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double v = pp[(size_t)min(j + u, tps - 1) * R];
+            acc[u] += (j + u < tps) ? v : 0.0;
+        }
+    }
+    const double t = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    const int sp = raw ? s : (s ^ 1);
+    w[(size_t)col * wstride + a.w_off[d] + (size_t)sp * R + k] =
+        mv_coef(t, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, k);
+}
+
+// Expansion: work-group = 128 rows x 4 rank quarters; lane l holds rows
+// 2l, 2l+1 (16-byte loads), each rank quarter's loads issued in batches of 8.
+__global__ __launch_bounds__(256) void k_mv_expand2(Mv2Tab a, int b, const double *__restrict__ D,
+                                                    const double *__restrict__ x, int ldx,
+                                                    double alpha, double beta, int trans,
+                                                    const double *__restrict__ w, size_t wstride,
+                                                    double *__restrict__ y, int ldy)
+{
+    __shared__ double2 red[3][64];
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = blockIdx.y;
+    x += (size_t)col * ldx;
+    y += (size_t)col * ldy;
+    w += (size_t)col * wstride;
+    const int N = a.N;
+    const int row0 = blockIdx.x * 128;
+    const int i = row0 + 2 * lane;
+    double2 acc = {0.0, 0.0};
+    if (D) {
+        const int leaf = i / b, il = i - leaf * b;
+        const double *__restrict__ Dl = D + (size_t)leaf * b * b;
+        const double *__restrict__ xl = x + (size_t)leaf * b;
+        double2 sl = {0.0, 0.0};
+        if (!trans) {
+            for (int j0 = g; j0 < b; j0 += 32) {
+                double2 dv[8];
+                double xv[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int j = min(j0 + 4 * u, b - 1);
+                    dv[u] = *(const double2 *)(Dl + (size_t)j * b + il);
+                    xv[u] = (j0 + 4 * u < b) ? xl[j] : 0.0;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    sl.x += dv[u].x * xv[u];
+                    sl.y += dv[u].y * xv[u];
+                }
+            }
+        } else {
+            for (int j = g; j < b; j += 4) {
+                sl.x += Dl[(size_t)il * b + j] * xl[j];
+                sl.y += Dl[(size_t)(il + 1) * b + j] * xl[j];
+            }
+        }
+        acc.x = alpha * sl.x;
+        acc.y = alpha * sl.y;
+    }
+    for (int d = 0; d < a.L; ++d) {
+        const int R = a.R[d], seg = a.seg[d];
+        const double *__restrict__ Q = a.Q[d];
+        const double *__restrict__ wb = w + a.w_off[d];
+        const bool uni = (seg & 127) == 0;
+        const int s = uni ? __builtin_amdgcn_readfirstlane(row0 / seg) : i / seg;
+        const double *__restrict__ wd = wb + (size_t)s * R;
+        for (int k0 = g; k0 < R; k0 += 32) {
+            d2v qv[8];
+            double wv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int k = min(k0 + 4 * u, R - 1);
+                qv[u] = __builtin_nontemporal_load((const d2v *)(Q + (size_t)k * N + i));
+                wv[u] = (k0 + 4 * u < R) ? wd[k] : 0.0;
+            }
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                acc.x += qv[u].x * wv[u];
+                acc.y += qv[u].y * wv[u];
+            }
+        }
+    }
+    if (g > 0) red[g - 1][lane] = acc;
+    __syncthreads();
+    if (g == 0) {
+        double2 r;
+        r.x = acc.x + red[0][lane].x + red[1][lane].x + red[2][lane].x;
+        r.y = acc.y + red[0][lane].y + red[1][lane].y + red[2][lane].y;
+        double2 *yp = (double2 *)(y + i);
+        if (beta != 0.0) {
+            const double2 y0 = *yp;
+            r.x += beta * y0.x;
+            r.y += beta * y0.y;
+        }
+        *yp = r;
+    }
+}
+
+static void mv2_tab(const dplrh_plan *p, int trans, Mv2Tab *t)
+{
+    memset(t, 0, sizeof(*t));
+    t->L = p->L;
+    t->N = p->N;
+    int task = 0, part = 0, fin = 0;
+    for (int d = 0; d < p->L; ++d) {
+        const int seg = p->N >> (d + 1), nseg = 2 << d, R = p->R[d];
+        const int TW = seg < 256 ? 256 : (seg < 1024 ? seg : 1024);
+        const int tps = seg <= TW ? 1 : seg / TW;
+        const int nkg = (R + KG - 1) / KG;
+        t->R[d] = R;
+        t->seg[d] = seg;
+        t->TW[d] = TW;
+        t->tps[d] = tps;
+        t->nkg[d] = nkg;
+        t->task_off[d] = task;
+        task += nkg * (p->N / TW);
+        t->part_off[d] = part;
+        if (tps > 1) part += nseg * tps * R;
+        t->fin_off[d] = fin;
+        if (tps > 1) fin += nseg * R;
+        t->w_off[d] = (int)p->soff[d];
+        const double *Ud = p->U + p->uoff[d], *Vd = p->V + p->uoff[d];
+        t->P[d] = trans ? Ud : Vd;
+        t->Q[d] = trans ? Vd : Ud;
+        t->S[d] = p->S + p->soff[d];
+    }
+    t->task_off[p->L] = task;
+    t->part_off[p->L] = part;
+    t->fin_off[p->L] = fin;
+    t->w_off[p->L] = (int)p->soff[p->L];
+}
+
+// Fast-path matvec for nc columns; the partial buffer p->part holds
+// part_per_col doubles per column (sized by plr_workspace for this layout).
+static int mv2_launch(dplrh_plan *p, int trans, int nc, double alpha, double beta,
+                      const double *x, int ldx, double *y, int ldy, hipStream_t st)
+{
+    Mv2Tab t;
+    mv2_tab(p, trans, &t);
+    if ((size_t)t.part_off[t.L] > p->part_per_col) {
+        plr_set_error("matvec workspace too small");
+        return -9;
+    }
+    if (t.task_off[t.L] > 0) {
+        hipLaunchKernelGGL(k_mv_reduce2, dim3((t.task_off[t.L] + 3) / 4, nc), dim3(256), 0, st, t,
+                           x, ldx, alpha, trans, 0, p->part, p->part_per_col, p->w, p->w_per_col);
+        PLR_HIP(hipGetLastError());
+    }
+    if (t.fin_off[t.L] > 0) {
+        hipLaunchKernelGGL(k_mv_finish, dim3((t.fin_off[t.L] + 63) / 64, nc), dim3(64), 0, st, t,
+                           alpha, trans, 0, p->part, p->part_per_col, p->w, p->w_per_col);
+        PLR_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mv_expand2, dim3(p->N / 128, nc), dim3(256), 0, st, t, p->b, p->D, x, ldx,
+                       alpha, beta, trans, p->w, p->w_per_col, y, ldy);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
 int plr_launch_reduce(const LevelTab &t, const double *x, int ldx, double alpha, int trans,
                       int raw, int ncols, double *part, size_t pstride, int *cnt, size_t cstride,
                       double *w, size_t wstride, hipStream_t st)
@@ -214,6 +499,16 @@ int plr_mv_launch(dplrh_plan *p, int trans, int ncols, double alpha, double beta
     const int chunk = ncols < kColChunk ? ncols : kColChunk;
     int rc = plr_workspace(p, chunk);
     if (rc) return rc;
+    const bool fast = plr_mm_ok(p) && !p->force_valu && ldx % 2 == 0 && ldy % 2 == 0;
+    if (fast) {
+        for (int c0 = 0; c0 < ncols; c0 += chunk) {
+            const int nc = ncols - c0 < chunk ? ncols - c0 : chunk;
+            rc = mv2_launch(p, trans, nc, alpha, beta, dx + (size_t)c0 * ldx, ldx,
+                            dy + (size_t)c0 * ldy, ldy, st);
+            if (rc) return rc;
+        }
+        return 0;
+    }
     LevelTab t;
     plr_level_tab(p, trans, &t);
     const int ntask = t.task_off[t.L];
@@ -236,23 +531,37 @@ int plr_mv_launch(dplrh_plan *p, int trans, int ncols, double alpha, double beta
     return 0;
 }
 
+static unsigned long long dbits(double v)
+{
+    unsigned long long u;
+    memcpy(&u, &v, sizeof(u));
+    return u;
+}
+
+static int gemv_body(dplrh_plan *p, int trans, double alpha, double beta, const double *dx,
+                     double *dy, hipStream_t st)
+{
+    if (alpha == 0.0) return dplrh_scale2(p->N, 1, beta, dy, p->N, st);
+    return plr_mv_launch(p, trans, 1, alpha, beta, dx, p->N, dy, p->N, st);
+}
+
 extern "C" int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
                           const double *dx, double *dy, void *stream)
 {
-    hipStream_t st = (hipStream_t)stream;
     if (p->N == 0) return 0;
-    if (alpha == 0.0) return dplrh_scale2(p->N, 1, beta, dy, p->N, stream);
-    return plr_mv_launch(p, trans, 1, alpha, beta, dx, p->N, dy, p->N, st);
+    const unsigned long long key[10] = {1, (unsigned long long)trans, dbits(alpha), dbits(beta),
+                                        (unsigned long long)dx, (unsigned long long)dy};
+    return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
+        return gemv_body(p, trans, alpha, beta, dx, dy, s);
+    });
 }
 
 static constexpr int kMmChunk = 128;
 
-extern "C" int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
-                          const double *dB, int ldb, double *dC, int ldc, void *stream)
+int plr_gemm_direct(dplrh_plan *p, int trans, int n, double alpha, double beta,
+                    const double *dB, int ldb, double *dC, int ldc, hipStream_t st)
 {
-    hipStream_t st = (hipStream_t)stream;
-    if (p->N == 0 || n == 0) return 0;
-    if (alpha == 0.0) return dplrh_scale(p->N, n, beta, dC, ldc, stream);
+    if (alpha == 0.0) return dplrh_scale(p->N, n, beta, dC, ldc, st);
     if (n >= 2 && plr_mm_ok(p) && ldb % 2 == 0 && !p->force_valu) {
         const int chunk = n < kMmChunk ? n : kMmChunk;
         int rc = plr_workspace(p, chunk);
@@ -266,4 +575,17 @@ extern "C" int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double 
         return 0;
     }
     return plr_mv_launch(p, trans, n, alpha, beta, dB, ldb, dC, ldc, st);
+}
+
+extern "C" int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
+                          const double *dB, int ldb, double *dC, int ldc, void *stream)
+{
+    if (p->N == 0 || n == 0) return 0;
+    const unsigned long long key[10] = {2, (unsigned long long)trans, (unsigned long long)n,
+                                        dbits(alpha), dbits(beta), (unsigned long long)dB,
+                                        (unsigned long long)ldb, (unsigned long long)dC,
+                                        (unsigned long long)ldc};
+    return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
+        return plr_gemm_direct(p, trans, n, alpha, beta, dB, ldb, dC, ldc, s);
+    });
 }

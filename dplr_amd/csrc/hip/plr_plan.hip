@@ -2,6 +2,7 @@
 // utility kernels and the C entry points of the HIP layer.
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "plr_common.h"
@@ -97,11 +98,17 @@ int plr_workspace(dplrh_plan *p, int ncols)
     if (ncols <= p->wk_cols) return 0;
     LevelTab t;
     plr_level_tab(p, 0, &t);
-    size_t part = 0;
-    for (int d = 0; d < p->L; ++d)
+    size_t part = 0, part2 = 0;
+    for (int d = 0; d < p->L; ++d) {
         if (t.tps[d] > 1) part += (size_t)(2 << d) * t.tps[d] * t.R[d];
+        // layout of the fast matvec path: tiles of up to 1024 rows
+        const int seg = p->N >> (d + 1);
+        if (seg > 1024) part2 += (size_t)(2 << d) * (seg / 1024) * t.R[d];
+    }
+    if (part2 > part) part = part2;
     const size_t w = p->soff[p->L];
     const size_t cnt = (size_t)t.cnt_off[p->L];
+    ++p->ws_gen;
     if (p->part) (void)hipFree(p->part);
     if (p->w) (void)hipFree(p->w);
     if (p->cnt) (void)hipFree(p->cnt);
@@ -120,6 +127,87 @@ int plr_workspace(dplrh_plan *p, int ncols)
     PLR_HIP(hipMemset(p->cnt, 0, p->cnt_per_col * ncols * sizeof(int)));
     PLR_HIP(hipDeviceSynchronize());
     p->wk_cols = ncols;
+    return 0;
+}
+
+// ---------------------------------------------------------------------------
+// HIP graph cache
+
+static int graphs_enabled()
+{
+    static int on = -1;
+    if (on < 0) {
+        const char *e = getenv("DPLR_GRAPHS");
+        on = (e && e[0] == '0') ? 0 : 1;
+    }
+    return on;
+}
+
+void plr_graphs_clear(dplrh_plan *p)
+{
+    for (auto &g : p->graphs) {
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        memset(&g, 0, sizeof(g));
+    }
+}
+
+int plr_run_cached_impl(dplrh_plan *p, const unsigned long long *key, hipStream_t user,
+                        int (*op)(void *ctx, hipStream_t), void *ctx)
+{
+    if (!graphs_enabled()) return op(ctx, user);
+    ++p->tick;
+    int slot = -1, oldest = 0;
+    for (int i = 0; i < 16; ++i) {
+        if (p->graphs[i].seen && !memcmp(p->graphs[i].key, key, sizeof(p->graphs[i].key))) {
+            slot = i;
+            break;
+        }
+        if (p->graphs[i].tick < p->graphs[oldest].tick) oldest = i;
+    }
+    if (slot >= 0 && p->graphs[slot].gen != p->ws_gen) { // stale: buffers moved
+        if (p->graphs[slot].exec) (void)hipGraphExecDestroy(p->graphs[slot].exec);
+        p->graphs[slot].exec = nullptr;
+        p->graphs[slot].seen = 0;
+        slot = -1;
+    }
+    if (slot < 0) { // first time: run eagerly, remember the call
+        const unsigned gen0 = p->ws_gen;
+        int rc = op(ctx, user);
+        if (rc) return rc;
+        auto &g = p->graphs[oldest];
+        if (g.exec) (void)hipGraphExecDestroy(g.exec);
+        memset(&g, 0, sizeof(g));
+        memcpy(g.key, key, sizeof(g.key));
+        g.seen = 1;
+        g.gen = p->ws_gen;
+        g.tick = p->tick;
+        (void)gen0;
+        return 0;
+    }
+    auto &g = p->graphs[slot];
+    g.tick = p->tick;
+    if (!g.exec) {
+        if (!p->cap) PLR_HIP(hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking));
+        PLR_HIP(hipStreamBeginCapture(p->cap, hipStreamCaptureModeThreadLocal));
+        int rc = op(ctx, p->cap);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(p->cap, &graph);
+        if (rc || e != hipSuccess || !graph || p->ws_gen != g.gen) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipGetLastError();
+            g.seen = 0;
+            return op(ctx, user); // could not capture: run eagerly
+        }
+        e = hipGraphInstantiate(&g.exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        if (e != hipSuccess) {
+            g.exec = nullptr;
+            g.seen = 0;
+            (void)hipGetLastError();
+            return op(ctx, user);
+        }
+    }
+    PLR_HIP(hipGraphLaunch(g.exec, user));
     return 0;
 }
 
@@ -161,6 +249,8 @@ extern "C" int dplrh_plan_create(const plr_flat *f, dplrh_plan **out)
 extern "C" void dplrh_plan_destroy(dplrh_plan *p)
 {
     if (!p) return;
+    plr_graphs_clear(p);
+    if (p->cap) (void)hipStreamDestroy(p->cap);
     void *bufs[] = {p->U, p->V, p->S, p->D, p->rank, p->part, p->w, p->cnt, p->tmp};
     for (void *b : bufs)
         if (b) (void)hipFree(b);

@@ -129,6 +129,137 @@ __global__ __launch_bounds__(256) void k_lu_inverse(const double *LU, const int 
     }
 }
 
+// In-place Gauss-Jordan inverse with partial pivoting in LDS, one
+// work-group per n x n column-major matrix (A + k*stride -> Inv + k*istride).
+// Step k picks the first row i >= k of largest |A[i][k]|: the same pivot
+// sequence as LU with partial pivoting, so info[k] is dgetrf's status (first
+// exactly-zero pivot, 1-based, or 0).  Row interchanges are undone as column
+// interchanges in reverse order at the end.
+__global__ __launch_bounds__(256) void k_gj_inverse(const double *A, int n, size_t stride,
+                                                    double *Inv, size_t istride, int *info)
+{
+    extern __shared__ double a[]; // n*n, column-major
+    __shared__ double sval[256];
+    __shared__ int sidx[256];
+    __shared__ int spiv[256];
+    __shared__ int s_info;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    const double *src = A + (size_t)blockIdx.x * stride;
+    for (int e = tid; e < n * n; e += nt) a[e] = src[e];
+    if (tid == 0) s_info = 0;
+    __syncthreads();
+    for (int k = 0; k < n; ++k) {
+        double best = -1.0;
+        int bi = n;
+        for (int i = k + tid; i < n; i += nt) {
+            const double v = fabs(a[(size_t)k * n + i]);
+            if (v > best) {
+                best = v;
+                bi = i;
+            }
+        }
+        sval[tid] = best;
+        sidx[tid] = bi;
+        __syncthreads();
+        for (int off = nt >> 1; off > 0; off >>= 1) {
+            if (tid < off) {
+                const double v2 = sval[tid + off];
+                const int i2 = sidx[tid + off];
+                if (v2 > sval[tid] || (v2 == sval[tid] && i2 < sidx[tid])) {
+                    sval[tid] = v2;
+                    sidx[tid] = i2;
+                }
+            }
+            __syncthreads();
+        }
+        int p = sidx[0];
+        if (p >= n) p = k;
+        const double pv = a[(size_t)k * n + p];
+        if (tid == 0) {
+            spiv[k] = p;
+            if (pv == 0.0 && s_info == 0) s_info = k + 1;
+        }
+        if (p != k)
+            for (int j = tid; j < n; j += nt) {
+                const double t = a[(size_t)j * n + k];
+                a[(size_t)j * n + k] = a[(size_t)j * n + p];
+                a[(size_t)j * n + p] = t;
+            }
+        __syncthreads();
+        const double inv = 1.0 / a[(size_t)k * n + k];
+        // row k scaled (its diagonal becomes inv below)
+        for (int j = tid; j < n; j += nt)
+            if (j != k) a[(size_t)j * n + k] *= inv;
+        __syncthreads();
+        // eliminate column k from every other row
+        for (int e = tid; e < n * n; e += nt) {
+            const int i = e % n, j = e / n;
+            if (i != k && j != k) a[(size_t)j * n + i] -= a[(size_t)k * n + i] * a[(size_t)j * n + k];
+        }
+        __syncthreads();
+        for (int i = tid; i < n; i += nt)
+            a[(size_t)k * n + i] = (i == k) ? inv : -a[(size_t)k * n + i] * inv;
+        __syncthreads();
+    }
+    for (int k = n - 1; k >= 0; --k) {
+        const int p = spiv[k];
+        if (p != k)
+            for (int i = tid; i < n; i += nt) {
+                const double t = a[(size_t)k * n + i];
+                a[(size_t)k * n + i] = a[(size_t)p * n + i];
+                a[(size_t)p * n + i] = t;
+            }
+        __syncthreads();
+    }
+    double *dst = Inv + (size_t)blockIdx.x * istride;
+    for (int e = tid; e < n * n; e += nt) dst[e] = a[e];
+    if (tid == 0) info[blockIdx.x] = s_info;
+}
+
+// Largest order the LDS Gauss-Jordan kernel takes on this device (0 if the
+// dynamic-LDS limit cannot be raised).
+static int gj_max_order()
+{
+    static int nmax = -1;
+    if (nmax < 0) {
+        int dev = 0, lds = 0;
+        (void)hipGetDevice(&dev);
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
+            lds = 65536;
+        int want = lds - 4096; // room for the static scratch
+        if (want > 65536 &&
+            hipFuncSetAttribute((const void *)k_gj_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                want) != hipSuccess) {
+            (void)hipGetLastError();
+            want = 65536 - 4096;
+        }
+        int n = 0;
+        while ((size_t)(n + 1) * (n + 1) * sizeof(double) <= (size_t)want) ++n;
+        nmax = n;
+    }
+    return nmax;
+}
+
+// inv of `count` n x n matrices; LDS Gauss-Jordan when it fits, else global
+// LU + inverse.  A is overwritten only on the fallback path.
+static int batched_inverse(double *A, int n, int count, double *Inv, int *piv, int *info,
+                           hipStream_t st)
+{
+    if (count <= 0 || n <= 0) return 0;
+    if (n <= gj_max_order()) {
+        hipLaunchKernelGGL(k_gj_inverse, dim3(count), dim3(256), (size_t)n * n * sizeof(double), st,
+                           A, n, (size_t)n * n, Inv, (size_t)n * n, info);
+        PLR_HIP(hipGetLastError());
+        return 0;
+    }
+    hipLaunchKernelGGL(k_lu_batched, dim3(count), dim3(256), 0, st, A, n, (size_t)n * n, piv, info);
+    PLR_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_lu_inverse, dim3(count), dim3(n < 256 ? ((n + 63) & ~63) : 256), 0, st, A,
+                       piv, n, (size_t)n * n, Inv, (size_t)n * n);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
 // Build K = [I, M_bot S_bot; M_top S_top, I] for every node of a level.
 // M_s[k][c] = T[(off + c)*tstride + s*R + k] (output of the raw reduction
 // over the level's own Ut columns).
@@ -201,6 +332,7 @@ static size_t col_off(const dplrh_plan *p, int d) { return p->N ? p->uoff[d] / p
 
 void plr_free_fact(dplrh_plan *p)
 {
+    ++p->ws_gen;
     void *bufs[] = {p->f.Dinv, p->f.Ut, p->f.Kinv, p->f.dinfo, p->f.work,
                     p->sw.T, p->sw.W, p->sw.part, p->sw.tmp, p->sw.cnt};
     for (void *b : bufs)
@@ -229,6 +361,7 @@ static int solve_ws(dplrh_plan *p, int cols)
 {
     plr_solve_ws &w = p->sw;
     if (cols <= w.cols) return 0;
+    ++p->ws_gen;
     void *old[] = {w.T, w.W, w.part, w.cnt};
     for (void *b : old)
         if (b) (void)hipFree(b);
@@ -249,6 +382,7 @@ static int solve_ws(dplrh_plan *p, int cols)
 static int solve_tmp(dplrh_plan *p, size_t elems)
 {
     if (elems <= p->sw.tmp_cap) return 0;
+    ++p->ws_gen;
     if (p->sw.tmp) (void)hipFree(p->sw.tmp);
     p->sw.tmp = nullptr;
     p->sw.tmp_cap = 0;
@@ -311,6 +445,7 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         PLR_HIP(hipMalloc(&f.dinfo, (size_t)maxnodes * sizeof(int)));
     }
     f.done = 0;
+    ++p->ws_gen; // factors are rewritten: graphs replaying solves stay valid, but be safe
     double *work = f.work;
     // pivots live behind the largest LU scratch
     size_t worksz = (size_t)N * b;
@@ -325,12 +460,10 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
 
     // 1. leaves
     PLR_HIP(hipMemcpyAsync(work, p->D, (size_t)N * b * sizeof(double), hipMemcpyDeviceToDevice, st));
-    hipLaunchKernelGGL(k_lu_batched, dim3(nleaf), dim3(256), 0, st, work, b, (size_t)b * b, piv,
-                       f.dinfo);
-    PLR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_lu_inverse, dim3(nleaf), dim3(b < 256 ? ((b + 63) & ~63) : 256), 0, st,
-                       work, piv, b, (size_t)b * b, f.Dinv, (size_t)b * b);
-    PLR_HIP(hipGetLastError());
+    {
+        int rc = batched_inverse(work, b, nleaf, f.Dinv, piv, f.dinfo, st);
+        if (rc) return rc;
+    }
     if (read_info(nleaf, f.dinfo, h, st)) return -10;
     for (int l = 0; l < nleaf; ++l)
         if (h[l]) fails.push_back({l, 0, h[l]});
@@ -371,12 +504,7 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         hipLaunchKernelGGL(k_capf_build, dim3(nodes), dim3(256), 0, st, p->sw.T, p->sw.t_per_col,
                            off, R, Sd, work);
         PLR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_lu_batched, dim3(nodes), dim3(256), 0, st, work, n2,
-                           (size_t)n2 * n2, piv, f.dinfo);
-        PLR_HIP(hipGetLastError());
-        hipLaunchKernelGGL(k_lu_inverse, dim3(nodes), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256), 0,
-                           st, work, piv, n2, (size_t)n2 * n2, f.Kinv + f.koff[d], (size_t)n2 * n2);
-        PLR_HIP(hipGetLastError());
+        if ((rc = batched_inverse(work, n2, nodes, f.Kinv + f.koff[d], piv, f.dinfo, st))) return rc;
         if (read_info(nodes, f.dinfo, h, st)) return -10;
         for (int n = 0; n < nodes; ++n)
             if (h[n]) fails.push_back({((long)(n + 1) << (L - d)) - 1, L - d, h[n]});
@@ -409,14 +537,9 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
 }
 
 // X := op(A)^{-1} B for n device columns (ldb, ldx >= N), after dplrh_factor.
-extern "C" int dplrh_solve(dplrh_plan *p, int trans, int n, const double *dB, int ldb,
-                           double *dX, int ldx, void *stream)
+static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb, double *dX,
+                      int ldx, hipStream_t st)
 {
-    hipStream_t st = (hipStream_t)stream;
-    if (!p->f.done) {
-        plr_set_error("solve before a successful factorization");
-        return -4;
-    }
     const int N = p->N, L = p->L;
     if (N == 0 || n == 0) return 0;
     const int chunk = n < 1024 ? n : 1024;
@@ -493,11 +616,8 @@ int dplrh_lp_lu_inverse(double *K, int n, int count, double *Kinv, int *info, hi
     int *piv = nullptr, *dinfo = nullptr;
     PLR_HIP(hipMalloc(&piv, (size_t)count * n * sizeof(int)));
     PLR_HIP(hipMalloc(&dinfo, (size_t)count * sizeof(int)));
-    hipLaunchKernelGGL(k_lu_batched, dim3(count), dim3(256), 0, st, K, n, (size_t)n * n, piv, dinfo);
-    PLR_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_lu_inverse, dim3(count), dim3(n < 256 ? ((n + 63) & ~63) : 256), 0, st, K,
-                       piv, n, (size_t)n * n, Kinv, (size_t)n * n);
-    PLR_HIP(hipGetLastError());
+    int rc0 = batched_inverse(K, n, count, Kinv, piv, dinfo, st);
+    if (rc0) return rc0;
     std::vector<int> h;
     int rc = read_info(count, dinfo, h, st);
     (void)hipFree(piv);
@@ -509,6 +629,22 @@ int dplrh_lp_lu_inverse(double *K, int n, int count, double *Kinv, int *info, hi
             break;
         }
     return 0;
+}
+
+extern "C" int dplrh_solve(dplrh_plan *p, int trans, int n, const double *dB, int ldb,
+                           double *dX, int ldx, void *stream)
+{
+    if (!p->f.done) {
+        plr_set_error("solve before a successful factorization");
+        return -4;
+    }
+    if (p->N == 0 || n == 0) return 0;
+    const unsigned long long key[10] = {3, (unsigned long long)trans, (unsigned long long)n,
+                                        (unsigned long long)dB, (unsigned long long)ldb,
+                                        (unsigned long long)dX, (unsigned long long)ldx};
+    return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
+        return solve_body(p, trans, n, dB, ldb, dX, ldx, s);
+    });
 }
 
 // ---------------------------------------------------------------------------

@@ -42,9 +42,23 @@ def test_geinv_matches_oracle(cuda, orc, m, nlvl, rank):
     assert rel(D.gepmv("N", B, b, api=orc), D.gepmv("N", Bo, b, api=orc)) < 1e-12
 
 
+def test_geinv_rank0_sibling_is_nan_like_reference(cuda, ref):
+    """eps = 0 and one off-diagonal block of rank 0: the reference divides a
+    zero correction column by its zero norm (src/dplrinv.c:406-421) and
+    returns NaN blocks; the drop-in reproduces that (and the factored solve
+    does not have the problem, see test_gpu_solve.py)."""
+    rng = np.random.default_rng(2)
+    A = PlrTree.from_blocks(512, 3, lambda d, n, w: 0 if (d, n, w) == (1, 1, "sw") else 4, rng)
+    B, info = D.geinv(A, 0.0)
+    Br, info_r = D.geinv(A, 0.0, api=ref)
+    assert info == info_r == 0
+    assert np.array_equal(np.isnan(B.dense()), np.isnan(Br.dense(api=ref)))
+    assert np.isnan(Br.dense(api=ref)).any()
+
+
 def test_geinv_varying_ranks_and_eps(cuda, orc, ref):
     rng = np.random.default_rng(2)
-    A = PlrTree.from_blocks(1024, 4, lambda d, n, w: (3 * d + n + (w == "sw") * 4) % 9, rng)
+    A = PlrTree.from_blocks(1024, 4, lambda d, n, w: 1 + (3 * d + n + (w == "sw") * 4) % 9, rng)
     B, info = D.geinv(A, 0.0)
     Br, info_r = D.geinv(A, 0.0, api=ref)
     assert info == info_r == 0 and ranks(B) == ranks(Br)

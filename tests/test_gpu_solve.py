@@ -55,12 +55,17 @@ def test_solve_matches_oracle_inverse(cuda, orc, m, nlvl, rank, trans):
 
 @pytest.mark.parametrize("nrhs", [1, 64])
 def test_solve_many_rhs_varying_ranks(cuda, orc, nrhs):
+    """Uneven ranks, some blocks of rank 0.  The reference's explicit inverse
+    is NaN on such trees at eps = 0 (a zero correction column is divided by
+    its zero norm, src/dplrinv.c:406-421), so the check is a dense solve."""
     import torch
     rng = np.random.default_rng(2)
     A = PlrTree.from_blocks(1024, 4, lambda d, n, w: (5 * d + 3 * n + (w == "ne") * 7) % 17, rng)
     B = rng.standard_normal((1024, nrhs))
+    Ad = A.dense()
     for trans in ("N", "T"):
-        assert rel(gpu_solve(A, B, trans, torch, cuda), oracle_solve(orc, A, B, trans)) < TOL
+        op = Ad if trans == "N" else Ad.T
+        assert rel(gpu_solve(A, B, trans, torch, cuda), np.linalg.solve(op, B)) < TOL
 
 
 def test_singular_leaf_reports_info(cuda, orc):
