@@ -81,9 +81,11 @@ int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, double *dX, 
  * hBU[d'], hBV[d'] (N*Rt[d'] doubles, [segment][column][row]) and hBsig[d']
  * (segments*Rt[d']), plus the leaves hBD.  Level d' buffers hold the own
  * columns first (R[d']) and the corrections from level d at column
- * coff[d'][d]. */
+ * coff[d'][d].  hnan (2^L ints, may be NULL) flags node (1 << d) - 1 + n when
+ * its Woodbury step normalised a zero column with a zero threshold. */
 int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][DPLRH_MAXL],
-                double **hBU, double **hBV, double **hBsig, double *hBD, void *stream);
+                double **hBU, double **hBV, double **hBsig, double *hBD, int *hnan,
+                void *stream);
 /* Rank-k update data: per level d the k appended columns of every block
  * ([segment][column][row], scaled as dplrgeprk stores them) and their sig;
  * the leaves hD are updated in place. */

@@ -77,9 +77,12 @@ struct InvLevels
 // u and v parts and sig.  Own columns come from OU/VT of level d'; the
 // correction columns from level d < d' use CU_d/VT_d, normalised first over
 // the level-d segment (giving the d2 scale that dplrgeprk receives) and then
-// per block.  lf (level d, segment s, column) is the factor the leaves get.
+// per block.  A column of the true rank whose scale is not finite (a zero
+// norm with a zero threshold: the reference divides 0 by 0 there) flags the
+// node whose Woodbury step made it in nanorg[(1 << depth) - 1 + node].
 __global__ void k_inv_scales(InvLevels a, int dp, const double *sq_ou, const double *sq_cu,
-                             const double *sq_vt, double *gu, double *gv, double *sig, double *lf)
+                             const double *sq_vt, const int *rank, double *gu, double *gv,
+                             double *sig, int *nanorg)
 {
     const int nseg = 2 << dp, Rt = a.Rt[dp];
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
@@ -91,6 +94,8 @@ __global__ void k_inv_scales(InvLevels a, int dp, const double *sq_ou, const dou
         const double f1 = seg_norm(sq_ou, a.nleaf, a.L, col, dp, q);
         const double f2 = seg_norm(sq_vt, a.nleaf, a.L, col, dp, q ^ 1);
         norm_rule(1.0, f1, f2, a.eps2[dp], 1.0, &s_, &g1, &g2);
+        if (c < rank[(2 << dp) - 2 + q] && !(isfinite(g1) && isfinite(g2)))
+            nanorg[(1 << dp) - 1 + (q >> 1)] = 1;
     } else {
         int d = dp - 1;
         while (d >= 0 && !(c >= a.coff[dp][d] && c < a.coff[dp][d] + a.R[d])) --d;
@@ -109,11 +114,12 @@ __global__ void k_inv_scales(InvLevels a, int dp, const double *sq_ou, const dou
         norm_rule(d2, f1, f2, a.eps2[dp], 1.0, &s_, &g1, &g2);
         g1 *= G1;
         g2 *= G2;
+        if (cp < rank[(2 << d) - 2 + (s ^ 1)] && !(isfinite(g1) && isfinite(g2)))
+            nanorg[(1 << d) - 1 + (s >> 1)] = 1;
     }
     gu[e] = g1;
     gv[e] = g2;
     sig[e] = s_;
-    (void)lf;
 }
 
 // Leaf factor of the correction columns of level d: G1 * d2 * G2 per
@@ -121,7 +127,7 @@ __global__ void k_inv_scales(InvLevels a, int dp, const double *sq_ou, const dou
 // true rank k of the correction (the rank of the block on the sibling
 // segment).
 __global__ void k_inv_leaf_factor(InvLevels a, int d, const double *sq_cu, const double *sq_vt,
-                                  const int *rank, double *lf)
+                                  const int *rank, double *lf, int *nanorg)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const int nseg = 2 << d, R = a.R[d];
@@ -135,6 +141,7 @@ __global__ void k_inv_leaf_factor(InvLevels a, int d, const double *sq_cu, const
     const double F2 = seg_norm(sq_vt, a.nleaf, a.L, col, d, s);
     double d2, G1, G2;
     norm_rule(1.0, F1, F2, a.eps2[d], 1.0, &d2, &G1, &G2);
+    if (!(isfinite(G1) && isfinite(G2))) nanorg[(1 << d) - 1 + (s >> 1)] = 1;
     lf[e] = G1 * d2 * G2;
 }
 
@@ -208,9 +215,12 @@ static double eps2_of(int d, int L, double eps)
 // Compute the explicit inverse of a factored plan and copy it to host
 // buffers: for every level d', hBU[d'] and hBV[d'] hold N*Rt[d'] doubles
 // laid out [segment q][column c][row], hBsig[d'] holds nseg*Rt[d'] doubles;
-// hBD holds the N/b leaves (column-major).  Rt and coff as in InvLevels.
+// hBD holds the N/b leaves (column-major); hnan (2^L ints, may be null) the
+// flags of the nodes whose Woodbury step divided 0 by 0.  Rt and coff as in
+// InvLevels.
 extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][DPLRH_MAXL],
-                           double **hBU, double **hBV, double **hBsig, double *hBD, void *stream)
+                           double **hBU, double **hBV, double **hBsig, double *hBD, int *hnan,
+                           void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
     if (!p->f.done) {
@@ -233,6 +243,7 @@ extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][
         a.eps2[d] = eps2_of(d, L, eps);
         for (int e = 0; e < L; ++e) a.coff[d][e] = coff[d][e];
     }
+    if (hnan) memset(hnan, 0, ((size_t)1 << L) * sizeof(int));
     if (L == 0) {
         PLR_HIP(hipMemcpy(hBD, p->f.Dinv, nd * sizeof(double), hipMemcpyDeviceToHost));
         return 0;
@@ -241,6 +252,7 @@ extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][
     double *gu = nullptr, *gv = nullptr, *sg = nullptr, *lf = nullptr, *BU = nullptr, *BV = nullptr;
     double *Wown = nullptr, *Wcor = nullptr, *BD = nullptr;
     size_t *lfoff = nullptr;
+    int *nanorg = nullptr;
     const double **tab = nullptr;
     int rc = 0;
     int maxRt = 1, maxR = 1;
@@ -268,8 +280,10 @@ extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][
         (rc = A((void **)&BV, (size_t)N * maxRt * 8)) ||
         (rc = A((void **)&Wown, ((size_t)2 << (L - 1)) * maxR * maxR * 8)) ||
         (rc = A((void **)&Wcor, ((size_t)2 << (L - 1)) * maxR * maxR * 8)) ||
-        (rc = A((void **)&BD, nd * 8)) || (rc = A((void **)&tab, 2 * (size_t)maxRt * sizeof(double *))))
+        (rc = A((void **)&BD, nd * 8)) || (rc = A((void **)&tab, 2 * (size_t)maxRt * sizeof(double *))) ||
+        (rc = A((void **)&nanorg, ((size_t)1 << L) * sizeof(int))))
         goto done;
+    PLR_HIP(hipMemsetAsync(nanorg, 0, ((size_t)1 << L) * sizeof(int), st));
     PLR_HIP(hipMemcpyAsync(lfoff, hlfoff.data(), L * sizeof(size_t), hipMemcpyHostToDevice, st));
 
     // 1. Vt: V with F_{d'}^T applied for every d' above, then Dinv^T.
@@ -323,7 +337,7 @@ extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][
             const int nl = (2 << d) * p->R[d];
             if (nl == 0) continue;
             hipLaunchKernelGGL(k_inv_leaf_factor, dim3((nl + 255) / 256), dim3(256), 0, st, a, d,
-                               sq_cu, sq_vt, p->rank, lf + hlfoff[d]);
+                               sq_cu, sq_vt, p->rank, lf + hlfoff[d], nanorg);
             PLR_HIP(hipGetLastError());
         }
         // 4. per level: scales, gathered factors, download
@@ -333,7 +347,7 @@ extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][
             if (R == 0) continue;
             const int ns = nseg * R;
             hipLaunchKernelGGL(k_inv_scales, dim3((ns + 255) / 256), dim3(256), 0, st, a, dp, sq_ou,
-                               sq_cu, sq_vt, gu, gv, sg, lf);
+                               sq_cu, sq_vt, p->rank, gu, gv, sg, nanorg);
             PLR_HIP(hipGetLastError());
             for (int c = 0; c < R; ++c) {
                 const double *su = nullptr, *sv = nullptr;
@@ -367,10 +381,14 @@ extern "C" int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][
                            VT, lf, lfoff, BD);
         PLR_HIP(hipGetLastError());
         PLR_HIP(hipMemcpyAsync(hBD, BD, nd * 8, hipMemcpyDeviceToHost, st));
+        if (hnan)
+            PLR_HIP(hipMemcpyAsync(hnan, nanorg, ((size_t)1 << L) * sizeof(int),
+                                   hipMemcpyDeviceToHost, st));
         PLR_HIP(hipStreamSynchronize(st));
     }
 done:
-    void *bufs[] = {VT, Y, OU, CU, sq, gu, gv, sg, lf, lfoff, BU, BV, Wown, Wcor, BD, (void *)tab};
+    void *bufs[] = {VT,   Y,    OU,   CU,  sq, gu,          gv,    sg,
+                    lf,   lfoff, BU,  BV,  Wown, Wcor,      BD,    (void *)tab, nanorg};
     for (void *x : bufs)
         if (x) (void)hipFree(x);
     return rc;

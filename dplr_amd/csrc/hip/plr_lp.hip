@@ -23,6 +23,7 @@
 #include <cstring>
 #include <vector>
 
+#include "plr_cap.h"
 #include "plr_common.h"
 
 typedef float f4 __attribute__((ext_vector_type(4)));
@@ -251,47 +252,6 @@ __global__ __launch_bounds__(256) void k_lp_expand(LpTab a, const T16 *__restric
         }
 }
 
-// Capacitance step: per node and column, rhs = [t_bot; t_top],
-// c = inv(K) rhs, W_top = -sig_top .* c1, W_bot = -sig_bot .* c2, and the
-// power-of-two scale of each segment's coefficients.  KT: FP32 or FP64.
-template <typename KT>
-__global__ __launch_bounds__(64) void k_lp_cap(const KT *__restrict__ Kinv, int R,
-                                               const float *__restrict__ S,
-                                               const float *__restrict__ T, size_t tstride,
-                                               float *__restrict__ W, float *__restrict__ wscale,
-                                               size_t sstride)
-{
-    extern __shared__ double sh[];
-    KT *rhs = (KT *)sh;
-    __shared__ float red[2][64];
-    const int node = blockIdx.x, col = blockIdx.y, n2 = 2 * R, tid = threadIdx.x;
-    const int top = 2 * node, bot = top + 1;
-    const float *t = T + (size_t)col * tstride;
-    float *w = W + (size_t)col * tstride;
-    const KT *k = Kinv + (size_t)node * n2 * n2;
-    for (int j = tid; j < n2; j += 64) rhs[j] = (KT)((j < R) ? t[(size_t)bot * R + j] : t[(size_t)top * R + j - R]);
-    __syncthreads();
-    float mx[2] = {0.f, 0.f};
-    for (int i = tid; i < n2; i += 64) {
-        KT acc = 0;
-        for (int j = 0; j < n2; ++j) acc += k[(size_t)j * n2 + i] * rhs[j];
-        const int sg = i < R ? top : bot, kk = i < R ? i : i - R;
-        const float v = -S[(size_t)sg * R + kk] * (float)acc;
-        w[(size_t)sg * R + kk] = v;
-        mx[i < R ? 0 : 1] = fmaxf(mx[i < R ? 0 : 1], fabsf(v));
-    }
-    red[0][tid] = mx[0];
-    red[1][tid] = mx[1];
-    __syncthreads();
-    if (tid < 2) {
-        float mv = 0.f;
-        for (int j = 0; j < 64; ++j) mv = fmaxf(mv, red[tid][j]);
-        // smallest power of two >= the largest magnitude (1 for an all-zero segment)
-        const float sc = mv > 0.f ? exp2f(ceilf(log2f(mv))) : 1.0f;
-        wscale[(size_t)col * sstride + (tid == 0 ? top : bot)] = sc;
-    }
-}
-
 // K = [I, M_bot S_bot; M_top S_top, I] in FP64 from the FP32 raw reduction.
 __global__ void k_lp_capf_build(const float *T, size_t tstride, int off, int R, const float *S,
                                 double *K)
@@ -371,26 +331,39 @@ __global__ void k_f64_to_f32(const double *src, size_t n, float *dst)
     if (e < n) dst[e] = (float)src[e];
 }
 
-// Per column: scale[c] = max |r[c]| (1 if zero), r32 = r / scale.
-__global__ __launch_bounds__(256) void k_lp_scale_rhs(const double *r, int ldr, int N,
-                                                      double *scale, float *r32)
+// Per column: scale[c] = max |r[c]|, r32 = r / scale (1 if zero).  Two
+// kernels: partial maxima of 64 row chunks per column, then the scaling
+// (each block folds the 64 partials itself).
+static constexpr int LP_SCALE_CHUNKS = 64;
+
+__global__ __launch_bounds__(256) void k_lp_colmax(const double *r, int ldr, int N, double *pmax)
 {
     __shared__ double red[256];
-    const int c = blockIdx.x;
+    const int c = blockIdx.y, ch = blockIdx.x;
+    const int len = (N + LP_SCALE_CHUNKS - 1) / LP_SCALE_CHUNKS;
+    const int beg = ch * len, end = min(N, beg + len);
     const double *rc = r + (size_t)c * ldr;
     double mv = 0.0;
-    for (int i = threadIdx.x; i < N; i += 256) mv = fmax(mv, fabs(rc[i]));
+    for (int i = beg + threadIdx.x; i < end; i += 256) mv = fmax(mv, fabs(rc[i]));
     red[threadIdx.x] = mv;
     __syncthreads();
     for (int off = 128; off > 0; off >>= 1) {
         if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
         __syncthreads();
     }
-    const double sc = red[0] > 0.0 ? red[0] : 1.0;
-    if (threadIdx.x == 0) scale[c] = red[0];
-    const double inv = 1.0 / sc;
-    float *o = r32 + (size_t)c * N;
-    for (int i = threadIdx.x; i < N; i += 256) o[i] = (float)(rc[i] * inv);
+    if (threadIdx.x == 0) pmax[(size_t)c * LP_SCALE_CHUNKS + ch] = red[0];
+}
+
+__global__ __launch_bounds__(256) void k_lp_scale_rhs(const double *r, int ldr, int N,
+                                                      const double *pmax, double *scale, float *r32)
+{
+    const int c = blockIdx.y;
+    double mv = 0.0;
+    for (int j = 0; j < LP_SCALE_CHUNKS; ++j) mv = fmax(mv, pmax[(size_t)c * LP_SCALE_CHUNKS + j]);
+    if (blockIdx.x == 0 && threadIdx.x == 0) scale[c] = mv;
+    const double inv = 1.0 / (mv > 0.0 ? mv : 1.0);
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i < N) r32[(size_t)c * N + i] = (float)(r[(size_t)c * ldr + i] * inv);
 }
 
 // x[c] (+)= scale[c] * z[c]  (FP64 accumulation of the correction; the
@@ -425,14 +398,14 @@ struct plr_lp
     size_t t_per_col, s_per_col;
     float *T, *W, *wscale, *part, *r32, *z32;
     size_t part_cap;
-    double *r, *scale;
+    double *r, *scale, *pmax;
 };
 
 static void lp_free(plr_lp *q)
 {
     if (!q) return;
     void *b[] = {q->Vh, q->Uh, q->Dh, q->Sf, q->K64, q->K32, q->T, q->W, q->wscale, q->part,
-                 q->r32, q->z32, q->r, q->scale};
+                 q->r32, q->z32, q->r, q->scale, q->pmax};
     for (void *x : b)
         if (x) (void)hipFree(x);
     delete q;
@@ -511,11 +484,11 @@ static int lp_ws(const dplrh_plan *p, plr_lp *q, int cols)
 {
     if (cols <= q->cols) return 0;
     ++const_cast<dplrh_plan *>(p)->ws_gen;
-    void *old[] = {q->T, q->W, q->wscale, q->r32, q->z32, q->r, q->scale};
+    void *old[] = {q->T, q->W, q->wscale, q->r32, q->z32, q->r, q->scale, q->pmax};
     for (void *x : old)
         if (x) (void)hipFree(x);
     q->T = q->W = q->wscale = q->r32 = q->z32 = nullptr;
-    q->r = q->scale = nullptr;
+    q->r = q->scale = q->pmax = nullptr;
     q->cols = 0;
     size_t t = 32, s = 2;
     for (int d = 0; d < p->L; ++d) {
@@ -532,6 +505,7 @@ static int lp_ws(const dplrh_plan *p, plr_lp *q, int cols)
     PLR_HIP(hipMalloc(&q->z32, (size_t)p->N * cols * sizeof(float)));
     PLR_HIP(hipMalloc(&q->r, (size_t)p->N * cols * sizeof(double)));
     PLR_HIP(hipMalloc(&q->scale, (size_t)cols * sizeof(double)));
+    PLR_HIP(hipMalloc(&q->pmax, (size_t)cols * LP_SCALE_CHUNKS * sizeof(double)));
     q->cols = cols;
     return 0;
 }
@@ -578,17 +552,14 @@ template <typename T16>
 static int lp_cap(const dplrh_plan *p, plr_lp *q, int d, const float *T, size_t tstride,
                   float *W, int ncols, hipStream_t st)
 {
-    const int Rp = q->Rp[d], n2 = 2 * Rp;
+    (void)p;
+    const int Rp = q->Rp[d];
     const float *Sf = q->Sf + q->soff[d];
-    if (d < 2) {
-        hipLaunchKernelGGL(k_lp_cap<double>, dim3(1 << d, ncols), dim3(64), n2 * sizeof(double),
-                           st, q->K64 + q->koff[d], Rp, Sf, T, tstride, W, q->wscale, q->s_per_col);
-    } else {
-        hipLaunchKernelGGL(k_lp_cap<float>, dim3(1 << d, ncols), dim3(64), n2 * sizeof(float), st,
-                           q->K32 + q->koff[d], Rp, Sf, T, tstride, W, q->wscale, q->s_per_col);
-    }
-    PLR_HIP(hipGetLastError());
-    return 0;
+    if (d < 2)
+        return plr_cap_launch<double, float, true>(q->K64 + q->koff[d], d, Rp, Sf, T, tstride, W,
+                                                   tstride, 0, ncols, q->wscale, q->s_per_col, st);
+    return plr_cap_launch<float, float, true>(q->K32 + q->koff[d], d, Rp, Sf, T, tstride, W, tstride,
+                                              0, ncols, q->wscale, q->s_per_col, st);
 }
 
 
@@ -774,8 +745,10 @@ static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int 
             PLR_HIP(hipMemcpy2DAsync(q->r, (size_t)N * sizeof(double), B, (size_t)ldb * sizeof(double),
                                      (size_t)N * sizeof(double), nc, hipMemcpyDeviceToDevice, st));
             if (it > 0 && (rc = plr_gemm_direct(p, 0, nc, -1.0, 1.0, X, ldx, q->r, N, st))) return rc;
-            hipLaunchKernelGGL(k_lp_scale_rhs, dim3(nc), dim3(256), 0, st, q->r, N, N, q->scale,
-                               q->r32);
+            hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, q->r, N, N,
+                               q->pmax);
+            hipLaunchKernelGGL(k_lp_scale_rhs, dim3((N + 255) / 256, nc), dim3(256), 0, st, q->r, N,
+                               N, q->pmax, q->scale, q->r32);
             PLR_HIP(hipGetLastError());
             if (it == iters) break; // residual of the final iterate only
             rc = q->type == 1 ? lp_solve_t<_Float16>(p, q, nc, st) : lp_solve_t<__bf16>(p, q, nc, st);

@@ -194,7 +194,7 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 
 struct Mv2Tab
 {
-    int L, N;
+    int L, N, nb;
     int R[PLR_MAXL], seg[PLR_MAXL], TW[PLR_MAXL], tps[PLR_MAXL], nkg[PLR_MAXL];
     int task_off[PLR_MAXL + 1], part_off[PLR_MAXL + 1], fin_off[PLR_MAXL + 1];
     int w_off[PLR_MAXL + 1];
@@ -207,84 +207,69 @@ __device__ __forceinline__ double mv_coef(double t, int raw, double alpha, const
     return raw ? t : alpha * S[(size_t)sidx * R + k] * t;
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 2)))
-void k_mv_reduce2(Mv2Tab a, const double *__restrict__ x,
+// Reduction t = P|_s^T x|_s for every level at once: a work-group owns 128
+// rows (lane l holds rows 2l, 2l+1, so x is read once for all levels) and
+// its four waves take the level-wise batches of KG columns round-robin.  The
+// 64-lane sums go through a per-wave LDS transpose; segments of 64 or 128
+// rows are finished here, longer ones leave one partial per 128-row chunk
+// for k_mv_finish.  Every lane issues its KG 16-byte loads back to back.
+__global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__restrict__ x,
                                                     int ldx, double alpha, int trans, int raw,
                                                     double *__restrict__ part, size_t pstride,
                                                     double *__restrict__ w, size_t wstride)
 {
     __shared__ double red[4][KG][65];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    const int task = blockIdx.x * 4 + wv;
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col = blockIdx.y;
-    const bool active = task < a.task_off[a.L];
-    int d = 0;
-    if (active)
-        while (task >= a.task_off[d + 1]) ++d;
-    const int loc = active ? task - a.task_off[d] : 0;
-    const int nkg = a.nkg[d], R = a.R[d], seg = a.seg[d], TW = a.TW[d], N = a.N;
-    const int kg = loc % nkg, blk = loc / nkg;
-    const int k0 = kg * KG, kn = min(KG, R - k0);
-    const int r0 = blk * TW;
+    const int N = a.N, row0 = blockIdx.x * 128, i = row0 + 2 * lane;
     x += (size_t)col * ldx;
     part += (size_t)col * pstride;
     w += (size_t)col * wstride;
-    double acc[KG];
-#pragma unroll
-    for (int kk = 0; kk < KG; ++kk) acc[kk] = 0.0;
-    if (active) {
-        const double *__restrict__ P = a.P[d] + (size_t)k0 * N;
-        for (int rs = r0; rs < r0 + TW; rs += 256) {
-            const int i = rs + 4 * lane;
-            d2v pv[KG][2];
-#pragma unroll
-            for (int kk = 0; kk < KG; ++kk) {
-                const d2v *q = (const d2v *)(P + (size_t)(kk < kn ? kk : kn - 1) * N + i);
-                pv[kk][0] = __builtin_nontemporal_load(q);
-                pv[kk][1] = __builtin_nontemporal_load(q + 1);
-            }
-            const d2v x0 = *(const d2v *)(x + i), x1 = *(const d2v *)(x + i + 2);
-#pragma unroll
-            for (int kk = 0; kk < KG; ++kk)
-                acc[kk] += pv[kk][0].x * x0.x + pv[kk][0].y * x0.y + pv[kk][1].x * x1.x +
-                           pv[kk][1].y * x1.y;
-        }
-    }
-#pragma unroll
-    for (int kk = 0; kk < KG; ++kk) red[wv][kk][lane] = acc[kk];
-    __syncthreads();
-    if (!active) return;
-    // lane (kk, quarter) sums the 16 lanes of one 64-row quarter
-    const int kk = lane & 15, qt = lane >> 4;
-    double sq = 0.0;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) sq += red[wv][kk][qt * 16 + j];
-    const int kc = k0 + kk;
-    const bool ok = kk < kn;
+    const d2v xv = *(const d2v *)(x + i);
     const int sp0 = raw ? 0 : 1; // output segment = s ^ sp0
-    if (seg == 64) {
-        const int s = r0 / 64 + qt;
-        if (ok) w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
-            mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
-        return;
-    }
-    sq += __shfl_xor(sq, 16, 64);
-    if (seg == 128) {
-        const int s = r0 / 128 + (qt >> 1);
-        if (ok && (qt & 1) == 0)
+    int d = 0, bd = 0;           // level of batch bi and its first batch
+    for (int bi = g; bi < a.nb; bi += 4) {
+        while (bi >= bd + a.nkg[d]) bd += a.nkg[d++];
+        const int R = a.R[d], k0 = (bi - bd) * KG, kn = min(KG, R - k0);
+        const double *__restrict__ P = a.P[d] + (size_t)k0 * N + i;
+        d2v pv[KG];
+#pragma unroll
+        for (int kk = 0; kk < KG; ++kk)
+            pv[kk] = __builtin_nontemporal_load((const d2v *)(P + (size_t)(kk < kn ? kk : kn - 1) * N));
+#pragma unroll
+        for (int kk = 0; kk < KG; ++kk) red[g][kk][lane] = pv[kk].x * xv.x + pv[kk].y * xv.y;
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        // lane (kk, quarter) sums the 16 lanes (32 rows) of one quarter
+        const int kk = lane & 15, qt = lane >> 4;
+        double sq = 0.0;
+#pragma unroll
+        for (int j = 0; j < 16; ++j) sq += red[g][kk][qt * 16 + j];
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        const int kc = k0 + kk, seg = a.seg[d];
+        const bool ok = kk < kn;
+        sq += __shfl_xor(sq, 16, 64); // 64 rows
+        if (seg == 64) {
+            const int s = row0 / 64 + (qt >> 1);
+            if (ok && (qt & 1) == 0)
+                w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
+                    mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
+            continue;
+        }
+        sq += __shfl_xor(sq, 32, 64); // 128 rows
+        if (!ok || qt != 0) continue;
+        const int s = row0 / seg;
+        if (seg == 128) {
             w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
                 mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
-        return;
-    }
-    sq += __shfl_xor(sq, 32, 64);
-    if (!ok || qt != 0) return;
-    const int s = r0 / seg;
-    if (a.tps[d] == 1) {
-        w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
-            mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
-    } else {
-        const int tile = (r0 - s * seg) / TW;
-        part[a.part_off[d] + ((size_t)s * a.tps[d] + tile) * R + kc] = sq;
+        } else {
+            const int tile = (row0 - s * seg) / 128;
+            part[a.part_off[d] + ((size_t)s * a.tps[d] + tile) * R + kc] = sq;
+        }
     }
 }
 
@@ -408,19 +393,18 @@ static void mv2_tab(const dplrh_plan *p, int trans, Mv2Tab *t)
     memset(t, 0, sizeof(*t));
     t->L = p->L;
     t->N = p->N;
-    int task = 0, part = 0, fin = 0;
+    int nb = 0, part = 0, fin = 0;
     for (int d = 0; d < p->L; ++d) {
         const int seg = p->N >> (d + 1), nseg = 2 << d, R = p->R[d];
-        const int TW = seg < 256 ? 256 : (seg < 1024 ? seg : 1024);
-        const int tps = seg <= TW ? 1 : seg / TW;
+        const int tps = seg <= 128 ? 1 : seg / 128;
         const int nkg = (R + KG - 1) / KG;
         t->R[d] = R;
         t->seg[d] = seg;
-        t->TW[d] = TW;
+        t->TW[d] = 128;
         t->tps[d] = tps;
         t->nkg[d] = nkg;
-        t->task_off[d] = task;
-        task += nkg * (p->N / TW);
+        t->task_off[d] = nb;
+        nb += nkg;
         t->part_off[d] = part;
         if (tps > 1) part += nseg * tps * R;
         t->fin_off[d] = fin;
@@ -431,7 +415,8 @@ static void mv2_tab(const dplrh_plan *p, int trans, Mv2Tab *t)
         t->Q[d] = trans ? Vd : Ud;
         t->S[d] = p->S + p->soff[d];
     }
-    t->task_off[p->L] = task;
+    t->nb = nb;
+    t->task_off[p->L] = nb;
     t->part_off[p->L] = part;
     t->fin_off[p->L] = fin;
     t->w_off[p->L] = (int)p->soff[p->L];
@@ -448,9 +433,9 @@ static int mv2_launch(dplrh_plan *p, int trans, int nc, double alpha, double bet
         plr_set_error("matvec workspace too small");
         return -9;
     }
-    if (t.task_off[t.L] > 0) {
-        hipLaunchKernelGGL(k_mv_reduce2, dim3((t.task_off[t.L] + 3) / 4, nc), dim3(256), 0, st, t,
-                           x, ldx, alpha, trans, 0, p->part, p->part_per_col, p->w, p->w_per_col);
+    if (t.nb > 0) {
+        hipLaunchKernelGGL(k_mv_reduce2, dim3(p->N / 128, nc), dim3(256), 0, st, t, x, ldx, alpha,
+                           trans, 0, p->part, p->part_per_col, p->w, p->w_per_col);
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[t.L] > 0) {

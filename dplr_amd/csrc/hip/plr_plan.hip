@@ -101,9 +101,9 @@ int plr_workspace(dplrh_plan *p, int ncols)
     size_t part = 0, part2 = 0;
     for (int d = 0; d < p->L; ++d) {
         if (t.tps[d] > 1) part += (size_t)(2 << d) * t.tps[d] * t.R[d];
-        // layout of the fast matvec path: tiles of up to 1024 rows
+        // layout of the fast matvec path: one partial per 128-row chunk
         const int seg = p->N >> (d + 1);
-        if (seg > 1024) part2 += (size_t)(2 << d) * (seg / 1024) * t.R[d];
+        if (seg > 128) part2 += (size_t)(2 << d) * (seg / 128) * t.R[d];
     }
     if (part2 > part) part = part2;
     const size_t w = p->soff[p->L];

@@ -22,6 +22,7 @@
 #include <cstring>
 #include <vector>
 
+#include "plr_cap.h"
 #include "plr_common.h"
 
 // ---------------------------------------------------------------------------
@@ -280,49 +281,13 @@ __global__ void k_capf_build(const double *T, size_t tstride, int off, int R,
     }
 }
 
-// Capacitance step of the solve for one level: per node and column,
-// rhs from the raw reduction T, c = inv(K) rhs (or inv(K)^T rhs), then the
-// negated expansion coefficients W (layout of T).
-//   trans == 0: rhs = [t_bot; t_top],           W_top = -S_top.*c1, W_bot = -S_bot.*c2
-//   trans == 1: rhs = [S_top.*t_top; S_bot.*t_bot], W_top = -c2,     W_bot = -c1
-__global__ __launch_bounds__(256) void k_cap_apply(const double *Kinv, int R, const double *S,
-                                                   const double *T, size_t tstride, double *W,
-                                                   size_t wstride, int trans)
+// Capacitance step of the solve for one level (the kernel is in plr_cap.h):
+// T is the raw reduction, W the negated expansion coefficients (layout of T).
+static int launch_cap(const double *Kinv, int d, int R, const double *S, const double *T,
+                      size_t tstride, double *W, int ncols, int trans, hipStream_t st)
 {
-    extern __shared__ double rhs[];
-    const int node = blockIdx.x, col = blockIdx.y, n2 = 2 * R;
-    const int top = 2 * node, bot = 2 * node + 1;
-    const double *t = T + (size_t)col * tstride;
-    double *w = W + (size_t)col * wstride;
-    const double *k = Kinv + (size_t)node * n2 * n2;
-    for (int j = threadIdx.x; j < n2; j += blockDim.x) {
-        double v;
-        if (!trans)
-            v = (j < R) ? t[(size_t)bot * R + j] : t[(size_t)top * R + j - R];
-        else
-            v = (j < R) ? S[(size_t)top * R + j] * t[(size_t)top * R + j]
-                        : S[(size_t)bot * R + j - R] * t[(size_t)bot * R + j - R];
-        rhs[j] = v;
-    }
-    __syncthreads();
-    for (int i = threadIdx.x; i < n2; i += blockDim.x) {
-        double acc = 0.0;
-        if (!trans)
-            for (int j = 0; j < n2; ++j) acc += k[(size_t)j * n2 + i] * rhs[j];
-        else
-            for (int j = 0; j < n2; ++j) acc += k[(size_t)i * n2 + j] * rhs[j];
-        if (!trans) {
-            if (i < R)
-                w[(size_t)top * R + i] = -S[(size_t)top * R + i] * acc;
-            else
-                w[(size_t)bot * R + i - R] = -S[(size_t)bot * R + i - R] * acc;
-        } else {
-            if (i < R)
-                w[(size_t)bot * R + i] = -acc;
-            else
-                w[(size_t)top * R + i - R] = -acc;
-        }
-    }
+    return plr_cap_launch<double, double, false>(Kinv, d, R, S, T, tstride, W, tstride, trans,
+                                                 ncols, nullptr, 0, st);
 }
 
 // ---------------------------------------------------------------------------
@@ -509,10 +474,9 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         for (int n = 0; n < nodes; ++n)
             if (h[n]) fails.push_back({((long)(n + 1) << (L - d)) - 1, L - d, h[n]});
         if (off > 0) {
-            hipLaunchKernelGGL(k_cap_apply, dim3(nodes, off), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256),
-                               n2 * sizeof(double), st, f.Kinv + f.koff[d], R, Sd, p->sw.T,
-                               p->sw.t_per_col, p->sw.W, p->sw.t_per_col, 0);
-            PLR_HIP(hipGetLastError());
+            if ((rc = launch_cap(f.Kinv + f.koff[d], d, R, Sd, p->sw.T, p->sw.t_per_col, p->sw.W, off,
+                                 0, st)))
+                return rc;
             if (mm)
                 rc = plr_mm_expand1(p, d, Ud, p->sw.W, p->sw.t_per_col, nullptr, 0, nullptr, N,
                                     1.0, 1.0, f.Ut, N, off, st);
@@ -583,11 +547,9 @@ static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb
                 rc = plr_launch_reduce(t1, Z, ldz, 1.0, 0, 1, nc, p->sw.part, p->sw.part_per_col,
                                        p->sw.cnt, p->sw.cnt_per_col, p->sw.T, p->sw.t_per_col, st);
             if (rc) return rc;
-            const int n2 = 2 * R;
-            hipLaunchKernelGGL(k_cap_apply, dim3(1 << d, nc), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256),
-                               n2 * sizeof(double), st, p->f.Kinv + p->f.koff[d], R, Sd, p->sw.T,
-                               p->sw.t_per_col, p->sw.W, p->sw.t_per_col, trans);
-            PLR_HIP(hipGetLastError());
+            if ((rc = launch_cap(p->f.Kinv + p->f.koff[d], d, R, Sd, p->sw.T, p->sw.t_per_col,
+                                 p->sw.W, nc, trans, st)))
+                return rc;
             if (mm)
                 rc = plr_mm_expand1(p, d, trans ? Vd : Ud, p->sw.W, p->sw.t_per_col, nullptr, 0,
                                     nullptr, ldz, 1.0, 1.0, Z, ldz, nc, st);
@@ -687,13 +649,10 @@ int plr_lvl_expand(dplrh_plan *p, int d, const double *Q, const double *W, size_
 int plr_lvl_cap(dplrh_plan *p, int d, const double *T, size_t tstride, double *W, int ncols,
                 int trans, hipStream_t st)
 {
-    const int R = p->R[d], n2 = 2 * R;
+    const int R = p->R[d];
     if (ncols <= 0 || R == 0) return 0;
-    hipLaunchKernelGGL(k_cap_apply, dim3(1 << d, ncols), dim3(n2 < 256 ? ((n2 + 63) & ~63) : 256),
-                       n2 * sizeof(double), st, p->f.Kinv + p->f.koff[d], R, p->S + p->soff[d], T,
-                       tstride, W, tstride, trans);
-    PLR_HIP(hipGetLastError());
-    return 0;
+    return launch_cap(p->f.Kinv + p->f.koff[d], d, R, p->S + p->soff[d], T, tstride, W, ncols,
+                      trans, st);
 }
 
 int plr_solve_reserve(dplrh_plan *p, int cols) { return solve_ws(p, cols); }

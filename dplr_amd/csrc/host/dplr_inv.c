@@ -10,6 +10,7 @@
  * exactly when the reference does.  Column order per block is therefore
  * [own | correction from the parent | from the grandparent | ...].
  */
+#include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -115,6 +116,44 @@ static void fill_inverse(DPlrMtx *B, const inv_ctx *c, int d, int n)
     fill_inverse(B->dh.se, c, d + 1, 2 * n + 1);
 }
 
+static void nan_fill(DPlrMtx *X)
+{
+    if (X->type == PLR_DMTX) {
+        for (size_t i = 0; i < (size_t)X->size * X->size; ++i) X->dd.d[i] = NAN;
+        return;
+    }
+    DPlrMtx *lr[2] = {X->dh.ne, X->dh.sw};
+    for (int i = 0; i < 2; ++i) {
+        const size_t n = (size_t)lr[i]->size * lr[i]->dlr.rank;
+        for (size_t j = 0; j < n; ++j) lr[i]->dlr.u[j] = lr[i]->dlr.v[j] = NAN;
+        for (int j = 0; j < lr[i]->dlr.rank; ++j) lr[i]->dlr.sig[j] = NAN;
+    }
+    nan_fill(X->dh.nw);
+    nan_fill(X->dh.se);
+}
+
+/* With a zero threshold the reference normalises a zero column by dividing
+ * by its zero norm (src/dplrinv.c:333-348, :378-404), and the NaN it leaves
+ * in a subtree reaches every later Woodbury step that reads that subtree
+ * (src/dplrinv.c:245-254 apply the subtree inverses to U and V): the whole
+ * subtree of the shallowest ancestor that performs one becomes NaN.  The
+ * GPU values already hold the NaN of the flagged node itself. */
+static void nan_spread(DPlrMtx *B, const plr_flat *f, const int *flag)
+{
+    for (int d = 1; d < f->L; ++d)
+        for (int n = 0; n < (1 << d); ++n) {
+            if (!flag[(1 << d) - 1 + n]) continue;
+            for (int da = 0; da < d; ++da) {
+                const int na = n >> (d - da);
+                if (rank_at(f, da, 2 * na) + rank_at(f, da, 2 * na + 1) == 0) continue;
+                DPlrMtx *X = B;
+                for (int k = da - 1; k >= 0; --k) X = ((na >> k) & 1) ? X->dh.se : X->dh.nw;
+                nan_fill(X);
+                break;
+            }
+        }
+}
+
 /* reference src/dplrinv.c:93 */
 void dplrgeinv(int m, DPlrMtx *A, DPlrMtx *B, double eps, int *info)
 {
@@ -137,6 +176,7 @@ void dplrgeinv(int m, DPlrMtx *A, DPlrMtx *B, double eps, int *info)
     inv_ctx c;
     memset(&c, 0, sizeof(c));
     c.f = &f;
+    int *nanflag = NULL;
     int rc = dplrh_plan_create(&f, &p), finfo = 0;
     if (!rc && m > 0) rc = dplrh_factor(p, &finfo, NULL);
     if (rc) {
@@ -165,19 +205,23 @@ void dplrgeinv(int m, DPlrMtx *A, DPlrMtx *B, double eps, int *info)
         }
     }
     c.BD = malloc(((size_t)f.N * f.b + 1) * sizeof(double));
-    if (!c.BD) {
+    nanflag = calloc((size_t)1 << f.L, sizeof(int));
+    if (!c.BD || !nanflag) {
         fprintf(stderr, "%s: host memory allocation failed\n", __func__);
         goto out;
     }
     if (m > 0) {
-        rc = dplrh_geinv(p, eps, c.Rt, c.coff, c.BU, c.BV, c.Bsig, c.BD, NULL);
+        rc = dplrh_geinv(p, eps, c.Rt, c.coff, c.BU, c.BV, c.Bsig, c.BD, nanflag, NULL);
         if (rc) {
             fprintf(stderr, "%s: GPU execution failed (%d): %s\n", __func__, rc, dplrh_last_error());
             goto out;
         }
     }
     shape_inverse(A, B, m, 0);
-    if (m > 0) fill_inverse(B, &c, 0, 0);
+    if (m > 0) {
+        fill_inverse(B, &c, 0, 0);
+        nan_spread(B, &f, nanflag);
+    }
     *info = 0;
 out:
     for (int d = 0; d < DPLRH_MAXL; ++d) {
@@ -186,6 +230,7 @@ out:
         free(c.Bsig[d]);
     }
     free(c.BD);
+    free(nanflag);
     dplrh_plan_destroy(p);
     plr_flat_free(&f);
 }

@@ -44,9 +44,10 @@ def test_geinv_matches_oracle(cuda, orc, m, nlvl, rank):
 
 def test_geinv_rank0_sibling_is_nan_like_reference(cuda, ref):
     """eps = 0 and one off-diagonal block of rank 0: the reference divides a
-    zero correction column by its zero norm (src/dplrinv.c:406-421) and
-    returns NaN blocks; the drop-in reproduces that (and the factored solve
-    does not have the problem, see test_gpu_solve.py)."""
+    zero correction column by its zero norm (src/dplrinv.c:378-404) and the
+    NaN spreads through the root's Woodbury step to the whole matrix; the
+    drop-in reproduces that (the factored solve does not have the problem,
+    see test_gpu_solve.py)."""
     rng = np.random.default_rng(2)
     A = PlrTree.from_blocks(512, 3, lambda d, n, w: 0 if (d, n, w) == (1, 1, "sw") else 4, rng)
     B, info = D.geinv(A, 0.0)
@@ -63,14 +64,21 @@ def test_geinv_varying_ranks_and_eps(cuda, orc, ref):
     Br, info_r = D.geinv(A, 0.0, api=ref)
     assert info == info_r == 0 and ranks(B) == ranks(Br)
     assert rel(B.dense(), Br.dense(api=ref)) < 1e-12
-    # eps > 0: sig below the per-level share of eps is zeroed (reference rule)
+    # eps > 0: sig below the per-level share of eps is zeroed (reference rule).
+    # The reference builds each level from the already truncated subtree
+    # inverses, the GPU from the exact factored inverse, so the two differ by
+    # a small fraction of the truncation error itself (1.5e-5 here).
     B1, _ = D.geinv(A, 1e-3)
     B2, _ = D.geinv(A, 1e-3, api=ref)
     assert ranks(B1) == ranks(B2)
     z1 = sum(int(np.sum(B1.lr_arrays(nd)[1] == 0)) for (_, _, _, nd) in B1.blocks())
     z2 = sum(int(np.sum(B2.lr_arrays(nd)[1] == 0)) for (_, _, _, nd) in B2.blocks())
     assert z1 == z2 and z1 > 0
-    assert rel(B1.dense(), B2.dense(api=ref)) < 1e-9
+    Ai = np.linalg.inv(A.dense(api=ref))
+    e_gpu, e_ref = rel(B1.dense(), Ai), rel(B2.dense(api=ref), Ai)
+    assert 1e-7 < e_ref < 1e-4
+    assert e_gpu <= 1.1 * e_ref
+    assert rel(B1.dense(), B2.dense(api=ref)) < 0.05 * e_ref
 
 
 def test_geinv_reuses_existing_b(cuda, orc):
