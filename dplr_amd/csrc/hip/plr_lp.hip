@@ -93,34 +93,43 @@ __global__ __launch_bounds__(64) void k_lp_reduce(LpTab a, const float *__restri
     const T16 *P = (const T16 *)a.P[d];
     const T16 *pa[2];
     const float *pb[4];
-    bool vb[4];
+    float vb[4]; // 1 for a real column, 0 for padding (multiplied, not selected)
 #pragma unroll
     for (int kt = 0; kt < 2; ++kt) pa[kt] = P + (size_t)(k0 + 16 * kt + m) * N;
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
         const int c = c0 + 16 * ct + m;
-        vb[ct] = c < a.ncols;
-        pb[ct] = Z + (size_t)(vb[ct] ? c : 0) * ldz;
+        vb[ct] = c < a.ncols ? 1.f : 0.f;
+        pb[ct] = Z + (size_t)(c < a.ncols ? c : 0) * ldz;
     }
     f4 acc[2][4];
 #pragma unroll
     for (int kt = 0; kt < 2; ++kt)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = f4{0.f, 0.f, 0.f, 0.f};
-    for (int rb = rbeg; rb < rend; rb += 32) {
-        const int r = rb + 8 * g;
-        v8 av[2], bv[4];
+    // four 32-row steps per batch, every load of the batch issued first
+    // (segments are whole multiples of 64 rows; a step past rend is masked)
+    for (int rb = rbeg; rb < rend; rb += 128) {
+        v8 av[4][2];
+        f8 zv[4][4];
 #pragma unroll
-        for (int kt = 0; kt < 2; ++kt) av[kt] = *(const v8 *)(pa[kt] + r);
+        for (int u = 0; u < 4; ++u) {
+            const int r = min(rb + 32 * u, rend - 32) + 8 * g;
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) {
-            f8 z = *(const f8 *)(pb[ct] + r);
-            bv[ct] = cvt8<T16>(vb[ct] ? z : f8{0, 0, 0, 0, 0, 0, 0, 0}, 1.0f);
+            for (int kt = 0; kt < 2; ++kt) av[u][kt] = *(const v8 *)(pa[kt] + r);
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) zv[u][ct] = *(const f8 *)(pb[ct] + r);
         }
 #pragma unroll
-        for (int kt = 0; kt < 2; ++kt)
+        for (int u = 0; u < 4; ++u) {
+            const float on = rb + 32 * u < rend ? 1.f : 0.f;
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = Lp<T16>::mfma(av[kt], bv[ct], acc[kt][ct]);
+            for (int ct = 0; ct < 4; ++ct) {
+                const v8 bv = cvt8<T16>(zv[u][ct], vb[ct] * on);
+#pragma unroll
+                for (int kt = 0; kt < 2; ++kt) acc[kt][ct] = Lp<T16>::mfma(av[u][kt], bv, acc[kt][ct]);
+            }
+        }
     }
     // lane holds T[k0 + 16kt + 4g + v][c0 + 16ct + m]
     if (tps == 1) {
@@ -145,32 +154,42 @@ __global__ __launch_bounds__(64) void k_lp_reduce(LpTab a, const float *__restri
             for (int v = 0; v < 4; ++v) pp[(16 * ct + m) * LP_KG + 16 * kt + 4 * g + v] = acc[kt][ct][v];
 }
 
+// Sum of the tile partials (fixed order): one work-group per (level,
+// segment, rank group, column group, 64-element chunk); four waves take every
+// fourth tile, eight loads in flight, combined through LDS.
+static constexpr int LP_FCH = LP_BLK / 64;
+
 __global__ __launch_bounds__(256) void k_lp_finish(LpTab a, const float *__restrict__ part,
                                                    float *__restrict__ out, size_t ostride)
 {
-    const int task = blockIdx.x;
+    __shared__ float red[4][64];
+    const int task = blockIdx.x / LP_FCH, chunk = blockIdx.x % LP_FCH;
     int d = 0;
     while (task >= a.fin_off[d + 1]) ++d;
     const int loc = task - a.fin_off[d];
     const int nkg = a.nkg[d], ncg = a.ncg, tps = a.tps[d], R = a.R[d];
     const int kg = loc % nkg, cg = (loc / nkg) % ncg, s = loc / (nkg * ncg);
-    for (int e = threadIdx.x; e < LP_BLK; e += blockDim.x) {
-        const int kl = e % LP_KG, cl = e / LP_KG;
-        const int c = cg * LP_CG + cl;
-        if (c >= a.ncols) continue;
-        const float *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * LP_BLK + e;
-        const size_t step = (size_t)nkg * ncg * LP_BLK;
-        float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int j = 0; j < tps; j += 8) {
+    const int el = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int e = chunk * 64 + el;
+    const float *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * LP_BLK + e;
+    const size_t step = (size_t)nkg * ncg * LP_BLK;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j0 = ph; j0 < tps; j0 += 32) {
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const float v = pp[(size_t)min(j + u, tps - 1) * step];
-                acc[u] += (j + u < tps) ? v : 0.f;
-            }
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + 4 * u;
+            const float v = pp[(size_t)min(j, tps - 1) * step];
+            acc[u] += j < tps ? v : 0.f;
         }
-        const float t = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-        out[(size_t)c * ostride + (size_t)s * R + kg * LP_KG + kl] = t;
     }
+    red[ph][el] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (ph != 0) return;
+    const float t = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
+    const int kl = e % LP_KG, cl = e / LP_KG;
+    const int c = cg * LP_CG + cl;
+    if (c >= a.ncols) return;
+    out[(size_t)c * ostride + (size_t)s * R + kg * LP_KG + kl] = t;
 }
 
 // Z^T (64 columns x 16 rows per wavefront): Z := beta*Z + Dinv X (leaf, when
@@ -188,14 +207,20 @@ __global__ __launch_bounds__(256) void k_lp_expand(LpTab a, const T16 *__restric
     const int c0 = blockIdx.y * LP_CG;
     const int N = a.N, b = a.b;
     if (r0 >= N) return;
+    // Every load is unconditional (clamped index) and padding columns are
+    // zeroed by a 0/1 factor, so the loads of one level go out together.
     int cc[4];
-    bool vc[4];
+    float vc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) {
-        cc[ct] = c0 + 16 * ct + m;
-        vc[ct] = cc[ct] < a.ncols;
-        if (!vc[ct]) cc[ct] = 0;
+        cc[ct] = min(c0 + 16 * ct + m, a.ncols - 1);
+        vc[ct] = c0 + 16 * ct + m < a.ncols ? 1.f : 0.f;
     }
+    int cd[4][4]; // columns of the D rows this lane holds: c0 + 16ct + 4g + v
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) cd[ct][v] = min(c0 + 16 * ct + 4 * g + v, a.ncols - 1);
     f4 acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) acc[ct] = f4{0.f, 0.f, 0.f, 0.f};
@@ -204,51 +229,55 @@ __global__ __launch_bounds__(256) void k_lp_expand(LpTab a, const T16 *__restric
         const T16 *D = Dl + (size_t)leaf * b * b; // row-major
         for (int j0 = 0; j0 < b; j0 += 32) {
             const v8 bq = *(const v8 *)(D + (size_t)(il + m) * b + j0 + 8 * g);
+            f8 x[4];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                f8 x = *(const f8 *)(X + (size_t)cc[ct] * ldx + (size_t)leaf * b + j0 + 8 * g);
-                const v8 av = cvt8<T16>(vc[ct] ? x : f8{0, 0, 0, 0, 0, 0, 0, 0}, 1.0f);
-                acc[ct] = Lp<T16>::mfma(av, bq, acc[ct]);
-            }
+            for (int ct = 0; ct < 4; ++ct)
+                x[ct] = *(const f8 *)(X + (size_t)cc[ct] * ldx + (size_t)leaf * b + j0 + 8 * g);
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[ct] = Lp<T16>::mfma(cvt8<T16>(x[ct], vc[ct]), bq, acc[ct]);
         }
     }
     for (int d = 0; d < a.L; ++d) {
         const int R = a.R[d], s = r0 / a.seglen[d];
         const T16 *Q = (const T16 *)a.Q[d];
         const float *wd = W + (size_t)s * R;
-        float isc[4];
+        float isc[4], sc[4][4];
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) isc[ct] = 1.0f / wscale[(size_t)cc[ct] * sstride + s];
+        for (int ct = 0; ct < 4; ++ct) isc[ct] = wscale[(size_t)cc[ct] * sstride + s];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sc[ct][v] = wscale[(size_t)cd[ct][v] * sstride + s];
         f4 al[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) al[ct] = f4{0.f, 0.f, 0.f, 0.f};
         for (int k0 = 0; k0 < R; k0 += 32) {
             const v8 bq = *(const v8 *)(Q + (size_t)(r0 + m) * R + k0 + 8 * g);
+            f8 w[4];
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                f8 w = *(const f8 *)(wd + (size_t)cc[ct] * wstride + k0 + 8 * g);
-                const v8 av = cvt8<T16>(vc[ct] ? w : f8{0, 0, 0, 0, 0, 0, 0, 0}, isc[ct]);
-                al[ct] = Lp<T16>::mfma(av, bq, al[ct]);
-            }
+            for (int ct = 0; ct < 4; ++ct) w[ct] = *(const f8 *)(wd + (size_t)cc[ct] * wstride + k0 + 8 * g);
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+                al[ct] = Lp<T16>::mfma(cvt8<T16>(w[ct], vc[ct] / isc[ct]), bq, al[ct]);
         }
         // D row 4g+v is column c0 + 16ct + 4g + v: undo that column's scale
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = c0 + 16 * ct + 4 * g + v;
-                const float sc = c < a.ncols ? wscale[(size_t)c * sstride + s] : 0.f;
-                acc[ct][v] += al[ct][v] * sc;
-            }
+            for (int v = 0; v < 4; ++v) acc[ct][v] += al[ct][v] * sc[ct][v];
     }
+    float zo[4][4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v)
+            zo[ct][v] = beta == 0.f ? 0.f : beta * Z[(size_t)cd[ct][v] * ldz + r0 + m];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = c0 + 16 * ct + 4 * g + v;
-            if (c >= a.ncols) continue;
-            float *z = Z + (size_t)c * ldz + r0 + m;
-            *z = (beta == 0.f ? 0.f : beta * *z) + acc[ct][v];
+            if (c < a.ncols) Z[(size_t)c * ldz + r0 + m] = zo[ct][v] + acc[ct][v];
         }
 }
 
@@ -524,7 +553,7 @@ static int lp_reduce(const dplrh_plan *p, plr_lp *q, int d, const float *Z, int 
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[1] > 0) {
-        hipLaunchKernelGGL(k_lp_finish, dim3(t.fin_off[1]), dim3(256), 0, st, t, q->part, out,
+        hipLaunchKernelGGL(k_lp_finish, dim3(t.fin_off[1] * LP_FCH), dim3(256), 0, st, t, q->part, out,
                            ostride);
         PLR_HIP(hipGetLastError());
     }

@@ -44,12 +44,17 @@ struct MmTab
 
 // ---------------------------------------------------------------------------
 
-__global__ __launch_bounds__(64) void k_mm_reduce(MmTab a, const double *__restrict__ X, int ldx,
+// NW waves per work-group split the rows of a tile (alternate 64-row blocks)
+// and are combined in LDS; NW = 1 when there are tasks enough to fill the
+// GPU (every level at once), NW = 4 for one level of long segments.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_mm_reduce(MmTab a, const double *__restrict__ X, int ldx,
                                                   double alpha, int trans, int raw,
                                                   double *__restrict__ part,
                                                   double *__restrict__ w, size_t wstride)
 {
-    const int lane = threadIdx.x, m = lane & 15, g = lane >> 4;
+    __shared__ double red[NW > 1 ? 2 : 1][NW > 1 ? MM_BLK : 1];
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
     const int task = blockIdx.x;
     int d = 0;
     while (task >= a.task_off[d + 1]) ++d;
@@ -88,7 +93,8 @@ __global__ __launch_bounds__(64) void k_mm_reduce(MmTab a, const double *__restr
 
     // Loads are unconditional (columns past the end read column 0) and are
     // zeroed after they land, so a whole 8-row chunk is in flight at once.
-    for (int rb = rbeg; rb < rend; rb += 64) {
+    // The four waves take alternate 64-row blocks of the tile.
+    for (int rb = rbeg + 64 * wv; rb < rend; rb += 64 * NW) {
         const int r = rb + 16 * g;
 #pragma unroll
         for (int t0 = 0; t0 < 16; t0 += 8) {
@@ -122,6 +128,45 @@ __global__ __launch_bounds__(64) void k_mm_reduce(MmTab a, const double *__restr
         }
     }
 
+    // combine the four waves in a fixed order: (w0 + w2) + (w1 + w3)
+    if (NW > 1) {
+        const int slot = (wv & 1), base = lane;
+        if (wv >= 2) {
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[slot][((kt * 4 + ct) * 4 + v) * 64 + base] = acc[kt][ct][v];
+        }
+        __syncthreads();
+        if (wv < 2) {
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[kt][ct][v] += red[slot][((kt * 4 + ct) * 4 + v) * 64 + base];
+        }
+        __syncthreads();
+        if (wv == 1) {
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[0][((kt * 4 + ct) * 4 + v) * 64 + base] = acc[kt][ct][v];
+        }
+        __syncthreads();
+        if (wv != 0) return;
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[kt][ct][v] += red[0][((kt * 4 + ct) * 4 + v) * 64 + base];
+    }
+
     // lane holds T[k0 + 16kt + g + 4v][c0 + 16ct + m]
     if (tps == 1) {
         const int sp = raw ? s : (s ^ 1);
@@ -153,90 +198,117 @@ __global__ __launch_bounds__(64) void k_mm_reduce(MmTab a, const double *__restr
                 pp[(ct * 16 + m) * MM_KG + kt * 16 + g + 4 * v] = acc[kt][ct][v];
 }
 
-// Sum the partial blocks of every segment (fixed tile order) into the
-// coefficients.  One work-group per (level, segment, rank group, column group).
+// Sum the partial blocks of every segment (fixed order) into the
+// coefficients.  One work-group per (level, segment, rank group, column
+// group, 64-element chunk of the block): its four waves sum every fourth
+// tile with eight loads in flight, then combine through LDS.
+static constexpr int MM_FCH = MM_BLK / 64; // chunks per block
+
 __global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int trans, int raw,
                                                    const double *__restrict__ part,
                                                    double *__restrict__ w, size_t wstride)
 {
-    const int task = blockIdx.x;
+    __shared__ double red[4][64];
+    const int task = blockIdx.x / MM_FCH, chunk = blockIdx.x % MM_FCH;
     int d = 0;
     while (task >= a.fin_off[d + 1]) ++d;
     const int loc = task - a.fin_off[d];
     const int nkg = a.nkg[d], ncg = a.ncg, tps = a.tps[d], R = a.R[d];
     const int kg = loc % nkg, cg = (loc / nkg) % ncg, s = loc / (nkg * ncg);
+    const int el = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int e = chunk * 64 + el;
+    const double *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * MM_BLK + e;
+    const size_t step = (size_t)nkg * ncg * MM_BLK;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j0 = ph; j0 < tps; j0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + 4 * u;
+            const double v = pp[(size_t)min(j, tps - 1) * step];
+            acc[u] += j < tps ? v : 0.0;
+        }
+    }
+    red[ph][el] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (ph != 0) return;
+    const double t = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
+    const int kl = e % MM_KG, cl = e / MM_KG;
+    const int k = kg * MM_KG + kl, c = cg * MM_CG + cl;
+    if (k >= R || c >= a.ncols) return;
     const int sp = raw ? s : (s ^ 1);
     const int sidx = trans ? s : (s ^ 1);
-    for (int e = threadIdx.x; e < MM_BLK; e += blockDim.x) {
-        const int kl = e % MM_KG, cl = e / MM_KG;
-        const int k = kg * MM_KG + kl, c = cg * MM_CG + cl;
-        if (k >= R || c >= a.ncols) continue;
-        // independent accumulators keep eight loads in flight
-        const double *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * MM_BLK + e;
-        const size_t step = (size_t)nkg * ncg * MM_BLK;
-        double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-        for (int j = 0; j < tps; j += 8) {
-#pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const double v = pp[(size_t)min(j + u, tps - 1) * step];
-                acc[u] += (j + u < tps) ? v : 0.0;
-            }
-        }
-        const double t = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-        double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
-        wc[k] = raw ? t : alpha * a.S[d][(size_t)sidx * R + k] * t;
-    }
+    double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+    wc[k] = raw ? t : alpha * a.S[d][(size_t)sidx * R + k] * t;
 }
 
 // Y^T (64 columns x 16 rows per wavefront) += W^T Q^T over all levels, plus
-// the leaf term; four wavefronts per work-group cover 64 rows.
+// the leaf term; four wavefronts per work-group cover 64 rows, which lie in
+// one segment of every level.  The A operands (W^T per level, X^T of the
+// leaf) are staged in LDS per 64-wide chunk with coalesced loads; the B
+// operands (Q^T, D^T) are read from HBM, 16 consecutive rows per lane group.
+__device__ __forceinline__ void mm_stage(double *sh, const double *src, size_t cstride, int nc,
+                                         int kn)
+{
+    // sh[c*65 + kk] = src[c*cstride + kk] for c < nc, kk < kn, else 0
+    for (int e0 = 0; e0 < 64 * 64; e0 += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 256 + (int)threadIdx.x, c = e >> 6, kk = e & 63;
+            // unconditional load, padding zeroed by a 0/1 factor (a select
+            // would let the compiler branch around the load)
+            const double x = src[(size_t)min(c, nc - 1) * cstride + min(kk, kn - 1)];
+            v[u] = x * (double)(c < nc && kk < kn);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 256 + (int)threadIdx.x;
+            sh[(e >> 6) * 65 + (e & 63)] = v[u];
+        }
+    }
+}
+
 __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__restrict__ D,
                                                    const double *__restrict__ X, int ldx,
                                                    double alpha, double beta, int trans,
                                                    const double *__restrict__ w, size_t wstride,
                                                    double *__restrict__ Y, int ldy)
 {
+    __shared__ double sh[64 * 65];
     const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
     const int wv = threadIdx.x >> 6;
-    const int r0 = blockIdx.x * 64 + wv * 16;
+    const int rb = blockIdx.x * 64, r0 = rb + wv * 16;
     const int c0 = blockIdx.y * MM_CG;
-    const int N = a.N, b = a.b;
-    if (r0 >= N) return;
-    // A operand (16 columns c of W^T or X^T, 4 k): lane holds [c0 + 16ct + m][k + g]
-    // B operand (4 k x 16 rows of Q^T or D^T):     lane holds [k + g][r0 + m]
-    bool vc[4];
-    int cc[4];
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct) {
-        cc[ct] = c0 + ct * 16 + m;
-        vc[ct] = cc[ct] < a.ncols;
-        if (!vc[ct]) cc[ct] = 0;
-    }
+    const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
+    // A operand (16 columns c, 4 k): lane holds sh[(16ct + m)*65 + k + g]
+    // B operand (4 k x 16 rows of Q^T or D^T): lane holds [k + g][r0 + m]
     d4 acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
 
-    // Every operand batch (four MFMA k-steps) is loaded before it is used;
-    // loads are unconditional and out-of-range columns are zeroed afterwards.
     if (D) {
-        const int leaf = r0 / b, il = r0 - leaf * b;
+        const int leaf = rb / b, il = r0 - leaf * b;
+        const size_t ds_i = trans ? b : 1, ds_j = trans ? 1 : b; // element (row, col) of op(D)
         const double *Dl = D + (size_t)leaf * b * b;
-        const double *xl = X + (size_t)leaf * b;
-        for (int j = 0; j < b; j += 16) {
-            double bq[4], av[4][4];
+        const double *xl = X + (size_t)c0 * ldx + (size_t)leaf * b;
+        for (int j0 = 0; j0 < b; j0 += 64) {
+            const int jn = min(64, b - j0);
+            __syncthreads();
+            mm_stage(sh, xl + j0, ldx, nc, jn);
+            __syncthreads();
+            for (int j = 0; j < jn; j += 16) {
+                double bq[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int jj = min(j + 4 * u, b - 4) + g;
-                bq[u] = trans ? Dl[(size_t)(il + m) * b + jj] : Dl[(size_t)jj * b + il + m];
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = j0 + min(j + 4 * u, jn - 4) + g;
+                    bq[u] = Dl[(size_t)(il + m) * ds_i + (size_t)jj * ds_j];
+                }
+                // rows past the chunk read zeros from sh (and real data from D)
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct) av[u][ct] = xl[(size_t)cc[ct] * ldx + jj];
-            }
+                for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const bool ok = j + 4 * u < b;
-#pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
-                    acc[ct] = mfma64(vc[ct] && ok ? av[u][ct] : 0.0, bq[u], acc[ct]);
+                    for (int ct = 0; ct < 4; ++ct)
+                        acc[ct] = mfma64(sh[(16 * ct + m) * 65 + j + 4 * u + g], bq[u], acc[ct]);
             }
         }
 #pragma unroll
@@ -245,24 +317,27 @@ __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__rest
     for (int d = 0; d < a.L; ++d) {
         const int R = a.R[d];
         if (R == 0) continue;
-        const int s = r0 / a.seglen[d];
+        const int s = rb / a.seglen[d];
         const double *Qd = a.Q[d];
-        const double *wd = w + a.w_off[d] + (size_t)s * R;
-        for (int k = 0; k < R; k += 16) {
-            double bq[4], av[4][4];
+        const double *wd = w + (size_t)c0 * wstride + a.w_off[d] + (size_t)s * R;
+        for (int k0 = 0; k0 < R; k0 += 64) {
+            const int kn = min(64, R - k0);
+            __syncthreads();
+            mm_stage(sh, wd + k0, wstride, nc, kn);
+            __syncthreads();
+            for (int k = 0; k < kn; k += 16) {
+                double bq[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int kk = min(k + 4 * u, R - 4) + g;
-                bq[u] = Qd[(size_t)kk * N + r0 + m];
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = k0 + min(k + 4 * u, kn - 4) + g;
+                    bq[u] = Qd[(size_t)kk * N + r0 + m];
+                }
+                // ranks past the chunk read zeros from sh (and real data from Q)
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct) av[u][ct] = wd[(size_t)cc[ct] * wstride + kk];
-            }
+                for (int u = 0; u < 4; ++u)
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const bool ok = k + 4 * u < R;
-#pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
-                    acc[ct] = mfma64(vc[ct] && ok ? av[u][ct] : 0.0, bq[u], acc[ct]);
+                    for (int ct = 0; ct < 4; ++ct)
+                        acc[ct] = mfma64(sh[(16 * ct + m) * 65 + k + 4 * u + g], bq[u], acc[ct]);
             }
         }
     }
@@ -366,12 +441,12 @@ int plr_mm_apply(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
     int rc = mm_part(p, (size_t)t.part_off[t.L]);
     if (rc) return rc;
     if (t.task_off[t.L] > 0) {
-        hipLaunchKernelGGL(k_mm_reduce, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx, alpha,
+        hipLaunchKernelGGL(k_mm_reduce<1>, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx, alpha,
                            trans, 0, p->tmp, w, wstride);
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[t.L] > 0) {
-        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[t.L]), dim3(256), 0, st, t, alpha, trans, 0,
+        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t, alpha, trans, 0,
                            p->tmp, w, wstride);
         PLR_HIP(hipGetLastError());
     }
@@ -393,12 +468,16 @@ int plr_mm_reduce1(dplrh_plan *p, int d, const double *P, const double *X, int l
     int rc = mm_part(p, (size_t)t.part_off[1]);
     if (rc) return rc;
     if (t.task_off[1] > 0) {
-        hipLaunchKernelGGL(k_mm_reduce, dim3(t.task_off[1]), dim3(64), 0, st, t, X, ldx, 1.0, 0, 1,
-                           p->tmp, T, tstride);
+        if (t.T[0] >= 256)
+            hipLaunchKernelGGL(k_mm_reduce<4>, dim3(t.task_off[1]), dim3(256), 0, st, t, X, ldx, 1.0,
+                               0, 1, p->tmp, T, tstride);
+        else
+            hipLaunchKernelGGL(k_mm_reduce<1>, dim3(t.task_off[1]), dim3(64), 0, st, t, X, ldx, 1.0,
+                               0, 1, p->tmp, T, tstride);
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[1] > 0) {
-        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[1]), dim3(256), 0, st, t, 1.0, 0, 1, p->tmp,
+        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[1] * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1, p->tmp,
                            T, tstride);
         PLR_HIP(hipGetLastError());
     }

@@ -194,7 +194,7 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 
 struct Mv2Tab
 {
-    int L, N, nb;
+    int L, N, nb, nch; // reduction batches of KG ranks, expansion chunks of 32 ranks
     int R[PLR_MAXL], seg[PLR_MAXL], TW[PLR_MAXL], tps[PLR_MAXL], nkg[PLR_MAXL];
     int task_off[PLR_MAXL + 1], part_off[PLR_MAXL + 1], fin_off[PLR_MAXL + 1];
     int w_off[PLR_MAXL + 1];
@@ -213,10 +213,12 @@ __device__ __forceinline__ double mv_coef(double t, int raw, double alpha, const
 // 64-lane sums go through a per-wave LDS transpose; segments of 64 or 128
 // rows are finished here, longer ones leave one partial per 128-row chunk
 // for k_mv_finish.  Every lane issues its KG 16-byte loads back to back.
+// The raw sum over segment s goes to w at segment s ^ 1 (the rows it is
+// expanded into); the expansion applies alpha and sig.
 __global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__restrict__ x,
-                                                    int ldx, double alpha, int trans, int raw,
-                                                    double *__restrict__ part, size_t pstride,
-                                                    double *__restrict__ w, size_t wstride)
+                                                    int ldx, double *__restrict__ part,
+                                                    size_t pstride, double *__restrict__ w,
+                                                    size_t wstride)
 {
     __shared__ double red[4][KG][65];
     const int lane = threadIdx.x & 63;
@@ -227,18 +229,24 @@ __global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__re
     part += (size_t)col * pstride;
     w += (size_t)col * wstride;
     const d2v xv = *(const d2v *)(x + i);
-    const int sp0 = raw ? 0 : 1; // output segment = s ^ sp0
-    int d = 0, bd = 0;           // level of batch bi and its first batch
-    for (int bi = g; bi < a.nb; bi += 4) {
-        while (bi >= bd + a.nkg[d]) bd += a.nkg[d++];
-        const int R = a.R[d], k0 = (bi - bd) * KG, kn = min(KG, R - k0);
-        const double *__restrict__ P = a.P[d] + (size_t)k0 * N + i;
-        d2v pv[KG];
+    // batch bi -> (level, first column) by a walk of the level table
+    auto locate = [&](int bi, int &dl, int &kl) {
+        int bd = 0;
+        dl = 0;
+        while (bi >= bd + a.nkg[dl]) bd += a.nkg[dl++];
+        kl = (bi - bd) * KG;
+    };
+    auto load = [&](d2v *v, int dl, int kl) {
+        const int kn = min(KG, a.R[dl] - kl);
+        const double *__restrict__ P = a.P[dl] + (size_t)kl * N + i;
 #pragma unroll
         for (int kk = 0; kk < KG; ++kk)
-            pv[kk] = __builtin_nontemporal_load((const d2v *)(P + (size_t)(kk < kn ? kk : kn - 1) * N));
+            v[kk] = __builtin_nontemporal_load((const d2v *)(P + (size_t)(kk < kn ? kk : kn - 1) * N));
+    };
+    auto reduce = [&](const d2v *v, int dl, int kl) {
+        const int R = a.R[dl], kn = min(KG, R - kl);
 #pragma unroll
-        for (int kk = 0; kk < KG; ++kk) red[g][kk][lane] = pv[kk].x * xv.x + pv[kk].y * xv.y;
+        for (int kk = 0; kk < KG; ++kk) red[g][kk][lane] = v[kk].x * xv.x + v[kk].y * xv.y;
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -250,54 +258,87 @@ __global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__re
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int kc = k0 + kk, seg = a.seg[d];
+        const int kc = kl + kk, seg = a.seg[dl];
         const bool ok = kk < kn;
         sq += __shfl_xor(sq, 16, 64); // 64 rows
         if (seg == 64) {
             const int s = row0 / 64 + (qt >> 1);
-            if (ok && (qt & 1) == 0)
-                w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
-                    mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
-            continue;
+            if (ok && (qt & 1) == 0) w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq;
+            return;
         }
         sq += __shfl_xor(sq, 32, 64); // 128 rows
-        if (!ok || qt != 0) continue;
+        if (!ok || qt != 0) return;
         const int s = row0 / seg;
         if (seg == 128) {
-            w[a.w_off[d] + (size_t)(s ^ sp0) * R + kc] =
-                mv_coef(sq, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, kc);
+            w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq;
         } else {
             const int tile = (row0 - s * seg) / 128;
-            part[a.part_off[d] + ((size_t)s * a.tps[d] + tile) * R + kc] = sq;
+            part[a.part_off[dl] + ((size_t)s * a.tps[dl] + tile) * R + kc] = sq;
         }
+    };
+    // Software pipeline: the loads of the wave's next batch are issued before
+    // the current batch is reduced.  Two buffers alternate (a register copy
+    // would wait for the prefetch) and no load sits under a branch that
+    // rejoins before its use (the join would wait for every load).
+    const int cnt = g < a.nb ? (a.nb - g + 3) / 4 : 0;
+    if (cnt == 0) return;
+    d2v b0[KG], b1[KG];
+    int d0, k0, d1, k1;
+    locate(g, d0, k0);
+    load(b0, d0, k0);
+    for (int it = 0;; it += 2) {
+        if (it + 1 >= cnt) {
+            reduce(b0, d0, k0);
+            break;
+        }
+        locate(g + 4 * (it + 1), d1, k1);
+        load(b1, d1, k1);
+        reduce(b0, d0, k0);
+        if (it + 2 >= cnt) {
+            reduce(b1, d1, k1);
+            break;
+        }
+        locate(g + 4 * (it + 2), d0, k0);
+        load(b0, d0, k0);
+        reduce(b1, d1, k1);
     }
 }
 
-// Sum the tile partials of long segments: one thread per (level, segment, rank).
-__global__ void k_mv_finish(Mv2Tab a, double alpha, int trans, int raw,
-                            const double *__restrict__ part, size_t pstride,
-                            double *__restrict__ w, size_t wstride)
+// Sum the tile partials of long segments (fixed order): a work-group covers
+// 32 consecutive (level, segment, rank) entries with eight tile phases each
+// (eight loads in flight per thread), combined through LDS.
+__global__ __launch_bounds__(256) void k_mv_finish(Mv2Tab a, const double *__restrict__ part,
+                                                   size_t pstride, double *__restrict__ w,
+                                                   size_t wstride)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ double red[8][32];
+    const int el = threadIdx.x & 31, ph = threadIdx.x >> 5;
+    const int e = blockIdx.x * 32 + el;
     const int col = blockIdx.y;
-    if (e >= a.fin_off[a.L]) return;
+    const bool ok = e < a.fin_off[a.L];
     int d = 0;
-    while (e >= a.fin_off[d + 1]) ++d;
-    const int loc = e - a.fin_off[d], R = a.R[d], tps = a.tps[d];
-    const int s = loc / R, k = loc % R;
-    const double *pp = part + (size_t)col * pstride + a.part_off[d] + (size_t)s * tps * R + k;
+    if (ok)
+        while (e >= a.fin_off[d + 1]) ++d;
+    const int loc = ok ? e - a.fin_off[d] : 0, R = a.R[d], tps = a.tps[d];
+    const int s = loc / (R ? R : 1), k = loc - s * R;
     double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    for (int j = 0; j < tps; j += 8) {
+    if (ok) {
+        const double *pp = part + (size_t)col * pstride + a.part_off[d] + (size_t)s * tps * R + k;
+        for (int j0 = ph; j0 < tps; j0 += 64) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const double v = pp[(size_t)min(j + u, tps - 1) * R];
-            acc[u] += (j + u < tps) ? v : 0.0;
+            for (int u = 0; u < 8; ++u) {
+                const int j = j0 + 8 * u;
+                const double v = pp[(size_t)min(j, tps - 1) * R];
+                acc[u] += j < tps ? v : 0.0;
+            }
         }
     }
-    const double t = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-    const int sp = raw ? s : (s ^ 1);
-    w[(size_t)col * wstride + a.w_off[d] + (size_t)sp * R + k] =
-        mv_coef(t, raw, alpha, a.S[d], trans ? s : (s ^ 1), R, k);
+    red[ph][el] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (ph != 0 || !ok) return;
+    const double t = ((red[0][el] + red[1][el]) + (red[2][el] + red[3][el])) +
+                     ((red[4][el] + red[5][el]) + (red[6][el] + red[7][el]));
+    w[(size_t)col * wstride + a.w_off[d] + (size_t)(s ^ 1) * R + k] = t;
 }
 
 // Expansion: work-group = 128 rows x 4 rank quarters; lane l holds rows
@@ -349,29 +390,68 @@ __global__ __launch_bounds__(256) void k_mv_expand2(Mv2Tab a, int b, const doubl
         acc.x = alpha * sl.x;
         acc.y = alpha * sl.y;
     }
-    for (int d = 0; d < a.L; ++d) {
-        const int R = a.R[d], seg = a.seg[d];
-        const double *__restrict__ Q = a.Q[d];
-        const double *__restrict__ wb = w + a.w_off[d];
+    double2 lr = {0.0, 0.0}; // sum over the levels, before alpha
+    // The levels are walked as chunks of 32 ranks (wave g takes ranks
+    // k0 + g + 4u); the loads of the next chunk are issued before the current
+    // one is used, with two buffers alternating (a register copy, or a load
+    // under a branch that rejoins, would wait for the prefetch).
+    auto locate = [&](int ci, int &dl, int &kl) {
+        int cb = 0;
+        dl = 0;
+        while (ci >= cb + (a.R[dl] + 31) / 32) cb += (a.R[dl++] + 31) / 32;
+        kl = (ci - cb) * 32;
+    };
+    auto load = [&](int dl, int kl, d2v *qv, double *wr, double *sr) {
+        const int R = a.R[dl], seg = a.seg[dl];
         const bool uni = (seg & 127) == 0;
         const int s = uni ? __builtin_amdgcn_readfirstlane(row0 / seg) : i / seg;
-        const double *__restrict__ wd = wb + (size_t)s * R;
-        for (int k0 = g; k0 < R; k0 += 32) {
-            d2v qv[8];
-            double wv[8];
+        const double *__restrict__ Q = a.Q[dl];
+        const double *__restrict__ wd = w + a.w_off[dl] + (size_t)s * R;
+        // coefficient of rows in segment s: alpha * sig of the block whose
+        // U (V when transposed) lives on s, times the raw sum
+        const double *__restrict__ sd = a.S[dl] + (size_t)(trans ? (s ^ 1) : s) * R;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int k = min(k0 + 4 * u, R - 1);
-                qv[u] = __builtin_nontemporal_load((const d2v *)(Q + (size_t)k * N + i));
-                wv[u] = (k0 + 4 * u < R) ? wd[k] : 0.0;
-            }
+        for (int u = 0; u < 8; ++u) {
+            const int k = min(kl + g + 4 * u, R - 1);
+            qv[u] = __builtin_nontemporal_load((const d2v *)(Q + (size_t)k * N + i));
+            wr[u] = wd[k];
+            sr[u] = sd[k];
+        }
+    };
+    auto comp = [&](int dl, int kl, const d2v *qv, const double *wr, const double *sr) {
+        const int R = a.R[dl];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                acc.x += qv[u].x * wv[u];
-                acc.y += qv[u].y * wv[u];
+        for (int u = 0; u < 8; ++u) {
+            const double c = wr[u] * sr[u] * (double)(kl + g + 4 * u < R);
+            lr.x += qv[u].x * c;
+            lr.y += qv[u].y * c;
+        }
+    };
+    if (a.nch > 0) {
+        d2v q0[8], q1[8];
+        double w0[8], w1[8], s0[8], s1[8];
+        int d0, k0, d1, k1;
+        locate(0, d0, k0);
+        load(d0, k0, q0, w0, s0);
+        for (int it = 0;; it += 2) {
+            if (it + 1 >= a.nch) {
+                comp(d0, k0, q0, w0, s0);
+                break;
             }
+            locate(it + 1, d1, k1);
+            load(d1, k1, q1, w1, s1);
+            comp(d0, k0, q0, w0, s0);
+            if (it + 2 >= a.nch) {
+                comp(d1, k1, q1, w1, s1);
+                break;
+            }
+            locate(it + 2, d0, k0);
+            load(d0, k0, q0, w0, s0);
+            comp(d1, k1, q1, w1, s1);
         }
     }
+    acc.x += alpha * lr.x;
+    acc.y += alpha * lr.y;
     if (g > 0) red[g - 1][lane] = acc;
     __syncthreads();
     if (g == 0) {
@@ -416,6 +496,8 @@ static void mv2_tab(const dplrh_plan *p, int trans, Mv2Tab *t)
         t->S[d] = p->S + p->soff[d];
     }
     t->nb = nb;
+    t->nch = 0;
+    for (int d = 0; d < p->L; ++d) t->nch += (p->R[d] + 31) / 32;
     t->task_off[p->L] = nb;
     t->part_off[p->L] = part;
     t->fin_off[p->L] = fin;
@@ -434,13 +516,13 @@ static int mv2_launch(dplrh_plan *p, int trans, int nc, double alpha, double bet
         return -9;
     }
     if (t.nb > 0) {
-        hipLaunchKernelGGL(k_mv_reduce2, dim3(p->N / 128, nc), dim3(256), 0, st, t, x, ldx, alpha,
-                           trans, 0, p->part, p->part_per_col, p->w, p->w_per_col);
+        hipLaunchKernelGGL(k_mv_reduce2, dim3(p->N / 128, nc), dim3(256), 0, st, t, x, ldx, p->part,
+                           p->part_per_col, p->w, p->w_per_col);
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[t.L] > 0) {
-        hipLaunchKernelGGL(k_mv_finish, dim3((t.fin_off[t.L] + 63) / 64, nc), dim3(64), 0, st, t,
-                           alpha, trans, 0, p->part, p->part_per_col, p->w, p->w_per_col);
+        hipLaunchKernelGGL(k_mv_finish, dim3((t.fin_off[t.L] + 31) / 32, nc), dim3(256), 0, st, t,
+                           p->part, p->part_per_col, p->w, p->w_per_col);
         PLR_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_mv_expand2, dim3(p->N / 128, nc), dim3(256), 0, st, t, p->b, p->D, x, ldx,
