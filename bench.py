@@ -48,6 +48,20 @@ def parse():
     return ap.parse_args()
 
 
+def measured_traffic():
+    """HBM bytes per matvec from the committed counter summary (FETCH_SIZE and
+    WRITE_SIZE runs scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "matvec_fp64_N65536_L10_r32.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                if line.startswith("traffic_bytes_per_launch:"):
+                    return int(line.split(":")[1])
+    except OSError:
+        pass
+    return None
+
+
 def event_ms(torch, fn, reps):
     """Average ms per call of fn(i), HIP events on the current stream (the
     stream every library call in this script is launched on)."""
@@ -249,8 +263,8 @@ def main():
                         "plain_fp64_rhs_s": nrhs / ((plain_ms + f64_ms) * 1e-3)},
         "apply_fp64_tflops": flops / (ms_apply * 1e-3) / 1e12,
         "roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": mv_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "kernel": "single-RHS matvec (k_mv_reduce + k_mv_expand)",
+                     "frac": mv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
+                     "kernel": "single-RHS FP64 matvec (k_mv_reduce2 + k_mv_finish + k_mv_expand2)",
                      "bytes_per_launch": mv_bytes, "us_per_launch": ms_mv * 1e3},
         "cpu_baseline": cpu,
         "wall_s_timed": wall,

@@ -19,18 +19,17 @@
 typedef double plr_d4 __attribute__((ext_vector_type(4)));
 
 template <typename KT, typename VT, bool LP, int NCT>
-__global__ __launch_bounds__(256) void k_cap_mm(const KT *__restrict__ Kinv, int R,
-                                                const VT *__restrict__ S,
-                                                const VT *__restrict__ T, size_t tstride,
-                                                VT *__restrict__ W, size_t wstride, int trans,
-                                                int ncols, float *__restrict__ wscale,
-                                                size_t sstride)
+__device__ __forceinline__ void cap_body(int node, int cblk, const KT *__restrict__ Kinv, int R,
+                                         const VT *__restrict__ S, const VT *__restrict__ T,
+                                         size_t tstride, VT *__restrict__ W, size_t wstride,
+                                         int trans, int ncols, float *__restrict__ wscale,
+                                         size_t sstride)
 {
     // rhs[j][column], j < n2p (rows n2..n2p-1 zero), row stride cw + 1
     extern __shared__ double rhs[];
     __shared__ unsigned smax[64][2];
     constexpr int cw = 16 * NCT, cwp = cw + 1;
-    const int node = blockIdx.x, c0 = blockIdx.y * cw, n2 = 2 * R, n2p = (n2 + 63) & ~63;
+    const int c0 = cblk * cw, n2 = 2 * R, n2p = (n2 + 63) & ~63;
     const int nc = min(cw, ncols - c0);
     const int top = 2 * node, bot = top + 1;
     const KT *__restrict__ k = Kinv + (size_t)node * n2 * n2;
@@ -121,6 +120,18 @@ __global__ __launch_bounds__(256) void k_cap_mm(const KT *__restrict__ Kinv, int
             wscale[(size_t)(c0 + cl) * sstride + (half ? bot : top)] = sc;
         }
     }
+}
+
+template <typename KT, typename VT, bool LP, int NCT>
+__global__ __launch_bounds__(256) void k_cap_mm(const KT *__restrict__ Kinv, int R,
+                                                const VT *__restrict__ S,
+                                                const VT *__restrict__ T, size_t tstride,
+                                                VT *__restrict__ W, size_t wstride, int trans,
+                                                int ncols, float *__restrict__ wscale,
+                                                size_t sstride)
+{
+    cap_body<KT, VT, LP, NCT>(blockIdx.x, blockIdx.y, Kinv, R, S, T, tstride, W, wstride, trans,
+                              ncols, wscale, sstride);
 }
 
 // Column tiles per work-group: at most four, with the staged right-hand

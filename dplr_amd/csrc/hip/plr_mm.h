@@ -1,0 +1,394 @@
+// FP64 MFMA building blocks of the multi-RHS products (device bodies shared
+// by the single-plan kernels and the batched kernels), and their tables.
+#pragma once
+
+#include <cstring>
+
+#include "plr_common.h"
+
+typedef double d4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ d4 mfma64(double a, double b, d4 c)
+{
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+static constexpr int MM_KG = 32;   // ranks per task (2 MFMA tiles)
+static constexpr int MM_CG = 64;   // columns per task (4 MFMA tiles)
+static constexpr int MM_TMAX = 256; // rows per reduction task
+static constexpr int MM_BLK = MM_KG * MM_CG;
+
+struct MmTab
+{
+    int L, N, b, ncols, ncg;
+    int R[PLR_MAXL], seglen[PLR_MAXL], T[PLR_MAXL], tps[PLR_MAXL], nkg[PLR_MAXL];
+    int task_off[PLR_MAXL + 1];
+    int part_off[PLR_MAXL + 1]; // in blocks of MM_BLK doubles
+    int w_off[PLR_MAXL + 1];
+    int fin_off[PLR_MAXL + 1];  // finishing tasks (segments x groups of levels with tps > 1)
+    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+};
+
+// ---------------------------------------------------------------------------
+
+// NW waves per work-group split the rows of a tile (alternate 64-row blocks)
+// and are combined in LDS; NW = 1 when there are tasks enough to fill the
+// GPU (every level at once), NW = 4 for one level of long segments.
+template <int NW>
+__device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const double *__restrict__ X,
+                                               int ldx, double alpha, int trans, int raw,
+                                               double *__restrict__ part, double *__restrict__ w,
+                                               size_t wstride)
+{
+    __shared__ double red[NW > 1 ? 2 : 1][NW > 1 ? MM_BLK : 1];
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4, wv = threadIdx.x >> 6;
+    int d = 0;
+    while (task >= a.task_off[d + 1]) ++d;
+    const int loc = task - a.task_off[d];
+    const int nkg = a.nkg[d], ncg = a.ncg, tps = a.tps[d];
+    const int kg = loc % nkg;
+    const int cg = (loc / nkg) % ncg;
+    const int st = loc / (nkg * ncg); // segment * tps + tile
+    const int s = st / tps, tile = st - s * tps;
+    const int R = a.R[d], N = a.N, seg = a.seglen[d], T = a.T[d];
+    const int k0 = kg * MM_KG, c0 = cg * MM_CG;
+    const int rbeg = s * seg + tile * T;
+    const int rend = min(rbeg + T, (s + 1) * seg);
+
+    // per-lane column pointers (zero operand for columns past the end)
+    const double *pa[2];
+    const double *pb[4];
+    bool va[2], vb[4];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt) {
+        const int k = k0 + kt * 16 + m;
+        va[kt] = k < R;
+        pa[kt] = a.P[d] + (size_t)(va[kt] ? k : 0) * N;
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+        const int c = c0 + ct * 16 + m;
+        vb[ct] = c < a.ncols;
+        pb[ct] = X + (size_t)(vb[ct] ? c : 0) * ldx;
+    }
+    d4 acc[2][4];
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+
+    // Loads are unconditional (columns past the end read column 0) and are
+    // zeroed after they land, so a whole 8-row chunk is in flight at once.
+    // The four waves take alternate 64-row blocks of the tile.
+    for (int rb = rbeg + 64 * wv; rb < rend; rb += 64 * NW) {
+        const int r = rb + 16 * g;
+#pragma unroll
+        for (int t0 = 0; t0 < 16; t0 += 8) {
+            double2 ua[2][4], ub[4][4];
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int h = 0; h < 4; ++h) ua[kt][h] = *(const double2 *)(pa[kt] + r + t0 + 2 * h);
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int h = 0; h < 4; ++h) ub[ct][h] = *(const double2 *)(pb[ct] + r + t0 + 2 * h);
+#pragma unroll
+            for (int tt = 0; tt < 8; ++tt) {
+                double av[2], bv[4];
+#pragma unroll
+                for (int kt = 0; kt < 2; ++kt) {
+                    const double v = (tt & 1) ? ua[kt][tt >> 1].y : ua[kt][tt >> 1].x;
+                    av[kt] = va[kt] ? v : 0.0;
+                }
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) {
+                    const double v = (tt & 1) ? ub[ct][tt >> 1].y : ub[ct][tt >> 1].x;
+                    bv[ct] = vb[ct] ? v : 0.0;
+                }
+#pragma unroll
+                for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = mfma64(av[kt], bv[ct], acc[kt][ct]);
+            }
+        }
+    }
+
+    // combine the four waves in a fixed order: (w0 + w2) + (w1 + w3)
+    if (NW > 1) {
+        const int slot = (wv & 1), base = lane;
+        if (wv >= 2) {
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[slot][((kt * 4 + ct) * 4 + v) * 64 + base] = acc[kt][ct][v];
+        }
+        __syncthreads();
+        if (wv < 2) {
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) acc[kt][ct][v] += red[slot][((kt * 4 + ct) * 4 + v) * 64 + base];
+        }
+        __syncthreads();
+        if (wv == 1) {
+#pragma unroll
+            for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) red[0][((kt * 4 + ct) * 4 + v) * 64 + base] = acc[kt][ct][v];
+        }
+        __syncthreads();
+        if (wv != 0) return;
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) acc[kt][ct][v] += red[0][((kt * 4 + ct) * 4 + v) * 64 + base];
+    }
+
+    // lane holds T[k0 + 16kt + g + 4v][c0 + 16ct + m]
+    if (tps == 1) {
+        const int sp = raw ? s : (s ^ 1);
+        const int sidx = trans ? s : (s ^ 1);
+#pragma unroll
+        for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const int c = c0 + ct * 16 + m;
+                if (c >= a.ncols) continue;
+                double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = k0 + kt * 16 + g + 4 * v;
+                    if (k < R)
+                        wc[k] = raw ? acc[kt][ct][v] : alpha * a.S[d][(size_t)sidx * R + k] * acc[kt][ct][v];
+                }
+            }
+        return;
+    }
+    // partial block: [c_local][k_local], c_local < 64, k_local < 32
+    double *pp = part + ((size_t)a.part_off[d] + ((size_t)st * nkg + kg) * ncg + cg) * MM_BLK;
+#pragma unroll
+    for (int kt = 0; kt < 2; ++kt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                pp[(ct * 16 + m) * MM_KG + kt * 16 + g + 4 * v] = acc[kt][ct][v];
+}
+
+
+// Sum the partial blocks of every segment (fixed order) into the
+// coefficients.  One work-group per (level, segment, rank group, column
+// group, 64-element chunk of the block): its four waves sum every fourth
+// tile with eight loads in flight, then combine through LDS.
+static constexpr int MM_FCH = MM_BLK / 64; // chunks per block
+
+__device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double alpha, int trans,
+                                               int raw, const double *__restrict__ part,
+                                               double *__restrict__ w, size_t wstride)
+{
+    __shared__ double red[4][64];
+    const int task = bx / MM_FCH, chunk = bx % MM_FCH;
+    int d = 0;
+    while (task >= a.fin_off[d + 1]) ++d;
+    const int loc = task - a.fin_off[d];
+    const int nkg = a.nkg[d], ncg = a.ncg, tps = a.tps[d], R = a.R[d];
+    const int kg = loc % nkg, cg = (loc / nkg) % ncg, s = loc / (nkg * ncg);
+    const int el = threadIdx.x & 63, ph = threadIdx.x >> 6;
+    const int e = chunk * 64 + el;
+    const double *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * MM_BLK + e;
+    const size_t step = (size_t)nkg * ncg * MM_BLK;
+    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j0 = ph; j0 < tps; j0 += 32) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int j = j0 + 4 * u;
+            const double v = pp[(size_t)min(j, tps - 1) * step];
+            acc[u] += j < tps ? v : 0.0;
+        }
+    }
+    red[ph][el] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (ph != 0) return;
+    const double t = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
+    const int kl = e % MM_KG, cl = e / MM_KG;
+    const int k = kg * MM_KG + kl, c = cg * MM_CG + cl;
+    if (k >= R || c >= a.ncols) return;
+    const int sp = raw ? s : (s ^ 1);
+    const int sidx = trans ? s : (s ^ 1);
+    double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+    wc[k] = raw ? t : alpha * a.S[d][(size_t)sidx * R + k] * t;
+}
+
+
+// Y^T (64 columns x 16 rows per wavefront) += W^T Q^T over all levels, plus
+// the leaf term; four wavefronts per work-group cover 64 rows, which lie in
+// one segment of every level.  The A operands (W^T per level, X^T of the
+// leaf) are staged in LDS per 64-wide chunk with coalesced loads; the B
+// operands (Q^T, D^T) are read from HBM, 16 consecutive rows per lane group.
+__device__ __forceinline__ void mm_stage(double *sh, const double *src, size_t cstride, int nc,
+                                         int kn)
+{
+    // sh[c*65 + kk] = src[c*cstride + kk] for c < nc, kk < kn, else 0
+    for (int e0 = 0; e0 < 64 * 64; e0 += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 256 + (int)threadIdx.x, c = e >> 6, kk = e & 63;
+            // unconditional load, padding zeroed by a 0/1 factor (a select
+            // would let the compiler branch around the load)
+            const double x = src[(size_t)min(c, nc - 1) * cstride + min(kk, kn - 1)];
+            v[u] = x * (double)(c < nc && kk < kn);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 256 + (int)threadIdx.x;
+            sh[(e >> 6) * 65 + (e & 63)] = v[u];
+        }
+    }
+}
+
+__device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
+                                               const double *__restrict__ D,
+                                               const double *__restrict__ X, int ldx, double alpha,
+                                               double beta, int trans, const double *__restrict__ w,
+                                               size_t wstride, double *__restrict__ Y, int ldy)
+{
+    __shared__ double sh[64 * 65];
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const int wv = threadIdx.x >> 6;
+    const int rb = bx * 64, r0 = rb + wv * 16;
+    const int c0 = by * MM_CG;
+    const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
+    // A operand (16 columns c, 4 k): lane holds sh[(16ct + m)*65 + k + g]
+    // B operand (4 k x 16 rows of Q^T or D^T): lane holds [k + g][r0 + m]
+    d4 acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
+
+    if (D) {
+        const int leaf = rb / b, il = r0 - leaf * b;
+        const size_t ds_i = trans ? b : 1, ds_j = trans ? 1 : b; // element (row, col) of op(D)
+        const double *Dl = D + (size_t)leaf * b * b;
+        const double *xl = X + (size_t)c0 * ldx + (size_t)leaf * b;
+        for (int j0 = 0; j0 < b; j0 += 64) {
+            const int jn = min(64, b - j0);
+            __syncthreads();
+            mm_stage(sh, xl + j0, ldx, nc, jn);
+            __syncthreads();
+            for (int j = 0; j < jn; j += 16) {
+                double bq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int jj = j0 + min(j + 4 * u, jn - 4) + g;
+                    bq[u] = Dl[(size_t)(il + m) * ds_i + (size_t)jj * ds_j];
+                }
+                // rows past the chunk read zeros from sh (and real data from D)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct)
+                        acc[ct] = mfma64(sh[(16 * ct + m) * 65 + j + 4 * u + g], bq[u], acc[ct]);
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[ct] *= alpha;
+    }
+    for (int d = 0; d < a.L; ++d) {
+        const int R = a.R[d];
+        if (R == 0) continue;
+        const int s = rb / a.seglen[d];
+        const double *Qd = a.Q[d];
+        const double *wd = w + (size_t)c0 * wstride + a.w_off[d] + (size_t)s * R;
+        for (int k0 = 0; k0 < R; k0 += 64) {
+            const int kn = min(64, R - k0);
+            __syncthreads();
+            mm_stage(sh, wd + k0, wstride, nc, kn);
+            __syncthreads();
+            for (int k = 0; k < kn; k += 16) {
+                double bq[4];
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const int kk = k0 + min(k + 4 * u, kn - 4) + g;
+                    bq[u] = Qd[(size_t)kk * N + r0 + m];
+                }
+                // ranks past the chunk read zeros from sh (and real data from Q)
+#pragma unroll
+                for (int u = 0; u < 4; ++u)
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct)
+                        acc[ct] = mfma64(sh[(16 * ct + m) * 65 + k + 4 * u + g], bq[u], acc[ct]);
+            }
+        }
+    }
+    // lane holds Y^T[c0 + 16ct + g + 4v][r0 + m]
+    double yo[4][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
+                yo[ct][v] = Y[(size_t)c * ldy + r0 + m];
+            }
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = c0 + ct * 16 + g + 4 * v;
+            if (c >= a.ncols) continue;
+            Y[(size_t)c * ldy + r0 + m] = (beta == 0.0 ? 0.0 : beta * yo[ct][v]) + acc[ct][v];
+        }
+}
+
+
+
+// Table for levels [d0, d1) with operands P (reduced), Q (expanded) and S.
+static inline void mm_tab(const dplrh_plan *p, int d0, int d1, const double *const *P,
+                   const double *const *Q, const double *const *S, int ncols, int w_local,
+                   MmTab *t)
+{
+    memset(t, 0, sizeof(*t));
+    t->L = d1 - d0;
+    t->N = p->N;
+    t->b = p->b;
+    t->ncols = ncols;
+    t->ncg = (ncols + MM_CG - 1) / MM_CG;
+    int task = 0, part = 0, fin = 0, woff = 0;
+    for (int i = 0; i < t->L; ++i) {
+        const int d = d0 + i;
+        const int seg = p->N >> (d + 1), nseg = 2 << d, R = p->R[d];
+        const int T = seg < MM_TMAX ? seg : MM_TMAX;
+        const int tps = seg / T;
+        const int nkg = (R + MM_KG - 1) / MM_KG;
+        t->R[i] = R;
+        t->seglen[i] = seg;
+        t->T[i] = T;
+        t->tps[i] = tps;
+        t->nkg[i] = nkg;
+        t->task_off[i] = task;
+        task += nkg * t->ncg * nseg * tps;
+        t->part_off[i] = part;
+        if (tps > 1) part += nkg * t->ncg * nseg * tps;
+        t->fin_off[i] = fin;
+        if (tps > 1) fin += nkg * t->ncg * nseg;
+        t->w_off[i] = w_local ? woff : (int)p->soff[d];
+        woff += nseg * R;
+        t->P[i] = P[i];
+        t->Q[i] = Q[i];
+        t->S[i] = S[i];
+    }
+    t->task_off[t->L] = task;
+    t->part_off[t->L] = part;
+    t->fin_off[t->L] = fin;
+    t->w_off[t->L] = w_local ? woff : (int)p->soff[d1];
+}
