@@ -45,6 +45,8 @@ def parse():
     ap.add_argument("--lp", default="f16", choices=["f16", "bf16"])
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--batch-count", type=int, default=512,
+                    help="matrices in the batched leg (0 skips it)")
     return ap.parse_args()
 
 
@@ -119,6 +121,95 @@ def cpu_baseline(args, log):
                       f"OpenBLAS 0.3.28, 1 thread ({total:.1f} s)",
             "apply_only_rhs_s": args.nrhs / t_apply,
             "solve_excl_inverse_rhs_s": args.nrhs / t_solve}
+
+
+def batch_specs(count, seed=7):
+    """Shapes of the batched workload: N log-uniform in [1024, 16384] snapped
+    to leaf * 2^L, leaf from {64, 128, 256}, rank uniform in {8..48} (at most
+    the order of the smallest off-diagonal block)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    specs = []
+    for _ in range(count):
+        leaf = int(rng.choice([64, 128, 256]))
+        n = float(np.exp(rng.uniform(np.log(1024), np.log(16384))))
+        L = max(1, int(round(np.log2(n / leaf))))
+        while (leaf << L) > 16384:
+            L -= 1
+        while (leaf << L) < 1024:
+            L += 1
+        r = min(int(rng.integers(8, 49)), leaf // 2)
+        specs.append((leaf << L, L, r))
+    return specs
+
+
+def batch_leg(args, torch, D, log):
+    """Many independent matrices in one go: FP64 apply + FP64 factored solve of
+    16 RHS per matrix for every matrix of the batch, one launch per phase,
+    replayed from a HIP graph.  Per-matrix latency from the same work done
+    one matrix at a time."""
+    import numpy as np
+    nrhs = 16
+    specs = batch_specs(args.batch_count)
+    t0 = time.perf_counter()
+    plans = []
+    for k, (m, L, r) in enumerate(specs):
+        t = D.PlrTree.random(m, L, r, 4.0, 1000 + k)
+        plans.append(D.Plan(t))
+        t.free()
+    for P in plans:
+        assert P.factor() == 0
+    torch.cuda.synchronize()
+    log(f"batch: {len(plans)} matrices generated, uploaded and factored in "
+        f"{time.perf_counter() - t0:.1f} s")
+    B = D.Batch(plans)
+    g = torch.Generator(device="cpu").manual_seed(11)
+    X = [torch.randn(nrhs, m, generator=g, dtype=torch.float64).to("cuda") for (m, _, _) in specs]
+    Y = [torch.zeros_like(x) for x in X]
+    Z = [torch.zeros_like(x) for x in X]
+
+    def both(_):
+        B.gepdmm(X, Y)
+        B.solve(X, Z)
+    for i in range(3):
+        both(i)
+    ms = event_ms(torch, both, 10)
+    ms_apply = event_ms(torch, lambda i: B.gepdmm(X, Y), 10)
+    ms_solve = event_ms(torch, lambda i: B.solve(X, Z), 10)
+    # accuracy of every solve through the single-plan FP64 apply
+    err = 0.0
+    for P, x, z in zip(plans, X, Z):
+        back = torch.zeros_like(x)
+        P.gepdmm("N", 1.0, z, 0.0, back)
+        err = max(err, float((back - x).norm() / x.norm()))
+    # per-matrix latency of the same work, one matrix at a time
+    lat = []
+    for P, x, y, z in zip(plans, X, Y, Z):
+        def one(_):
+            P.gepdmm("N", 1.0, x, 0.0, y)
+            P.solve("N", x, z)
+        for i in range(2):
+            one(i)
+        lat.append(event_ms(torch, one, 3))
+    lat = np.array(lat)
+    total = len(plans) * nrhs
+    out = {"matrices": len(plans), "rhs_per_matrix": nrhs,
+           "rhs_s": total / (ms * 1e-3), "ms_per_batch": ms,
+           "apply_ms": ms_apply, "solve_ms": ms_solve,
+           "max_rel_residual": err, "dtype": "f64",
+           "per_matrix_ms_p50": float(np.percentile(lat, 50)),
+           "per_matrix_ms_p99": float(np.percentile(lat, 99)),
+           "one_at_a_time_rhs_s": total / (lat.sum() * 1e-3),
+           "sizes": f"N in [{min(s[0] for s in specs)}, {max(s[0] for s in specs)}], "
+                    f"rank in [{min(s[2] for s in specs)}, {max(s[2] for s in specs)}], "
+                    "leaf in {64, 128, 256}"}
+    log(f"batch: {out['rhs_s']:.0f} RHS/s ({ms:.2f} ms per batch of {total} RHS; apply "
+        f"{ms_apply:.2f} ms, solve {ms_solve:.2f} ms); max residual {err:.1e}; one matrix at a "
+        f"time p50 {out['per_matrix_ms_p50']:.3f} ms p99 {out['per_matrix_ms_p99']:.3f} ms")
+    B.close()
+    for P in plans:
+        P.close()
+    return out
 
 
 def main():
@@ -230,6 +321,14 @@ def main():
     log(f"step (apply + mixed solve): {ms_step * 1e3:.0f} us -> {rhs_s:.0f} RHS/s; "
         f"plain fp64 {plain_ms * 1e3:.0f} us -> {plain_rhs_s:.0f} RHS/s")
 
+    batch = None
+    if args.batch_count > 0:
+        for P in plans:
+            P.close()
+        plans = []
+        torch.cuda.empty_cache()
+        batch = batch_leg(args, torch, D, log)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -267,6 +366,7 @@ def main():
                      "kernel": "single-RHS FP64 matvec (k_mv_reduce2 + k_mv_finish + k_mv_expand2)",
                      "bytes_per_launch": mv_bytes, "us_per_launch": ms_mv * 1e3},
         "cpu_baseline": cpu,
+        "batch": batch,
         "wall_s_timed": wall,
     }
     print(json.dumps(out), flush=True)

@@ -6,4 +6,4 @@ ctypes mirror of both for Python callers and tests.
 """
 from ._lib import LIB_PATH, lib  # noqa: F401
 from .tree import PlrTree, geinv, gepdmm, gepmv, geprk  # noqa: F401
-from .plan import Plan  # noqa: F401
+from .plan import Batch, Plan  # noqa: F401

@@ -60,6 +60,16 @@ size_t dplrh_plan_bytes(const dplrh_plan *p);
 int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
                const double *dx, double *dy, void *stream);
 /* C := alpha*op(A)*B + beta*C on device column-major blocks. */
+/* Batches of independent plans (plr_batch.hip): one launch per phase covers
+ * every matrix, replayed from a HIP graph.  X[i], Y[i] are device
+ * column-major N_i x nrhs blocks with leading dimensions ldx[i], ldy[i]. */
+typedef struct dplrh_batch dplrh_batch;
+int dplrh_batch_create(int count, dplrh_plan *const *plans, dplrh_batch **out);
+void dplrh_batch_destroy(dplrh_batch *B);
+int dplrh_batch_gemm(dplrh_batch *B, int nrhs, const double *const *X, const int *ldx,
+                     double *const *Y, const int *ldy, void *stream);
+int dplrh_batch_solve(dplrh_batch *B, int nrhs, const double *const *X, const int *ldx,
+                      double *const *Y, const int *ldy, void *stream);
 int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
                const double *dB, int ldb, double *dC, int ldc, void *stream);
 

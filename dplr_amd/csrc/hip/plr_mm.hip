@@ -65,6 +65,8 @@ static int mm_part(dplrh_plan *p, size_t blocks)
     return 0;
 }
 
+int plr_mm_part(dplrh_plan *p, size_t blocks) { return mm_part(p, blocks); }
+
 // Reduce-finish-expand over all levels for ncols columns (the multi-RHS
 // product): Y := alpha*op(A)*X + beta*Y.  w must hold soff[L] doubles per
 // column (stride wstride).

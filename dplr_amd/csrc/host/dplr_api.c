@@ -222,3 +222,24 @@ int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, doub
 }
 
 const char *dplrx_last_error(void) { return dplrh_last_error(); }
+
+/* ---- batches ------------------------------------------------------------ */
+
+int dplrx_batch_create(int count, dplrx_plan *const *plans, dplrx_batch **batch)
+{
+    return dplrh_batch_create(count, plans, batch);
+}
+
+void dplrx_batch_destroy(dplrx_batch *batch) { dplrh_batch_destroy(batch); }
+
+int dplrx_batch_gepdmm(dplrx_batch *batch, int nrhs, const double *const *dX, const int *ldx,
+                       double *const *dY, const int *ldy, void *stream)
+{
+    return dplrh_batch_gemm(batch, nrhs, dX, ldx, dY, ldy, stream);
+}
+
+int dplrx_batch_solve(dplrx_batch *batch, int nrhs, const double *const *dB, const int *ldb,
+                      double *const *dX, const int *ldx, void *stream)
+{
+    return dplrh_batch_solve(batch, nrhs, dB, ldb, dX, ldx, stream);
+}

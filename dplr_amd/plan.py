@@ -99,3 +99,51 @@ class Plan:
         check(lib.dplrx_solve_refined(self.p, n, _dptr(B), _ld(B), _dptr(X), _ld(X),
                                       iters, rp, _stream(stream)), "dplrx_solve_refined")
         return res
+
+
+class Batch:
+    """Independent plans applied or solved together: one kernel launch per
+    phase covers every matrix, replayed from a HIP graph (dplrx_batch_*).
+    The plans must stay alive as long as the batch."""
+
+    def __init__(self, plans):
+        self.plans = list(plans)
+        arr = (c_void_p * len(self.plans))(*[p.p for p in self.plans])
+        b = c_void_p()
+        check(lib.dplrx_batch_create(len(self.plans), arr, byref(b)), "dplrx_batch_create")
+        self.b = b
+
+    def close(self):
+        if getattr(self, "b", None):
+            lib.dplrx_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _io(self, X, Y):
+        n = len(self.plans)
+        xs = (c_void_p * n)(*[x.data_ptr() for x in X])
+        ys = (c_void_p * n)(*[y.data_ptr() for y in Y])
+        lx = (c_int * n)(*[_ld(x) for x in X])
+        ly = (c_int * n)(*[_ld(y) for y in Y])
+        return xs, lx, ys, ly
+
+    def gepdmm(self, X, Y, stream=None):
+        """Y[i] := A_i X[i]; X[i], Y[i] are (nrhs, N_i) device tensors."""
+        nrhs = X[0].shape[0]
+        xs, lx, ys, ly = self._io(X, Y)
+        check(lib.dplrx_batch_gepdmm(self.b, nrhs, xs, lx, ys, ly, _stream(stream)),
+              "dplrx_batch_gepdmm")
+        return Y
+
+    def solve(self, B, X, stream=None):
+        """X[i] := A_i^{-1} B[i] (every plan factored)."""
+        nrhs = B[0].shape[0]
+        bs, lb, xs, lx = self._io(B, X)
+        check(lib.dplrx_batch_solve(self.b, nrhs, bs, lb, xs, lx, _stream(stream)),
+              "dplrx_batch_solve")
+        return X

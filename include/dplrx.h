@@ -65,6 +65,23 @@ int dplrx_factor_mixed(dplrx_plan *plan, int type, int *info, void *stream);
 int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
                         int ldx, int iters, double *res, void *stream);
 
+/* ---- batches of independent matrices ----------------------------------- */
+
+/* A batch over `count` existing plans (not owned; they must outlive the
+ * batch).  Every plan needs leaves of a multiple of 64 rows.  One call
+ * runs every matrix at once: one kernel launch per phase covers all of
+ * them, and the sequence is replayed from a HIP graph on later calls. */
+typedef struct dplrh_batch dplrx_batch;
+int dplrx_batch_create(int count, dplrx_plan *const *plans, dplrx_batch **batch);
+void dplrx_batch_destroy(dplrx_batch *batch);
+/* Y[i] := A_i X[i] for nrhs device columns per matrix (host arrays of
+ * device pointers and leading dimensions; leading dimensions even). */
+int dplrx_batch_gepdmm(dplrx_batch *batch, int nrhs, const double *const *dX, const int *ldx,
+                       double *const *dY, const int *ldy, void *stream);
+/* X[i] := A_i^{-1} B[i], every plan factored with dplrx_factor. */
+int dplrx_batch_solve(dplrx_batch *batch, int nrhs, const double *const *dB, const int *ldb,
+                      double *const *dX, const int *ldx, void *stream);
+
 const char *dplrx_last_error(void);
 
 /* ---- synthetic inputs (deterministic, counter-based RNG) --------------- */

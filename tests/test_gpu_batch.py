@@ -1,0 +1,98 @@
+"""Batched apply and solve of independent matrices of different order, leaf
+size and rank (dplrx_batch_*): every matrix of the batch against its own
+single-plan call, and against the oracle's dense product for small ones."""
+import numpy as np
+import pytest
+
+import dplr_amd as D
+from dplr_amd.tree import PlrTree
+
+pytestmark = pytest.mark.gpu
+
+# (m, nlvl, rank): leaves of 64, 128 and 256 rows, ranks that are not
+# multiples of the MFMA tiles, depths from 1 to 6
+SPECS = [(1024, 4, 8), (2048, 5, 16), (4096, 6, 48), (1024, 3, 20), (2048, 3, 33),
+         (8192, 5, 12), (512, 1, 40), (4096, 4, 24)]
+NRHS = 16
+
+
+def rel(a, b):
+    return float((a - b).norm() / b.norm())
+
+
+@pytest.fixture(scope="module")
+def batch(cuda):
+    import torch
+    trees = [PlrTree.random(m, L, r, 4.0, 100 + k) for k, (m, L, r) in enumerate(SPECS)]
+    plans = [D.Plan(t) for t in trees]
+    B = D.Batch(plans)
+    g = torch.Generator(device="cpu").manual_seed(5)
+    X = [torch.randn(NRHS, m, generator=g, dtype=torch.float64).to("cuda") for (m, _, _) in SPECS]
+    yield trees, plans, B, X
+    B.close()
+    for p in plans:
+        p.close()
+    for t in trees:
+        t.free()
+
+
+def test_batched_apply_matches_single(batch):
+    import torch
+    trees, plans, B, X = batch
+    print("batched apply", SPECS, flush=True)
+    Y = [torch.zeros_like(x) for x in X]
+    B.gepdmm(X, Y)
+    for k, (P, x, y) in enumerate(zip(plans, X, Y)):
+        y1 = torch.zeros_like(x)
+        P.gepdmm("N", 1.0, x, 0.0, y1)
+        assert rel(y, y1) < 1e-13, SPECS[k]
+    # the dense product for the two smallest
+    for k in (0, 6):
+        A = torch.tensor(trees[k].dense(), device="cuda")
+        ref = (A @ X[k].T).T
+        assert rel(Y[k], ref) < 1e-12, SPECS[k]
+
+
+def test_batched_solve_matches_single_and_replays(batch):
+    import torch
+    trees, plans, B, X = batch
+    print("batched solve", SPECS, flush=True)
+    for P in plans:
+        assert P.factor() == 0
+    Z = [torch.zeros_like(x) for x in X]
+    for rep in range(3):  # eager build, capture, replay
+        for z in Z:
+            z.zero_()
+        B.solve(X, Z)
+        for k, (P, x, z) in enumerate(zip(plans, X, Z)):
+            z1 = torch.zeros_like(x)
+            P.solve("N", x, z1)
+            assert rel(z, z1) < 1e-12, (rep, SPECS[k])
+            back = torch.zeros_like(x)
+            P.gepdmm("N", 1.0, z, 0.0, back)
+            assert rel(back, x) < 1e-12, (rep, SPECS[k])
+    # new right-hand sides in new buffers: the call's pointers are re-uploaded
+    X2 = [2.0 * x + 1.0 for x in X]
+    Z2 = [torch.zeros_like(x) for x in X]
+    B.solve(X2, Z2)
+    for k, (P, x, z) in enumerate(zip(plans, X2, Z2)):
+        back = torch.zeros_like(x)
+        P.gepdmm("N", 1.0, z, 0.0, back)
+        assert rel(back, x) < 1e-12, SPECS[k]
+
+
+def test_batch_rebuilds_after_plan_workspace_changes(batch):
+    import torch
+    trees, plans, B, X = batch
+    for P in plans:
+        assert P.factor() == 0
+    # a wider single-plan solve reallocates that plan's workspace
+    big = torch.randn(3 * NRHS, SPECS[2][0], dtype=torch.float64, device="cuda")
+    out = torch.zeros_like(big)
+    plans[2].solve("N", big, out)
+    Z = [torch.zeros_like(x) for x in X]
+    B.solve(X, Z)
+    for k, (P, x, z) in enumerate(zip(plans, X, Z)):
+        back = torch.zeros_like(x)
+        P.gepdmm("N", 1.0, z, 0.0, back)
+        assert rel(back, x) < 1e-12, SPECS[k]
