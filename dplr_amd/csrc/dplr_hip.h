@@ -60,6 +60,10 @@ size_t dplrh_plan_bytes(const dplrh_plan *p);
 int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
                const double *dx, double *dy, void *stream);
 /* C := alpha*op(A)*B + beta*C on device column-major blocks. */
+/* Outer-product gradient projected onto the plan's factors (plr_grad.hip). */
+int dplrh_grad_outer(dplrh_plan *p, int n, double alpha, const double *A, int lda,
+                     const double *B, int ldb, double *dU, double *dV, double *dS, double *dD,
+                     void *stream);
 /* Batches of independent plans (plr_batch.hip): one launch per phase covers
  * every matrix, replayed from a HIP graph.  X[i], Y[i] are device
  * column-major N_i x nrhs blocks with leading dimensions ldx[i], ldy[i]. */

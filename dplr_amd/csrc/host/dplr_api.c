@@ -243,3 +243,12 @@ int dplrx_batch_solve(dplrx_batch *batch, int nrhs, const double *const *dB, con
 {
     return dplrh_batch_solve(batch, nrhs, dB, ldb, dX, ldx, stream);
 }
+
+/* ---- adjoints ----------------------------------------------------------- */
+
+int dplrx_grad_outer(dplrx_plan *plan, int n, double alpha, const double *dGa, int lda,
+                     const double *dGb, int ldb, double *dU, double *dV, double *dS, double *dD,
+                     void *stream)
+{
+    return dplrh_grad_outer(plan, n, alpha, dGa, lda, dGb, ldb, dU, dV, dS, dD, stream);
+}

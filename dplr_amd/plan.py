@@ -80,6 +80,37 @@ class Plan:
                               _ld(X), _stream(stream)), "dplrx_solve")
         return X
 
+    # ---- level-wise layout and adjoints -------------------------------------
+
+    def layout(self):
+        """Offsets of the level-wise arrays: (col_off, s_off) where level d's
+        U/V columns are [col_off[d], col_off[d+1]) of the (cols, m) arrays and
+        its sig entries are [s_off[d], s_off[d+1]), segment q at q * R_d."""
+        col, so = [0], [0]
+        for d, R in enumerate(self.ranks):
+            col.append(col[-1] + R)
+            so.append(so[-1] + (2 << d) * R)
+        return col, so
+
+    def grad_outer(self, alpha, Ga, Gb, stream=None):
+        """Gradient of <alpha * Ga Gb^T, A> with respect to the factors, as
+        (dU, dV, dS, dD) device tensors in the plan's layouts: dU, dV of shape
+        (sum R_d, m) (row k = column k of the level-wise array), dS of
+        sum_d 2^(d+1) R_d entries, dD of shape (m/leaf, leaf, leaf) with each
+        leaf column-major (dD[l][j][i] = d A(l*leaf + i, l*leaf + j))."""
+        import torch
+        col, so = self.layout()
+        dev = Ga.device
+        dU = torch.zeros(col[-1], self.m, dtype=torch.float64, device=dev)
+        dV = torch.zeros_like(dU)
+        dS = torch.zeros(max(so[-1], 1), dtype=torch.float64, device=dev)
+        dD = torch.zeros(self.m // self.leaf, self.leaf, self.leaf, dtype=torch.float64, device=dev)
+        n = Ga.shape[0]
+        check(lib.dplrx_grad_outer(self.p, n, alpha, _dptr(Ga), _ld(Ga), _dptr(Gb), _ld(Gb),
+                                   _dptr(dU), _dptr(dV), _dptr(dS), _dptr(dD), _stream(stream)),
+              "dplrx_grad_outer")
+        return dU, dV, dS[:so[-1]], dD
+
     LP_TYPES = {"f16": 1, "bf16": 2}
 
     def factor_mixed(self, dtype="f16", stream=None):

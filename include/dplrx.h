@@ -65,6 +65,19 @@ int dplrx_factor_mixed(dplrx_plan *plan, int type, int *info, void *stream);
 int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
                         int ldx, int iters, double *res, void *stream);
 
+/* ---- adjoints ----------------------------------------------------------- */
+
+/* Gradient of <G, A> with respect to A's factors for G = alpha * Ga Gb^T
+ * (Ga, Gb: device m x n blocks, lda, ldb >= m), written in the plan's
+ * layouts: dU and dV like the level-wise U and V (column-major m x sum_d
+ * R_d), dS like sig (sum_d 2^(d+1) R_d), dD like the leaves (m x leaf).
+ * For y = A^{-1} b and a loss with gradient ybar: Ga = A^{-T} ybar (a
+ * transposed dplrx_solve, which is also the gradient with respect to b),
+ * Gb = y, alpha = -1.  For y = A x: Ga = ybar, Gb = x, alpha = 1. */
+int dplrx_grad_outer(dplrx_plan *plan, int n, double alpha, const double *dGa, int lda,
+                     const double *dGb, int ldb, double *dU, double *dV, double *dS, double *dD,
+                     void *stream);
+
 /* ---- batches of independent matrices ----------------------------------- */
 
 /* A batch over `count` existing plans (not owned; they must outlive the
