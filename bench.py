@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512,
                     help="matrices in the batched leg (0 skips it)")
+    ap.add_argument("--grad-m", type=int, default=16384,
+                    help="order of the Gaussian-kernel gradient leg (0 skips it)")
     return ap.parse_args()
 
 
@@ -212,6 +214,64 @@ def batch_leg(args, torch, D, log):
     return out
 
 
+def grad_leg(args, torch, D, log):
+    """Gaussian-kernel PLR (x ~ U[0,1] sorted, l = 0.05, nugget 1e-2,
+    compressed at tol 1e-10 with leaf 128): forward y = A^{-1} b and
+    q = b^T y, backward dq/db and dq/d(U, V, sig, leaves) by the adjoint
+    kernels (transposed solve + projected outer product), FP64."""
+    import ctypes
+    import numpy as np
+    from dplr_amd.autograd import plr_solve
+    m, leaf = args.grad_m, 128
+    nlvl = int(round(np.log2(m // leaf)))
+    x = np.sort(np.random.default_rng(0).uniform(0, 1, m))
+    A = np.exp(-(x[:, None] - x[None, :]) ** 2 / (2 * 0.05 ** 2))
+    A[np.diag_indices(m)] += 1e-2
+    A = np.asfortranarray(A)
+    T = D.PlrTree(m)
+    info = ctypes.c_int(0)
+    t0 = time.perf_counter()
+    D.lib.dplrd2gep(m, A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, T.ref, 1e-10, nlvl,
+                    ctypes.byref(info))
+    t_cmp = time.perf_counter() - t0
+    del A
+    assert info.value == 0
+    P = D.Plan(T)
+    ranks = P.ranks
+    T.free()
+    t0 = time.perf_counter()
+    assert P.factor() == 0
+    torch.cuda.synchronize()
+    t_fac = time.perf_counter() - t0
+    b = torch.tensor(np.random.default_rng(2).standard_normal((1, m)), device="cuda")
+
+    def fwd(_):
+        return (b * plr_solve(b, P)).sum()
+
+    def fwd_bwd(_):
+        bb = b.clone().requires_grad_(True)
+        q = (bb * plr_solve(bb, P)).sum()
+        q.backward()
+        return bb.grad
+
+    for i in range(3):
+        fwd_bwd(i)
+    ms_f = event_ms(torch, fwd, 10)
+    ms_fb = event_ms(torch, fwd_bwd, 10)
+    # consistency: dq/db = y + A^{-T} b and, for the symmetric kernel, = 2 y
+    g = fwd_bwd(0)
+    y = torch.zeros_like(b)
+    P.solve("N", b, y)
+    err = float((g - 2 * y).norm() / (2 * y).norm())
+    out = {"m": m, "leaf": leaf, "levels": nlvl, "ranks": ranks, "dtype": "f64",
+           "compress_s": t_cmp, "factor_s": t_fac, "forward_ms": ms_f,
+           "forward_backward_ms": ms_fb, "grad_b_vs_2y": err}
+    log(f"gradient leg: N={m} ranks {ranks}; compress {t_cmp:.1f} s, factor {t_fac * 1e3:.1f} ms; "
+        f"forward {ms_f:.3f} ms, forward+backward {ms_fb:.3f} ms; |dq/db - 2y|/|2y| = {err:.1e}")
+    P.close()
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -329,6 +389,10 @@ def main():
         torch.cuda.empty_cache()
         batch = batch_leg(args, torch, D, log)
 
+    grad = None
+    if args.grad_m > 0:
+        grad = grad_leg(args, torch, D, log)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -367,6 +431,7 @@ def main():
                      "bytes_per_launch": mv_bytes, "us_per_launch": ms_mv * 1e3},
         "cpu_baseline": cpu,
         "batch": batch,
+        "gradient": grad,
         "wall_s_timed": wall,
     }
     print(json.dumps(out), flush=True)

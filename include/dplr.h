@@ -98,6 +98,17 @@ void dplrinih(int m, DPlrMtx *A);
  * columns and NaN-fills the rest. */
 void dplrresizelr(DPlrMtx *A, int capacity);
 
+/* ---- compression --------------------------------------------------------- */
+
+/* Replaces dplrd2gep, reference src/dplrd2p.c:187.  Compresses the dense
+ * host m x m matrix A (leading dimension lda) into B with nlvl levels
+ * (m divisible by 2^nlvl) and absolute error eps split as the reference
+ * splits it (eps/4 per level, eps/2 at the last one): each off-diagonal
+ * block keeps its singular triplets with sig >= its share of eps.  The SVDs
+ * of one level run as one batched call on the GPU.  *info = 0 on success,
+ * -1 on an argument error. */
+void dplrd2gep(int m, double *A, int lda, DPlrMtx *B, double eps, int nlvl, int *info);
+
 /* ---- helpers (host) ---------------------------------------------------- */
 
 /* Replaces dplrgep2d, reference src/dplrp2d.c:98.  Dense expansion of the
