@@ -26,6 +26,7 @@ struct MmTab
     int part_off[PLR_MAXL + 1]; // in blocks of MM_BLK doubles
     int w_off[PLR_MAXL + 1];
     int fin_off[PLR_MAXL + 1];  // finishing tasks (segments x groups of levels with tps > 1)
+    int sx; // the reduction stores alpha * t; the expansion multiplies by sig when staging
     const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
 };
 
@@ -156,7 +157,6 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
     // lane holds T[k0 + 16kt + g + 4v][c0 + 16ct + m]
     if (tps == 1) {
         const int sp = raw ? s : (s ^ 1);
-        const int sidx = trans ? s : (s ^ 1);
 #pragma unroll
         for (int kt = 0; kt < 2; ++kt)
 #pragma unroll
@@ -168,7 +168,7 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
                 for (int v = 0; v < 4; ++v) {
                     const int k = k0 + kt * 16 + g + 4 * v;
                     if (k < R)
-                        wc[k] = raw ? acc[kt][ct][v] : alpha * a.S[d][(size_t)sidx * R + k] * acc[kt][ct][v];
+                        wc[k] = raw ? acc[kt][ct][v] : alpha * acc[kt][ct][v];
                 }
             }
         return;
@@ -184,6 +184,161 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
                 pp[(ct * 16 + m) * MM_KG + kt * 16 + g + 4 * v] = acc[kt][ct][v];
 }
 
+
+// All-level reduction for the multi-RHS product: a work-group of 4 waves owns
+// 256 rows x 64 columns of X and every 16-rank tile of every level.  Each
+// 64-row block of X is staged once in LDS (its loads in flight during the
+// previous block's MFMAs) and serves all levels; wave w takes tiles w, w+4,
+// w+8 of a pass (one wave per SIMD, accumulators of three tiles),
+// streaming its P rows in half-blocks of 8 rows per lane with the next
+// half-block's loads issued before the current one is used.  Levels whose segments end inside the
+// work-group's rows write their coefficients when a segment ends; longer
+// segments write the partial blocks of mm_reduce_body (tiles of 256 rows),
+// summed by mm_finish_body.
+static constexpr int MMF_RB = 256, MMF_NW = 4, MMF_TPW = 3;
+
+__device__ __forceinline__ void mm_reduce_fused_body(const MmTab &a, int rbk, int cg,
+                                                     const double *__restrict__ X, int ldx,
+                                                     double alpha, int trans, int raw,
+                                                     double *__restrict__ part,
+                                                     double *__restrict__ w, size_t wstride)
+{
+    __shared__ double xs[64 * 65];
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const int wv = threadIdx.x >> 6;
+    const int rb0 = rbk * MMF_RB, c0 = cg * MM_CG, N = a.N;
+    int ntile = 0;
+    for (int d = 0; d < a.L; ++d) ntile += (a.R[d] + 15) / 16;
+    // X staging: each thread loads 8 pairs of rows (16 bytes each)
+    double2 xr[8];
+    auto xload = [&](int r0) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = u * 256 + (int)threadIdx.x, c = e >> 5, pr = e & 31;
+            const int cc = min(c0 + c, a.ncols - 1);
+            xr[u] = *(const double2 *)(X + (size_t)cc * ldx + r0 + 2 * pr);
+        }
+    };
+    auto xstore = [&](double *sh) {
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = u * 256 + (int)threadIdx.x, c = e >> 5, pr = e & 31;
+            const double on = c0 + c < a.ncols ? 1.0 : 0.0;
+            sh[c * 65 + 2 * pr] = xr[u].x * on;
+            sh[c * 65 + 2 * pr + 1] = xr[u].y * on;
+        }
+    };
+    for (int pass = 0; pass * MMF_NW * MMF_TPW < ntile; ++pass) {
+        int td[MMF_TPW], tk[MMF_TPW];
+        bool tv[MMF_TPW];
+        const double *pa[MMF_TPW];
+        double pon[MMF_TPW];
+#pragma unroll
+        for (int j = 0; j < MMF_TPW; ++j) {
+            const int t = pass * MMF_NW * MMF_TPW + wv + MMF_NW * j;
+            tv[j] = t < ntile;
+            int d = 0, cum = 0;
+            const int tt = tv[j] ? t : 0;
+            while (tt >= cum + (a.R[d] + 15) / 16) cum += (a.R[d++] + 15) / 16;
+            td[j] = d;
+            tk[j] = (tt - cum) * 16;
+            const int k = tk[j] + m;
+            pon[j] = (tv[j] && k < a.R[d]) ? 1.0 : 0.0;
+            pa[j] = a.P[d] + (size_t)min(k, a.R[d] - 1) * N;
+        }
+        d4 acc[MMF_TPW][4];
+#pragma unroll
+        for (int j = 0; j < MMF_TPW; ++j)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[j][ct] = d4{0.0, 0.0, 0.0, 0.0};
+        double2 pA[MMF_TPW][4], pB[MMF_TPW][4];
+        // half-block k: rows rb0 + 64*(k/2) + 16g + 8*(k%2) + 0..7 of every tile
+        auto pload = [&](double2 (*pv)[4], int k) {
+            const int r = rb0 + 64 * (k >> 1) + 16 * g + 8 * (k & 1);
+#pragma unroll
+            for (int j = 0; j < MMF_TPW; ++j)
+#pragma unroll
+                for (int h = 0; h < 4; ++h) pv[j][h] = *(const double2 *)(pa[j] + r + 2 * h);
+        };
+        auto compute = [&](const double *sh, double2 (*pv)[4], int half) {
+#pragma unroll
+            for (int tt = 0; tt < 8; ++tt) {
+                const int t = 8 * half + tt;
+                double bq[4];
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) bq[ct] = sh[(16 * ct + m) * 65 + 16 * g + t];
+#pragma unroll
+                for (int j = 0; j < MMF_TPW; ++j) {
+                    const double av = ((tt & 1) ? pv[j][tt >> 1].y : pv[j][tt >> 1].x) * pon[j];
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct) acc[j][ct] = mfma64(av, bq[ct], acc[j][ct]);
+                }
+            }
+        };
+        // tile j's output for the segment ending at row rend of its level
+        auto flush = [&](int j, int rend) {
+            const int d = td[j], R = a.R[d], seg = a.seglen[d];
+            const int s = (rend - 1) / seg;
+            const int tps = a.tps[d], T = a.T[d];
+            if (tps == 1) {
+                // stores only: a load here would drain the prefetch in flight
+                const int sp = raw ? s : (s ^ 1);
+                const double f = raw ? 1.0 : alpha;
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) {
+                    const int c = c0 + ct * 16 + m;
+                    if (c >= a.ncols) continue;
+                    double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) {
+                        const int k = tk[j] + g + 4 * v;
+                        if (k < R) wc[k] = f * acc[j][ct][v];
+                    }
+                }
+            } else {
+                const int tile = (rend - T - s * seg) / T;
+                const int st = s * tps + tile, kg = tk[j] / MM_KG;
+                double *pp = part + ((size_t)a.part_off[d] + ((size_t)st * a.nkg[d] + kg) * a.ncg + cg) * MM_BLK;
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                    for (int v = 0; v < 4; ++v)
+                        pp[(ct * 16 + m) * MM_KG + (tk[j] % MM_KG) + g + 4 * v] = acc[j][ct][v];
+            }
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[j][ct] = d4{0.0, 0.0, 0.0, 0.0};
+        };
+        auto segment_ends = [&](int rend) {
+#pragma unroll
+            for (int j = 0; j < MMF_TPW; ++j)
+                if (tv[j] && (rend - rb0) % min(a.seglen[td[j]], MMF_RB) == 0) flush(j, rend);
+        };
+        __syncthreads(); // the previous pass is done with the buffer
+        xload(rb0);
+        pload(pA, 0);
+        xstore(xs);
+        __syncthreads();
+        // blocks 0..2: the next block of X and the next half-blocks of P are
+        // in flight while the current ones are used (the roles of the two
+        // P buffers are the same in every iteration, so nothing is copied)
+#pragma unroll 1
+        for (int blk = 0; blk < 3; ++blk) {
+            xload(rb0 + 64 * (blk + 1));
+            pload(pB, 2 * blk + 1);
+            compute(xs, pA, 0);
+            pload(pA, 2 * blk + 2);
+            compute(xs, pB, 1);
+            segment_ends(rb0 + 64 * (blk + 1));
+            __syncthreads();
+            xstore(xs);
+            __syncthreads();
+        }
+        pload(pB, 7);
+        compute(xs, pA, 0);
+        compute(xs, pB, 1);
+        segment_ends(rb0 + 256);
+    }
+}
 
 // Sum the partial blocks of every segment (fixed order) into the
 // coefficients.  One work-group per (level, segment, rank group, column
@@ -223,9 +378,8 @@ __device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double al
     const int k = kg * MM_KG + kl, c = cg * MM_CG + cl;
     if (k >= R || c >= a.ncols) return;
     const int sp = raw ? s : (s ^ 1);
-    const int sidx = trans ? s : (s ^ 1);
     double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
-    wc[k] = raw ? t : alpha * a.S[d][(size_t)sidx * R + k] * t;
+    wc[k] = raw ? t : alpha * t;
 }
 
 
@@ -235,7 +389,7 @@ __device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double al
 // leaf) are staged in LDS per 64-wide chunk with coalesced loads; the B
 // operands (Q^T, D^T) are read from HBM, 16 consecutive rows per lane group.
 __device__ __forceinline__ void mm_stage(double *sh, const double *src, size_t cstride, int nc,
-                                         int kn)
+                                         int kn, const double *scale = nullptr)
 {
     // sh[c*65 + kk] = src[c*cstride + kk] for c < nc, kk < kn, else 0
     for (int e0 = 0; e0 < 64 * 64; e0 += 8 * 256) {
@@ -246,7 +400,8 @@ __device__ __forceinline__ void mm_stage(double *sh, const double *src, size_t c
             // unconditional load, padding zeroed by a 0/1 factor (a select
             // would let the compiler branch around the load)
             const double x = src[(size_t)min(c, nc - 1) * cstride + min(kk, kn - 1)];
-            v[u] = x * (double)(c < nc && kk < kn);
+            const double f = scale ? scale[min(kk, kn - 1)] : 1.0;
+            v[u] = x * f * (double)(c < nc && kk < kn);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
@@ -311,7 +466,10 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
         for (int k0 = 0; k0 < R; k0 += 64) {
             const int kn = min(64, R - k0);
             __syncthreads();
-            mm_stage(sh, wd + k0, wstride, nc, kn);
+            // apply tables: coefficient = sig (of the block whose U, or V
+            // when transposed, lives on segment s) times the stored alpha * t
+            mm_stage(sh, wd + k0, wstride, nc, kn,
+                     a.sx ? a.S[d] + (size_t)(trans ? (s ^ 1) : s) * R + k0 : nullptr);
             __syncthreads();
             for (int k = 0; k < kn; k += 16) {
                 double bq[4];
@@ -363,6 +521,7 @@ static inline void mm_tab(const dplrh_plan *p, int d0, int d1, const double *con
     t->b = p->b;
     t->ncols = ncols;
     t->ncg = (ncols + MM_CG - 1) / MM_CG;
+    t->sx = !w_local;
     int task = 0, part = 0, fin = 0, woff = 0;
     for (int i = 0; i < t->L; ++i) {
         const int d = d0 + i;

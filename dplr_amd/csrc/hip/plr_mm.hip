@@ -29,6 +29,14 @@ __global__ __launch_bounds__(64 * NW) void k_mm_reduce(MmTab a, const double *__
     mm_reduce_body<NW>(a, blockIdx.x, X, ldx, alpha, trans, raw, part, w, wstride);
 }
 
+__global__ __launch_bounds__(256) void k_mm_reduce_fused(MmTab a, const double *__restrict__ X,
+                                                         int ldx, double alpha, int trans, int raw,
+                                                         double *__restrict__ part,
+                                                         double *__restrict__ w, size_t wstride)
+{
+    mm_reduce_fused_body(a, blockIdx.x, blockIdx.y, X, ldx, alpha, trans, raw, part, w, wstride);
+}
+
 __global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int trans, int raw,
                                                    const double *__restrict__ part,
                                                    double *__restrict__ w, size_t wstride)
@@ -86,8 +94,12 @@ int plr_mm_apply(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
     int rc = mm_part(p, (size_t)t.part_off[t.L]);
     if (rc) return rc;
     if (t.task_off[t.L] > 0) {
-        hipLaunchKernelGGL(k_mm_reduce<1>, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx, alpha,
-                           trans, 0, p->tmp, w, wstride);
+        if (p->N % MMF_RB == 0 && ldx % 2 == 0 && !p->force_valu)
+            hipLaunchKernelGGL(k_mm_reduce_fused, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF_NW), 0, st, t, X,
+                               ldx, alpha, trans, 0, p->tmp, w, wstride);
+        else
+            hipLaunchKernelGGL(k_mm_reduce<1>, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx,
+                               alpha, trans, 0, p->tmp, w, wstride);
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[t.L] > 0) {
