@@ -47,6 +47,8 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--batch-count", type=int, default=512,
                     help="matrices in the batched leg (0 skips it)")
+    ap.add_argument("--precision-legs", default="65536:32,131072:64",
+                    help="N:rank pairs of the mixed-precision comparison ('' skips it)")
     ap.add_argument("--grad-m", type=int, default=16384,
                     help="order of the Gaussian-kernel gradient leg (0 skips it)")
     return ap.parse_args()
@@ -214,6 +216,52 @@ def batch_leg(args, torch, D, log):
     return out
 
 
+def precision_leg(args, torch, D, log):
+    """Mixed precision against the plain FP64 solve, per (N, rank) and per
+    16-bit type: refined solves needed for rel err <= 1e-10 against the FP64
+    solve, the error reached, and the solve times (64 RHS)."""
+    import numpy as np
+    out = []
+    nrhs = args.nrhs
+    for item in [x for x in args.precision_legs.split(",") if x]:
+        m, r = (int(v) for v in item.split(":"))
+        nlvl = int(round(np.log2(m // 64)))
+        t = D.PlrTree.random(m, nlvl, r, 4.0, 7)
+        P = D.Plan(t)
+        t.free()
+        assert P.factor() == 0
+        g = torch.Generator(device="cpu").manual_seed(3)
+        B = torch.randn(nrhs, m, generator=g, dtype=torch.float64).to("cuda")
+        Z64 = torch.zeros_like(B)
+        Z = torch.zeros_like(B)
+        P.solve("N", B, Z64)
+        ms64 = event_ms(torch, lambda i: P.solve("N", B, Z64), 5)
+        for lp in ("f16", "bf16"):
+            assert P.factor_mixed(lp) == 0
+            iters, err = None, None
+            for it in range(1, 16):
+                P.solve_refined(B, Z, it)
+                torch.cuda.synchronize()
+                err = float((Z - Z64).norm() / Z64.norm())
+                if err <= TARGET_ERR:
+                    iters = it
+                    break
+            if iters is None:
+                out.append({"m": m, "rank": r, "lowprec": lp, "reached": False, "rel_err": err})
+                continue
+            ms = event_ms(torch, lambda i: P.solve_refined(B, Z, iters), 5)
+            ms1 = event_ms(torch, lambda i: P.solve_refined(B, Z, 1), 5)
+            rec = {"m": m, "rank": r, "lowprec": lp, "refine_solves": iters, "rel_err": err,
+                   "mixed_solve_ms": ms, "single_lowprec_solve_ms": ms1, "fp64_solve_ms": ms64,
+                   "speedup_vs_fp64_solve": ms64 / ms}
+            out.append(rec)
+            log(f"precision N={m} r={r} {lp}: {iters} solves -> {err:.1e}, {ms:.3f} ms vs FP64 "
+                f"{ms64:.3f} ms ({ms64 / ms:.2f}x); one {lp} solve {ms1:.3f} ms")
+        P.close()
+        torch.cuda.empty_cache()
+    return out
+
+
 def grad_leg(args, torch, D, log):
     """Gaussian-kernel PLR (x ~ U[0,1] sorted, l = 0.05, nugget 1e-2,
     compressed at tol 1e-10 with leaf 128): forward y = A^{-1} b and
@@ -375,6 +423,11 @@ def main():
     t0 = time.perf_counter()
     ms_step = event_ms(torch, step, args.steps)
     wall = time.perf_counter() - t0
+    # per-step distribution (each step bracketed by its own events)
+    per = []
+    for i in range(args.steps):
+        per.append(event_ms(torch, lambda j: step(i), 1))
+    p50, p99 = float(np.percentile(per, 50)), float(np.percentile(per, 99))
     rhs_s = nrhs / (ms_step * 1e-3)
     plain_ms = ms_apply + ms_solve64
     plain_rhs_s = nrhs / (plain_ms * 1e-3)
@@ -388,6 +441,8 @@ def main():
         plans = []
         torch.cuda.empty_cache()
         batch = batch_leg(args, torch, D, log)
+
+    precision = precision_leg(args, torch, D, log) if args.precision_legs else None
 
     grad = None
     if args.grad_m > 0:
@@ -408,6 +463,8 @@ def main():
         "steps": args.steps,
         "warmup": args.warmup,
         "ms_per_step": ms_step,
+        "ms_per_step_p50": p50,
+        "ms_per_step_p99": p99,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -431,6 +488,7 @@ def main():
                      "bytes_per_launch": mv_bytes, "us_per_launch": ms_mv * 1e3},
         "cpu_baseline": cpu,
         "batch": batch,
+        "precision": precision,
         "gradient": grad,
         "wall_s_timed": wall,
     }

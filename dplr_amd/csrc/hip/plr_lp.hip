@@ -224,6 +224,13 @@ __global__ __launch_bounds__(256) void k_lp_expand(LpTab a, const T16 *__restric
     f4 acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) acc[ct] = f4{0.f, 0.f, 0.f, 0.f};
+    // the old Z values are loaded first (used only when beta != 0), so their
+    // round trip overlaps the operand loads below
+    float zo[4][4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) zo[ct][v] = Z[(size_t)cd[ct][v] * ldz + r0 + m];
     if (Dl) {
         const int leaf = r0 / b, il = r0 - leaf * b;
         const T16 *D = Dl + (size_t)leaf * b * b; // row-major
@@ -266,18 +273,13 @@ __global__ __launch_bounds__(256) void k_lp_expand(LpTab a, const T16 *__restric
 #pragma unroll
             for (int v = 0; v < 4; ++v) acc[ct][v] += al[ct][v] * sc[ct][v];
     }
-    float zo[4][4];
-#pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-        for (int v = 0; v < 4; ++v)
-            zo[ct][v] = beta == 0.f ? 0.f : beta * Z[(size_t)cd[ct][v] * ldz + r0 + m];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = c0 + 16 * ct + 4 * g + v;
-            if (c < a.ncols) Z[(size_t)c * ldz + r0 + m] = zo[ct][v] + acc[ct][v];
+            if (c < a.ncols)
+                Z[(size_t)c * ldz + r0 + m] = (beta == 0.f ? 0.f : beta * zo[ct][v]) + acc[ct][v];
         }
 }
 

@@ -425,34 +425,44 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
     const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
     // A operand (16 columns c, 4 k): lane holds sh[(16ct + m)*65 + k + g]
     // B operand (4 k x 16 rows of Q^T or D^T): lane holds [k + g][r0 + m]
+    // Every global load of a chunk (its B operands, and Y at the start) is
+    // issued before the LDS staging of the chunk's A operands, so one round
+    // trip serves both; barriers keep the compiler from sinking them.
     d4 acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
+    // lane holds Y^T[c0 + 16ct + g + 4v][r0 + m]
+    double yo[4][4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
+            yo[ct][v] = Y[(size_t)c * ldy + r0 + m];
+        }
 
     if (D) {
         const int leaf = rb / b, il = r0 - leaf * b;
         const size_t ds_i = trans ? b : 1, ds_j = trans ? 1 : b; // element (row, col) of op(D)
         const double *Dl = D + (size_t)leaf * b * b;
         const double *xl = X + (size_t)c0 * ldx + (size_t)leaf * b;
-        for (int j0 = 0; j0 < b; j0 += 64) {
-            const int jn = min(64, b - j0);
+        for (int j0 = 0; j0 < b; j0 += 32) {
+            const int jn = min(32, b - j0);
+            double bq[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int jj = j0 + min(4 * u, jn - 4) + g;
+                bq[u] = Dl[(size_t)(il + m) * ds_i + (size_t)jj * ds_j];
+            }
             __syncthreads();
             mm_stage(sh, xl + j0, ldx, nc, jn);
             __syncthreads();
-            for (int j = 0; j < jn; j += 16) {
-                double bq[4];
+            // rows past the chunk read zeros from sh (and real data from D)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int jj = j0 + min(j + 4 * u, jn - 4) + g;
-                    bq[u] = Dl[(size_t)(il + m) * ds_i + (size_t)jj * ds_j];
-                }
-                // rows past the chunk read zeros from sh (and real data from D)
+            for (int u = 0; u < 8; ++u)
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int ct = 0; ct < 4; ++ct)
-                        acc[ct] = mfma64(sh[(16 * ct + m) * 65 + j + 4 * u + g], bq[u], acc[ct]);
-            }
+                for (int ct = 0; ct < 4; ++ct)
+                    acc[ct] = mfma64(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
         }
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[ct] *= alpha;
@@ -463,40 +473,27 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
         const int s = rb / a.seglen[d];
         const double *Qd = a.Q[d];
         const double *wd = w + (size_t)c0 * wstride + a.w_off[d] + (size_t)s * R;
-        for (int k0 = 0; k0 < R; k0 += 64) {
-            const int kn = min(64, R - k0);
+        for (int k0 = 0; k0 < R; k0 += 32) {
+            const int kn = min(32, R - k0);
+            double bq[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const int kk = k0 + min(4 * u, kn - 4) + g;
+                bq[u] = Qd[(size_t)kk * N + r0 + m];
+            }
             __syncthreads();
             // apply tables: coefficient = sig (of the block whose U, or V
             // when transposed, lives on segment s) times the stored alpha * t
             mm_stage(sh, wd + k0, wstride, nc, kn,
                      a.sx ? a.S[d] + (size_t)(trans ? (s ^ 1) : s) * R + k0 : nullptr);
             __syncthreads();
-            for (int k = 0; k < kn; k += 16) {
-                double bq[4];
+            // ranks past the chunk read zeros from sh (and real data from Q)
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const int kk = k0 + min(k + 4 * u, kn - 4) + g;
-                    bq[u] = Qd[(size_t)kk * N + r0 + m];
-                }
-                // ranks past the chunk read zeros from sh (and real data from Q)
+            for (int u = 0; u < 8; ++u)
 #pragma unroll
-                for (int u = 0; u < 4; ++u)
-#pragma unroll
-                    for (int ct = 0; ct < 4; ++ct)
-                        acc[ct] = mfma64(sh[(16 * ct + m) * 65 + k + 4 * u + g], bq[u], acc[ct]);
-            }
+                for (int ct = 0; ct < 4; ++ct)
+                    acc[ct] = mfma64(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
         }
-    }
-    // lane holds Y^T[c0 + 16ct + g + 4v][r0 + m]
-    double yo[4][4];
-    if (beta != 0.0) {
-#pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
-                yo[ct][v] = Y[(size_t)c * ldy + r0 + m];
-            }
     }
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct)

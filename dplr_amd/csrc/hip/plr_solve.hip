@@ -501,6 +501,9 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
 }
 
 // X := op(A)^{-1} B for n device columns (ldb, ldx >= N), after dplrh_factor.
+int plr_solve_tail(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
+                   hipStream_t st);
+
 static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb, double *dX,
                       int ldx, hipStream_t st)
 {
@@ -518,8 +521,12 @@ static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb
         double *X = dX + (size_t)c0 * ldx;
         double *Z = X;
         int ldz = ldx;
+        int dstop = L; // levels [dstop, L) and the leaves done by the tail kernel
         if (!trans) {
-            if (mm)
+            const int d0 = mm ? plr_solve_tail(p, B, ldb, X, ldx, nc, st) : -1;
+            if (d0 >= 0)
+                dstop = d0;
+            else if (mm)
                 rc = plr_mm_expand1(p, -1, nullptr, nullptr, 0, p->f.Dinv, 0, B, ldb, 1.0, 0.0, X,
                                     ldx, nc, st);
             else
@@ -536,7 +543,7 @@ static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb
         for (int k = 0; k < L; ++k) {
             const int d = trans ? k : L - 1 - k;
             const int R = p->R[d];
-            if (R == 0) continue;
+            if (R == 0 || d >= dstop) continue;
             const double *Ud = p->f.Ut + p->uoff[d], *Vd = p->V + p->uoff[d];
             const double *Sd = p->S + p->soff[d];
             LevelTab t1;

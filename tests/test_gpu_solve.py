@@ -92,3 +92,24 @@ def test_full_size_solve_residual(cuda, orc):
         for j in range(2):
             r = D.gepmv(trans, A, X[:, j], api=orc) - B[:, j]
             assert np.linalg.norm(r) / np.linalg.norm(B[:, j]) < 1e-13
+
+
+@pytest.mark.parametrize("m,nlvl,n", [(512, 3, 20), (1024, 4, 16), (4096, 6, 33), (8192, 7, 2)])
+def test_solve_bottom_levels_in_one_kernel(cuda, orc, m, nlvl, n):
+    """Leaf 64 and several right-hand sides: the leaves and the levels whose
+    segments have <= 256 rows run in one kernel per 512-row span (all of the
+    tree for m = 512); ranks vary per block up to 48."""
+    import torch
+    print(f"tail solve m={m} L={nlvl} n={n}", flush=True)
+    rng = np.random.default_rng(m + n)
+    A = PlrTree.from_blocks(m, nlvl, lambda d, k, w: 1 + (5 * d + 3 * k + (w == "sw")) % 48, rng)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    B = torch.tensor(rng.standard_normal((n, m)), device=cuda)
+    X = torch.zeros_like(B)
+    P.solve("N", B, X)
+    Ad = A.dense(api=orc)
+    ref = np.linalg.solve(Ad, B.cpu().numpy().T).T
+    assert np.linalg.norm(X.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-12
+    P.close()
+    A.free()
