@@ -466,7 +466,8 @@ void plr_lp_release(const dplrh_plan *p)
         }
 }
 
-static void lp_tab(const dplrh_plan *p, const plr_lp *q, int d0, int d1, int ncols, LpTab *t)
+static void lp_tab(const dplrh_plan *p, const plr_lp *q, int d0, int d1, int ncols, LpTab *t,
+                   int tmax = LP_TMAX)
 {
     memset(t, 0, sizeof(*t));
     t->L = d1 - d0;
@@ -478,7 +479,7 @@ static void lp_tab(const dplrh_plan *p, const plr_lp *q, int d0, int d1, int nco
     int task = 0, part = 0, fin = 0;
     for (int i = 0; i < t->L; ++i) {
         const int d = d0 + i, seg = p->N >> (d + 1), nseg = 2 << d, R = q->Rp[d];
-        const int T = seg < LP_TMAX ? seg : LP_TMAX, tps = seg / T, nkg = R / LP_KG;
+        const int T = seg < tmax ? seg : tmax, tps = seg / T, nkg = R / LP_KG;
         t->R[i] = R;
         t->seglen[i] = seg;
         t->T[i] = T;
@@ -756,6 +757,9 @@ static int lp_solve_t(const dplrh_plan *p, plr_lp *q, int n, hipStream_t st)
     return rc;
 }
 
+static int lp_solve_v2_any(const dplrh_plan *p, plr_lp *q, int n, int first, double *X, int ldx,
+                           hipStream_t st);
+
 // X := A^{-1} B by mixed-precision solves refined with FP64 residuals:
 // `iters` solves (iters-1 residuals).  When res != NULL, res[c] receives
 // max|b - A x| of column c after the last solve (one more FP64 residual).
@@ -778,6 +782,12 @@ static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int 
             if (it > 0 && (rc = plr_gemm_direct(p, 0, nc, -1.0, 1.0, X, ldx, q->r, N, st))) return rc;
             hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, q->r, N, N,
                                q->pmax);
+            if (it < iters) {
+                rc = lp_solve_v2_any(p, q, nc, it == 0, X, ldx, st);
+                if (rc < 0) return rc;
+                if (rc == 0) continue;
+                rc = 0;
+            }
             hipLaunchKernelGGL(k_lp_scale_rhs, dim3((N + 255) / 256, nc), dim3(256), 0, st, q->r, N,
                                N, q->pmax, q->scale, q->r32);
             PLR_HIP(hipGetLastError());
@@ -814,4 +824,756 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
     return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
         return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, nullptr, s);
     });
+}
+
+// ---------------------------------------------------------------------------
+// Solve, second form: the bottom levels in LDS, then one pass per upper level.
+//
+//   k_lp_tail  one work-group per (span of S rows, CW right-hand sides):
+//              z = r / max|r| per column, z|_leaf = Dinv z|_leaf, then every
+//              level d >= d0 (segments inside the span) with z in LDS, and
+//              finally T of level d0 - 1 (the span is one of its segments).
+//   k_lp_level one work-group per (256 rows, 64 right-hand sides) of level
+//              d < d0: the rows of inv(K) for its segment (recomputed per
+//              work-group), z += Ut_d W_d, then the partial sums of
+//              V_{d-1}^T z over its rows (summed by k_lp_finish), or, at
+//              d = 0, x (+)= max|r| * z in FP64.
+// Two kernels per upper level instead of four, and one pass over z.
+
+static constexpr int LT_THREADS = 512; // eight waves
+static constexpr int LT_MAXT = 4;      // 16-row leaf tiles per wave (span <= 512)
+static constexpr int LV_RB = 256, LV_CW = 64, LV_THREADS = 512;
+
+struct LpTailTab
+{
+    int N, b, L, d0, span, RT, tsegs; // RT: largest Rp over levels d0-1 .. L-1
+    int Rp[PLR_MAXL];
+    const void *V[PLR_MAXL], *Ut[PLR_MAXL];
+    const float *S[PLR_MAXL];
+    const float *K32[PLR_MAXL];
+    const double *K64[PLR_MAXL];
+    const void *D;
+};
+
+struct LpLvlTab
+{
+    int N, R, Rn, seg, segn, tps, nkg, ncg;
+    const void *Ut, *Vn;
+    const float *S;
+    const float *K32;
+    const double *K64;
+};
+
+__device__ __forceinline__ f4 mfma_f32_4(float a, float b, f4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+// smallest power of two >= v (1 for v = 0); v >= 0 as the bits of a float
+__device__ __forceinline__ float lp_pow2(unsigned bits)
+{
+    const float v = __uint_as_float(bits);
+    if (!(v > 0.f)) return v == 0.f ? 1.f : v; // NaN stays NaN
+    int e;
+    const float f = frexpf(v, &e);
+    return ldexpf(1.f, f == 0.5f ? e - 1 : e);
+}
+
+// Capacitance rows for P tasks at once: task p computes the 16 x 16 tile
+// c = inv(K)[i0[p] .. +15, :] rhs[:, 16 columns], where row j of rhs is
+// lo[p][j * rs + m] for j < R and hi[p][(j - R) * rs + m] above (LDS), and
+// multiplies row i by -sv[p][v] (sig of that row).  Every K and sig load of
+// the P tasks is issued before the first MFMA.  K is column-major n2 x n2:
+// FP64 (FP64 MFMA) when K64 != null, else FP32 (FP32 MFMA).  Lane result:
+// w[p][v] = -sig * c[i0[p] + 4g + v][m].
+template <int P>
+__device__ __forceinline__ void lp_cap_tiles(const float *const *K32, const double *const *K64,
+                                             int n2, int R, const int *i0,
+                                             const float *const *lo, const float *const *hi,
+                                             int rs, const float *const *sv, int m, int g,
+                                             f4 *w)
+{
+    float s[P][4];
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) s[p][v] = sv[p][v];
+    if (K64[0]) {
+        plr_d4 acc[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] = plr_d4{0.0, 0.0, 0.0, 0.0};
+        for (int j0 = 0; j0 < n2; j0 += 64) {
+            double av[P][16];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    av[p][u] = K64[p][(size_t)(j0 + 4 * u + g) * n2 + i0[p] + m];
+#pragma unroll
+            for (int p = 0; p < P; ++p)
+#pragma unroll
+                for (int u = 0; u < 16; ++u) {
+                    const int j = j0 + 4 * u + g;
+                    const float *row = j < R ? lo[p] + j * rs : hi[p] + (j - R) * rs;
+                    acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[p][u], (double)row[m], acc[p], 0, 0, 0);
+                }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) w[p][v] = (float)(-(double)s[p][v] * acc[p][v]);
+        return;
+    }
+    f4 acc[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) acc[p] = f4{0.f, 0.f, 0.f, 0.f};
+    for (int j0 = 0; j0 < n2; j0 += 64) {
+        float av[P][16];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int u = 0; u < 16; ++u) av[p][u] = K32[p][(size_t)(j0 + 4 * u + g) * n2 + i0[p] + m];
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int j = j0 + 4 * u + g;
+                const float *row = j < R ? lo[p] + j * rs : hi[p] + (j - R) * rs;
+                acc[p] = mfma_f32_4(av[p][u], row[m], acc[p]);
+            }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) w[p][v] = -s[p][v] * acc[p][v];
+}
+
+// LDS of k_lp_tail: z [CW][S+8], T [tsegs][RT][CW+4], W [tsegs][CW][RT+8],
+// smax [tsegs][CW] (floats).
+static size_t lpt_lds(int S, int CW, int RT, int tsegs)
+{
+    return ((size_t)CW * (S + 8) + (size_t)tsegs * RT * (CW + 4) + (size_t)tsegs * CW * (RT + 8) +
+            (size_t)tsegs * CW) * sizeof(float);
+}
+
+// T[c][k] sums over segments of `seg` rows of the span: sum_i V[k][row0 + i]
+// z[c][i], in batches of eight 32-row MFMA steps per wave (the eight V loads
+// of a batch go out together).  Segments of at most 256 rows end inside a
+// batch and are written by emit(s, kt, ct, acc); longer ones leave one
+// partial per batch, emitted with s = -1 - (batch index in the segment).
+template <typename T16, int NCT, typename Emit>
+__device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, const float *z,
+                                               int zs, int S, int seg, int nkt, int wv, int m,
+                                               int g, Emit emit)
+{
+    typedef typename Lp<T16>::v8 v8;
+    const int nstep = S / 32, spq = seg / 32, total = nstep * nkt * NCT;
+    for (int e0 = wv * 8; e0 < total; e0 += 64) {
+        v8 av[8];
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = e0 + q, grp = e / nstep, rs = e - grp * nstep, kt = grp / NCT;
+            av[q] = *(const v8 *)(V + (size_t)(kt * 16 + m) * N + row0 + rs * 32 + 8 * g);
+        }
+        f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int e = e0 + q, grp = e / nstep, rs = e - grp * nstep;
+            const int kt = grp / NCT, ct = grp - kt * NCT;
+            const f8 zv = *(const f8 *)(z + (ct * 16 + m) * zs + rs * 32 + 8 * g);
+            acc = Lp<T16>::mfma(av[q], cvt8<T16>(zv, 1.f), acc);
+            if ((rs + 1) % spq == 0 && spq <= 8) {
+                emit(rs / spq, kt, ct, acc);
+                acc = f4{0.f, 0.f, 0.f, 0.f};
+            } else if (q == 7 && spq > 8) {
+                emit(-1 - (rs % spq) / 8, kt, ct, acc);
+            }
+        }
+    }
+}
+
+template <typename T16, int CW>
+__global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const double *__restrict__ r,
+                                                        int ldr, const double *__restrict__ pmax,
+                                                        double *__restrict__ scale,
+                                                        float *__restrict__ Z, int ldz,
+                                                        float *__restrict__ Tout, size_t tstride,
+                                                        int ncols)
+{
+    typedef typename Lp<T16>::v8 v8;
+    constexpr int NCT = CW / 16;
+    extern __shared__ float lds[];
+    __shared__ double cinv[CW];
+    const int S = a.span, zs = S + 8, RT = a.RT, nsm = a.tsegs;
+    const int ts = CW + 4, tsz = RT * ts;   // T[s][k][c]
+    const int wsk = RT + 8, wsz = CW * wsk; // W[s][c][k]
+    float *z = lds;
+    float *T = z + CW * zs;
+    float *W = T + nsm * tsz;
+    unsigned *smax = (unsigned *)(W + nsm * wsz); // [s][c]
+    const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wv = tid >> 6;
+    const int row0 = blockIdx.x * S, c0 = blockIdx.y * CW, N = a.N, b = a.b;
+    const int nc = min(CW, ncols - c0);
+
+    // 1. column scales (max |r| from the 64 chunk maxima, eight threads per
+    // column), then z = r / max|r|
+    {
+        double *cp = (double *)T; // [CW][8], T is free until the first level
+        const int c = tid >> 3, part = tid & 7;
+        if (c < CW) {
+            const double *pm = pmax + (size_t)(c0 + min(c, nc - 1)) * LP_SCALE_CHUNKS + part * 8;
+            double pv[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) pv[u] = pm[u];
+            double mv = 0.0;
+#pragma unroll
+            for (int u = 0; u < 8; ++u) mv = fmax(mv, pv[u]);
+            cp[c * 8 + part] = mv;
+        }
+        __syncthreads();
+        if (tid < CW) {
+            double mv = 0.0;
+#pragma unroll
+            for (int j = 0; j < 8; ++j) mv = fmax(mv, cp[tid * 8 + j]);
+            if (tid < nc && blockIdx.x == 0) scale[c0 + tid] = mv;
+            cinv[tid] = tid < nc ? 1.0 / (mv > 0.0 ? mv : 1.0) : 0.0;
+        }
+        __syncthreads();
+    }
+    {
+        const int hs = S / 2, tot = CW * hs;
+        for (int e0 = 0; e0 < tot; e0 += 16 * LT_THREADS) {
+            double2 v[16];
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = min(e0 + u * LT_THREADS + tid, tot - 1);
+                const int c = e / hs, i = 2 * (e - c * hs);
+                v[u] = *(const double2 *)(r + (size_t)(c0 + min(c, nc - 1)) * ldr + row0 + i);
+            }
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int e = e0 + u * LT_THREADS + tid;
+                if (e < tot) {
+                    const int c = e / hs, i = 2 * (e - c * hs);
+                    z[c * zs + i] = (float)(v[u].x * cinv[c]);
+                    z[c * zs + i + 1] = (float)(v[u].y * cinv[c]);
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    // 2. leaves: each wave takes up to four 16-row tiles (M = rows, N =
+    // columns, K = leaf columns), all D loads of a 64-column step together;
+    // results stay in registers until every wave has read its leaf
+    {
+        const T16 *Dh = (const T16 *)a.D;
+        const int ntile = S / 16;
+        f4 acc[LT_MAXT][NCT];
+        int i0[LT_MAXT], lf[LT_MAXT];
+#pragma unroll
+        for (int q = 0; q < LT_MAXT; ++q) {
+            i0[q] = min(wv + 8 * q, ntile - 1) * 16;
+            lf[q] = i0[q] / b;
+#pragma unroll
+            for (int ct = 0; ct < NCT; ++ct) acc[q][ct] = f4{0.f, 0.f, 0.f, 0.f};
+        }
+        for (int j0 = 0; j0 < b; j0 += 64) {
+            v8 a0[LT_MAXT], a1[LT_MAXT];
+#pragma unroll
+            for (int q = 0; q < LT_MAXT; ++q) {
+                const T16 *D = Dh + ((size_t)(row0 / b + lf[q]) * b + i0[q] - lf[q] * b + m) * b + 8 * g + j0;
+                a0[q] = *(const v8 *)D;
+                a1[q] = *(const v8 *)(D + 32);
+            }
+#pragma unroll
+            for (int q = 0; q < LT_MAXT; ++q) {
+                const float *zb = z + m * zs + lf[q] * b + 8 * g + j0;
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) {
+                    const f8 x0 = *(const f8 *)(zb + ct * 16 * zs);
+                    const f8 x1 = *(const f8 *)(zb + ct * 16 * zs + 32);
+                    acc[q][ct] = Lp<T16>::mfma(a0[q], cvt8<T16>(x0, 1.f), acc[q][ct]);
+                    acc[q][ct] = Lp<T16>::mfma(a1[q], cvt8<T16>(x1, 1.f), acc[q][ct]);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < LT_MAXT; ++q)
+            if (wv + 8 * q < ntile)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct)
+                    *(f4 *)(z + (ct * 16 + m) * zs + i0[q] + 4 * g) = acc[q][ct];
+        __syncthreads();
+    }
+
+    // 3. levels L-1 .. d0
+    for (int d = a.L - 1; d >= a.d0; --d) {
+        const int R = a.Rp[d], seg = N >> (d + 1), nsg = S / seg, nkt = R / 16;
+        if (R == 0) continue;
+        const int sg0 = row0 / seg; // global index of the span's first segment
+        const T16 *V = (const T16 *)a.V[d], *U = (const T16 *)a.Ut[d];
+        // (a) T[s][k][c] = sum over segment s of V[k][i] z[c][i]
+        for (int e = tid; e < nsg * CW; e += LT_THREADS) smax[e] = 0u;
+        lp_tail_reduce<T16, NCT>(V, N, row0, z, zs, S, seg, nkt, wv, m, g,
+                                 [&](int s, int kt, int ct, f4 acc) {
+                                     float *t = T + s * tsz + (kt * 16 + 4 * g) * ts + ct * 16 + m;
+#pragma unroll
+                                     for (int v = 0; v < 4; ++v) t[v * ts] = acc[v];
+                                 });
+        __syncthreads();
+        // (b) c = inv(K) [T_bot; T_top] per node, W_top = -S_top .* c[0:R],
+        // W_bot = -S_bot .* c[R:2R], with the largest |W| per (segment, column);
+        // two tasks per wave and pass
+        {
+            const int n2 = 2 * R, nit = n2 / 16, ntask = (nsg / 2) * nit * NCT;
+            for (int t0 = wv * 2; t0 < ntask; t0 += 16) {
+                const float *k32[2], *lo[2], *hi[2], *sv[2];
+                const double *k64[2];
+                int i0[2], sl[2], ct[2], kk[2];
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    const int task = min(t0 + p, ntask - 1);
+                    ct[p] = task % NCT;
+                    const int it = (task / NCT) % nit, nl = task / (NCT * nit);
+                    const int top = 2 * nl, bot = top + 1;
+                    const size_t ko = (size_t)(sg0 / 2 + nl) * n2 * n2;
+                    k32[p] = a.K32[d] ? a.K32[d] + ko : nullptr;
+                    k64[p] = a.K64[d] ? a.K64[d] + ko : nullptr;
+                    i0[p] = it * 16;
+                    lo[p] = T + bot * tsz + ct[p] * 16;
+                    hi[p] = T + top * tsz + ct[p] * 16;
+                    // the four rows of a lane lie in one half
+                    const int i = it * 16 + 4 * g, h = i >= R;
+                    kk[p] = h ? i - R : i;
+                    sl[p] = h ? bot : top;
+                    sv[p] = a.S[d] + (size_t)(sg0 + sl[p]) * R + kk[p];
+                }
+                f4 w[2];
+                lp_cap_tiles<2>(k32, k64, n2, R, i0, lo, hi, ts, sv, m, g, w);
+#pragma unroll
+                for (int p = 0; p < 2; ++p) {
+                    if (t0 + p >= ntask) continue;
+                    *(f4 *)(W + sl[p] * wsz + (ct[p] * 16 + m) * wsk + kk[p]) = w[p];
+                    const float mx = fmaxf(fmaxf(fabsf(w[p][0]), fabsf(w[p][1])),
+                                           fmaxf(fabsf(w[p][2]), fabsf(w[p][3])));
+                    atomicMax(&smax[sl[p] * CW + ct[p] * 16 + m], __float_as_uint(mx));
+                }
+            }
+        }
+        __syncthreads();
+        // (c) z[c][i] += sum_k Ut[i][k] W[s(i)][c][k]  (M = columns, N = rows),
+        // batches of eight MFMA steps per wave with their Ut loads together
+        {
+            const int nks = R / 32, total = (S / 16) * NCT * nks;
+            for (int e0 = wv * 8; e0 < total; e0 += 64) {
+                v8 bq[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int e = e0 + q, ks = e % nks, it = (e / nks) / NCT;
+                    bq[q] = *(const v8 *)(U + (size_t)(row0 + it * 16 + m) * R + ks * 32 + 8 * g);
+                }
+                f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int e = e0 + q, ks = e % nks, grp = e / nks, it = grp / NCT, ct = grp - it * NCT;
+                    const int s = (it * 16) / seg;
+                    const unsigned *sm = smax + s * CW + ct * 16;
+                    const f8 w8 = *(const f8 *)(W + s * wsz + (ct * 16 + m) * wsk + ks * 32 + 8 * g);
+                    acc = Lp<T16>::mfma(cvt8<T16>(w8, 1.f / lp_pow2(sm[m])), bq[q], acc);
+                    if (ks == nks - 1) {
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            z[(ct * 16 + 4 * g + v) * zs + it * 16 + m] += acc[v] * lp_pow2(sm[4 * g + v]);
+                        acc = f4{0.f, 0.f, 0.f, 0.f};
+                    }
+                }
+            }
+        }
+        __syncthreads();
+    }
+
+    // 4. T of level d0 - 1 over the span (one of its segments); a segment of
+    // more than 256 rows leaves partials combined in a fixed order
+    if (a.d0 > 0) {
+        const int dn = a.d0 - 1, R = a.Rp[dn], nkt = R / 16, sg = row0 / S;
+        const int nh = S > 256 ? S / 256 : 1; // partials per (kt, ct)
+        float *red = T;                        // free now: [kt][ct][h][lane][4]
+        if (R > 0) {
+            lp_tail_reduce<T16, NCT>((const T16 *)a.V[dn], N, row0, z, zs, S, S, nkt, wv, m, g,
+                                     [&](int s, int kt, int ct, f4 acc) {
+                                         if (s >= 0) {
+                                             const int c = ct * 16 + m;
+                                             if (c < nc)
+                                                 *(f4 *)(Tout + (size_t)(c0 + c) * tstride +
+                                                         (size_t)sg * R + kt * 16 + 4 * g) = acc;
+                                         } else {
+                                             const int h = -1 - s;
+                                             *(f4 *)(red + ((((kt * NCT + ct) * nh + h) * 64) + lane) * 4) = acc;
+                                         }
+                                     });
+        }
+        if (nh > 1) {
+            __syncthreads();
+            for (int tt = wv; tt < nkt * NCT; tt += 8) {
+                const int kt = tt / NCT, ct = tt - kt * NCT, c = ct * 16 + m;
+                f4 x = *(const f4 *)(red + ((tt * nh) * 64 + lane) * 4);
+                for (int h = 1; h < nh; ++h) x += *(const f4 *)(red + (((tt * nh + h) * 64) + lane) * 4);
+                if (c < nc)
+                    *(f4 *)(Tout + (size_t)(c0 + c) * tstride + (size_t)sg * R + kt * 16 + 4 * g) = x;
+            }
+        }
+    }
+
+    // 5. z of the span
+    for (int e = tid; e < CW * (S / 4); e += LT_THREADS) {
+        const int c = e / (S / 4), i = 4 * (e - c * (S / 4));
+        if (c < nc) *(f4 *)(Z + (size_t)(c0 + c) * ldz + row0 + i) = *(const f4 *)(z + c * zs + i);
+    }
+}
+
+// LDS of k_lp_level: rhs [2R][CW+4], W [CW][R+8] (floats), zt [CW][RB+8] (16-bit)
+static size_t lpl_lds(int R)
+{
+    return ((size_t)2 * R * (LV_CW + 4) + (size_t)LV_CW * (R + 8)) * sizeof(float) +
+           (size_t)LV_CW * (LV_RB + 8) * 2;
+}
+
+// Every global load of the work-group (old z, Ut, V of level d-1, T, K, sig)
+// is issued at the start, so the kernel waits on memory about once.
+static constexpr int LV_MAXR = 64;
+
+template <typename T16>
+__global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float *__restrict__ Tin,
+                                                         size_t tstride, float *__restrict__ Z,
+                                                         int ldz, float *__restrict__ part,
+                                                         const double *__restrict__ scale,
+                                                         double *__restrict__ X, int ldx, int first,
+                                                         int ncols)
+{
+    typedef typename Lp<T16>::v8 v8;
+    constexpr int KS = LV_MAXR / 32;   // expansion k-steps (R <= 64)
+    constexpr int NV = LV_MAXR / 32;   // reduction tasks per wave (Rn <= 64)
+    extern __shared__ float lds[];
+    __shared__ unsigned smax[LV_CW];
+    const int R = a.R, n2 = 2 * R, rs = LV_CW + 4, wsk = R + 8, zts = LV_RB + 8;
+    float *rhs = lds;                   // [n2][rs]
+    float *W = rhs + n2 * rs;           // [CW][wsk]
+    T16 *zt = (T16 *)(W + LV_CW * wsk); // [CW][zts]
+    const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wv = tid >> 6;
+    const int r0 = blockIdx.x * LV_RB, c0 = blockIdx.y * LV_CW, N = a.N;
+    const int nc = min(LV_CW, ncols - c0);
+    const int s = r0 / a.seg, istop = !(s & 1), top = s & ~1, bot = top + 1;
+    const int nks = R / 32, nkt = R / 16, nktn = a.Rn / 16;
+
+    // old z (wave wv owns rows r0 + 32 wv .. + 31, columns as the expansion's D)
+    float zo[2][4][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = ct * 16 + 4 * g + v;
+                zo[rt][ct][v] = Z[(size_t)(c0 + min(c, nc - 1)) * ldz + r0 + wv * 32 + rt * 16 + m];
+            }
+    // Ut rows of the expansion
+    const T16 *U = (const T16 *)a.Ut;
+    v8 bq[2][KS];
+    if (R > 0) // a level of rank 0 has no Ut rows
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ks = 0; ks < KS; ++ks)
+                bq[rt][ks] = *(const v8 *)(U + (size_t)(r0 + wv * 32 + rt * 16 + m) * R +
+                                           min(ks, nks - 1) * 32 + 8 * g);
+    // V of level d-1 for this wave's reduction tasks (kt, ct) = wv + 8p
+    v8 av[NV][8];
+    if (a.Rn > 0) {
+        const T16 *Vn = (const T16 *)a.Vn;
+#pragma unroll
+        for (int p = 0; p < NV; ++p) {
+            const int task = min(wv + 8 * p, nktn * 4 - 1), kt = task >> 2;
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                av[p][u] = *(const v8 *)(Vn + (size_t)(kt * 16 + m) * N + r0 + 32 * u + 8 * g);
+        }
+    }
+    // 1. rhs = [T_bot; T_top] of the node
+    for (int e0 = 0; e0 < n2 * LV_CW; e0 += 8 * LV_THREADS) {
+        float v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = min(e0 + u * LV_THREADS + tid, n2 * LV_CW - 1);
+            const int cl = e / n2, j = e - cl * n2, hi = j >= R;
+            const int sg = hi ? top : bot, kk = hi ? j - R : j;
+            v[u] = Tin[(size_t)(c0 + min(cl, nc - 1)) * tstride + (size_t)sg * R + kk] *
+                   (cl < nc ? 1.f : 0.f);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * LV_THREADS + tid;
+            if (e < n2 * LV_CW) {
+                const int cl = e / n2, j = e - cl * n2;
+                rhs[j * rs + cl] = v[u];
+            }
+        }
+    }
+    if (tid < LV_CW) smax[tid] = 0u;
+    __syncthreads();
+    // 2. W_s = -S_s .* (rows of inv(K) for segment s) rhs, largest |W| per
+    // column: tasks (kt, ct), wave wv takes wv and wv + 8
+    if (R > 0) {
+        const size_t ko = (size_t)(s >> 1) * n2 * n2;
+        const int ntask = nkt * 4;
+        const float *k32[2], *lo[2], *hi[2], *sv[2];
+        const double *k64[2];
+        int i0[2], ct[2], kk[2];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int task = min(wv + 8 * p, ntask - 1);
+            ct[p] = task & 3;
+            kk[p] = (task >> 2) * 16;
+            i0[p] = (istop ? 0 : R) + kk[p];
+            k32[p] = a.K64 ? nullptr : a.K32 + ko;
+            k64[p] = a.K64 ? a.K64 + ko : nullptr;
+            lo[p] = rhs + ct[p] * 16;
+            hi[p] = rhs + R * rs + ct[p] * 16;
+            kk[p] += 4 * g;
+            sv[p] = a.S + (size_t)s * R + kk[p];
+        }
+        f4 w[2];
+        if (ntask > 8)
+            lp_cap_tiles<2>(k32, k64, n2, R, i0, lo, hi, rs, sv, m, g, w);
+        else
+            lp_cap_tiles<1>(k32, k64, n2, R, i0, lo, hi, rs, sv, m, g, w);
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            if (wv + 8 * p >= ntask) continue;
+            *(f4 *)(W + (ct[p] * 16 + m) * wsk + kk[p]) = w[p];
+            const float mx = fmaxf(fmaxf(fabsf(w[p][0]), fabsf(w[p][1])), fmaxf(fabsf(w[p][2]), fabsf(w[p][3])));
+            atomicMax(&smax[ct[p] * 16 + m], __float_as_uint(mx));
+        }
+    }
+    __syncthreads();
+    // 3. z += Ut W (M = columns, N = rows): two 16-row tiles per wave
+    {
+        float isc[4], sc[4][4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            isc[ct] = 1.f / lp_pow2(smax[ct * 16 + m]);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) sc[ct][v] = lp_pow2(smax[ct * 16 + 4 * g + v]);
+        }
+        f4 acc[2][4];
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            if (ks >= nks) break;
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const v8 aw = cvt8<T16>(*(const f8 *)(W + (ct * 16 + m) * wsk + ks * 32 + 8 * g), isc[ct]);
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) acc[rt][ct] = Lp<T16>::mfma(aw, bq[rt][ks], acc[rt][ct]);
+            }
+        }
+        // lane holds dz[c = ct*16 + 4g + v][i = 32 wv + 16 rt + m]
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = ct * 16 + 4 * g + v, i = wv * 32 + rt * 16 + m;
+                    const float zn = zo[rt][ct][v] * (c < nc ? 1.f : 0.f) + acc[rt][ct][v] * sc[ct][v];
+                    if (X) {
+                        if (c < nc) {
+                            double *xp = X + (size_t)(c0 + c) * ldx + r0 + i;
+                            const double sr = scale[c0 + c] > 0.0 ? scale[c0 + c] : 1.0;
+                            *xp = (first ? 0.0 : *xp) + sr * (double)zn;
+                        }
+                    } else {
+                        if (c < nc) Z[(size_t)(c0 + c) * ldz + r0 + i] = zn;
+                        zt[c * zts + i] = (T16)zn;
+                    }
+                }
+    }
+    if (a.Rn == 0) return;
+    __syncthreads();
+    // 4. partial V_{d-1}^T z over the 256 rows, in k_lp_finish's layout
+    // (tiles of 256 rows of the segment of level d-1)
+    {
+        const int sn = r0 / a.segn, tile = (r0 - sn * a.segn) / LV_RB, st = sn * a.tps + tile;
+#pragma unroll
+        for (int p = 0; p < NV; ++p) {
+            const int task = wv + 8 * p;
+            if (task >= nktn * 4) break;
+            const int ct = task & 3, kt = task >> 2;
+            const T16 *pz = zt + (ct * 16 + m) * zts + 8 * g;
+            f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = Lp<T16>::mfma(av[p][u], *(const v8 *)(pz + 32 * u), acc);
+            const int k = kt * 16 + 4 * g, kg = k / LP_KG, kl = k - kg * LP_KG;
+            float *pp = part + ((size_t)(st * a.nkg + kg) * a.ncg + blockIdx.y) * LP_BLK +
+                        (ct * 16 + m) * LP_KG + kl;
+            *(f4 *)pp = acc;
+        }
+    }
+}
+
+// Host side of the second form.
+struct lp_v2
+{
+    int ok, S, cw, d0, RT, tsegs;
+};
+
+static lp_v2 lp_v2_shape(const dplrh_plan *p, const plr_lp *q)
+{
+    lp_v2 v;
+    memset(&v, 0, sizeof(v));
+    const int N = p->N, L = p->L, b = p->b;
+    if (N == 0 || b % 64 != 0 || b > 512) return v;
+    for (int d = 0; d < L; ++d)
+        if (q->Rp[d] > 64) return v;
+    for (int S = 512; S >= b; S >>= 1) {
+        if (S > N) continue;
+        int d0 = 0;
+        while ((N >> d0) > S) ++d0;
+        if ((N >> d0) != S || d0 > L) continue;
+        if (S < N && S < LV_RB) continue; // the upper-level pass takes 256-row blocks
+        int RT = 32;
+        for (int d = d0 > 0 ? d0 - 1 : 0; d < L; ++d) RT = q->Rp[d] > RT ? q->Rp[d] : RT;
+        const int tsegs = S / b > 2 ? S / b : 2;
+        for (int cw = 64; cw >= 32; cw -= 32)
+            if (lpt_lds(S, cw, RT, tsegs) + 1024 <= 160 * 1024) { // + static LDS
+                v.ok = 1;
+                v.S = S;
+                v.cw = cw;
+                v.d0 = d0;
+                v.RT = RT;
+                v.tsegs = tsegs;
+                return v;
+            }
+    }
+    return v;
+}
+
+// partial blocks the upper levels need (k_lp_level writes, k_lp_finish sums)
+static size_t lp_v2_part(const dplrh_plan *p, const plr_lp *q, const lp_v2 &v, int n)
+{
+    size_t need = 0;
+    for (int d = 0; d + 1 < v.d0; ++d) {
+        LpTab t;
+        lp_tab(p, q, d, d + 1, n, &t, LV_RB);
+        if ((size_t)t.part_off[1] > need) need = t.part_off[1];
+    }
+    return need;
+}
+
+template <typename T16, int CW>
+static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_lp *q, int n,
+                          hipStream_t st)
+{
+    static bool attr = false;
+    if (!attr) {
+        // the kernel's static LDS counts against the 160 KB too
+        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_tail<T16, CW>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        attr = true;
+    }
+    hipLaunchKernelGGL((k_lp_tail<T16, CW>), grid, dim3(LT_THREADS), lds, st, tt, q->r, tt.N,
+                       q->pmax, q->scale, q->z32, tt.N, q->T, q->t_per_col, n);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+// One low-precision solve of the scaled residual q->r (column maxima in
+// q->pmax) added into X: X = (first ? 0 : X) + max|r| * solve_lp(r / max|r|).
+template <typename T16>
+static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, int first,
+                       double *X, int ldx, hipStream_t st)
+{
+    const int N = p->N, L = p->L;
+    const T16 *Vh = (const T16 *)q->Vh, *Uh = (const T16 *)q->Uh;
+    int rc = lp_part(p, q, lp_v2_part(p, q, v, n));
+    if (rc) return rc;
+    LpTailTab tt;
+    memset(&tt, 0, sizeof(tt));
+    tt.N = N;
+    tt.b = p->b;
+    tt.L = L;
+    tt.d0 = v.d0;
+    tt.span = v.S;
+    tt.RT = v.RT;
+    tt.tsegs = v.tsegs;
+    for (int d = 0; d < L; ++d) {
+        tt.Rp[d] = q->Rp[d];
+        tt.V[d] = Vh + q->off[d];
+        tt.Ut[d] = Uh + q->off[d];
+        tt.S[d] = q->Sf + q->soff[d];
+        tt.K32[d] = d >= 2 ? q->K32 + q->koff[d] : nullptr;
+        tt.K64[d] = d < 2 ? q->K64 + q->koff[d] : nullptr;
+    }
+    tt.D = q->Dh;
+    const size_t lds = lpt_lds(v.S, v.cw, v.RT, v.tsegs);
+    const dim3 grid(N / v.S, (n + v.cw - 1) / v.cw);
+    rc = v.cw == 64 ? lp_launch_tail<T16, 64>(tt, lds, grid, q, n, st)
+                    : lp_launch_tail<T16, 32>(tt, lds, grid, q, n, st);
+    if (rc) return rc;
+    if (v.d0 == 0) {
+        const size_t ne = (size_t)N * n;
+        hipLaunchKernelGGL(k_lp_update, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, q->z32,
+                           N, q->scale, X, ldx, n, first);
+        PLR_HIP(hipGetLastError());
+        return 0;
+    }
+    static bool attr = false;
+    if (!attr) {
+        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        attr = true;
+    }
+    const int ncg = (n + LV_CW - 1) / LV_CW;
+    for (int d = v.d0 - 1; d >= 0; --d) {
+        LpLvlTab lt;
+        memset(&lt, 0, sizeof(lt));
+        lt.N = N;
+        lt.R = q->Rp[d];
+        lt.Rn = d > 0 ? q->Rp[d - 1] : 0;
+        lt.seg = N >> (d + 1);
+        lt.segn = N >> d;
+        lt.tps = lt.segn / LV_RB;
+        lt.nkg = lt.Rn / LP_KG;
+        lt.ncg = ncg;
+        lt.Ut = Uh + q->off[d];
+        lt.Vn = d > 0 ? Vh + q->off[d - 1] : nullptr;
+        lt.S = q->Sf + q->soff[d];
+        lt.K32 = d >= 2 ? q->K32 + q->koff[d] : nullptr;
+        lt.K64 = d < 2 ? q->K64 + q->koff[d] : nullptr;
+        hipLaunchKernelGGL(k_lp_level<T16>, dim3(N / LV_RB, ncg), dim3(LV_THREADS), lpl_lds(lt.R),
+                           st, lt, q->T, q->t_per_col, q->z32, N, q->part, q->scale,
+                           d == 0 ? X : nullptr, ldx, first, n);
+        PLR_HIP(hipGetLastError());
+        if (d > 0 && lt.Rn > 0) {
+            LpTab t;
+            lp_tab(p, q, d - 1, d, n, &t, LV_RB);
+            hipLaunchKernelGGL(k_lp_finish, dim3(t.fin_off[1] * LP_FCH), dim3(256), 0, st, t, q->part,
+                               q->T, q->t_per_col);
+            PLR_HIP(hipGetLastError());
+        }
+    }
+    return 0;
+}
+
+static int lp_solve_v2_any(const dplrh_plan *p, plr_lp *q, int n, int first, double *X, int ldx,
+                           hipStream_t st)
+{
+    const lp_v2 v = lp_v2_shape(p, q);
+    if (!v.ok) return 1; // shape not covered: the caller takes the first form
+    return q->type == 1 ? lp_solve_v2<_Float16>(p, q, v, n, first, X, ldx, st)
+                        : lp_solve_v2<__bf16>(p, q, v, n, first, X, ldx, st);
 }

@@ -55,3 +55,37 @@ def test_refined_residual_report(cuda):
     res = P.solve_refined(B, X, 6, with_residual=True)
     assert res.shape == (3,) and np.all(res < 1e-9 * B.abs().max().item())
     P.close()
+
+
+@pytest.mark.parametrize("m,nlvl,rank,n,dtype", [
+    (512, 3, 32, 5, "f16"),      # whole matrix in the bottom kernel
+    (1024, 4, 20, 3, "f16"),     # one upper level, writes x directly
+    (8192, 7, 32, 70, "f16"),    # several column groups in both kernels
+    (8192, 6, 32, 16, "bf16"),   # leaf 128
+    (8192, 7, 64, 9, "f16"),     # rank 64: narrower bottom kernel
+    (4096, 6, 80, 4, "f16"),     # rank above 64: the per-level kernels
+])
+def test_refined_solve_shapes(cuda, m, nlvl, rank, n, dtype):
+    import torch
+    print(f"mixed shapes m={m} L={nlvl} r={rank} n={n} {dtype}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=43)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    assert P.factor_mixed(dtype) == 0
+    B = torch.tensor(np.random.default_rng(6).standard_normal((n, m)), device=cuda)
+    X64 = torch.zeros_like(B)
+    P.solve("N", B, X64)
+    ref = X64.cpu().numpy()
+    X = torch.zeros_like(B)
+    P.solve_refined(B, X, 1)
+    e1 = rel(X.cpu().numpy(), ref)
+    X = torch.zeros_like(B)
+    P.solve_refined(B, X, 10)
+    e10 = rel(X.cpu().numpy(), ref)
+    print(f"one solve {e1:.1e}, ten {e10:.1e}", flush=True)
+    assert e1 < 5e-2
+    assert e10 <= 1e-10
+    # every column on its own reaches the target too
+    for c in range(n):
+        assert rel(X[c].cpu().numpy(), ref[c]) <= 1e-10
+    P.close()
