@@ -155,6 +155,9 @@ int plr_mm_ok(const dplrh_plan *p);
 int plr_mm_apply(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
                  const double *X, int ldx, double *Y, int ldy, double *w, size_t wstride,
                  hipStream_t st);
+int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
+                   const double *X, int ldx, const double *C, int ldc, double *Y, int ldy,
+                   double *w, size_t wstride, double *bmax, hipStream_t st);
 int plr_mm_reduce1(dplrh_plan *p, int d, const double *P, const double *X, int ldx, int ncols,
                    double *T, size_t tstride, hipStream_t st);
 int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,

@@ -430,13 +430,14 @@ struct plr_lp
     float *T, *W, *wscale, *part, *r32, *z32;
     size_t part_cap;
     double *r, *scale, *pmax;
+    double *bmax; // maxima of |r| per (column, 128-row block) from the residual product
 };
 
 static void lp_free(plr_lp *q)
 {
     if (!q) return;
     void *b[] = {q->Vh, q->Uh, q->Dh, q->Sf, q->K64, q->K32, q->T, q->W, q->wscale, q->part,
-                 q->r32, q->z32, q->r, q->scale, q->pmax};
+                 q->r32, q->z32, q->r, q->scale, q->pmax, q->bmax};
     for (void *x : b)
         if (x) (void)hipFree(x);
     delete q;
@@ -758,23 +759,51 @@ static int lp_solve_t(const dplrh_plan *p, plr_lp *q, int n, hipStream_t st)
 }
 
 static int lp_solve_v2_any(const dplrh_plan *p, plr_lp *q, int n, int first, double *X, int ldx,
-                           hipStream_t st);
+                           const double *r, int ldr, const double *pm, int pm_n, hipStream_t st);
+static bool lp_v2_ok(const dplrh_plan *p, const plr_lp *q);
 
 // X := A^{-1} B by mixed-precision solves refined with FP64 residuals:
 // `iters` solves (iters-1 residuals).  When res != NULL, res[c] receives
 // max|b - A x| of column c after the last solve (one more FP64 residual).
+//
+// Second form (shapes lp_v2_shape covers, no residual report): the first
+// solve reads B in place; every later residual r = B - A x comes out of one
+// multi-RHS product that reads B for its beta term and also writes the
+// maxima of |r| per (column, 128-row block), so nothing is copied and no
+// separate maximum pass runs.
 static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int ldb, double *dX,
                          int ldx, int iters, double *res, hipStream_t st)
 {
     const int N = p->N;
     if (N == 0 || n == 0) return 0;
     int rc = 0;
-    const int chunk = n < 256 ? n : 256;
+    const int chunk = n < 128 ? n : 128;
     if ((rc = lp_ws(p, q, chunk))) return rc;
+    const bool direct = !res && lp_v2_ok(p, q) && ldb % 2 == 0 && ldx % 2 == 0 &&
+                        ((uintptr_t)dB & 15) == 0 && N % 256 == 0;
+    if (direct) {
+        if ((rc = plr_workspace(p, (chunk + 63) & ~63))) return rc;
+        if (!q->bmax) PLR_HIP(hipMalloc(&q->bmax, (size_t)128 * (N / 128) * sizeof(double)));
+    }
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int nc = n - c0 < chunk ? n - c0 : chunk;
         const double *B = dB + (size_t)c0 * ldb;
         double *X = dX + (size_t)c0 * ldx;
+        if (direct) {
+            hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, B, ldb, N,
+                               q->pmax);
+            PLR_HIP(hipGetLastError());
+            rc = lp_solve_v2_any(p, q, nc, 1, X, ldx, B, ldb, q->pmax, LP_SCALE_CHUNKS, st);
+            if (rc) return rc < 0 ? rc : -1;
+            for (int it = 1; it < iters; ++it) {
+                rc = plr_mm_apply_c(p, 0, nc, -1.0, 1.0, X, ldx, B, ldb, q->r, N, p->w,
+                                    p->w_per_col, q->bmax, st);
+                if (rc) return rc < 0 ? rc : -1;
+                rc = lp_solve_v2_any(p, q, nc, 0, X, ldx, q->r, N, q->bmax, N / 128, st);
+                if (rc) return rc < 0 ? rc : -1;
+            }
+            continue;
+        }
         for (int it = 0; it < iters + (res ? 1 : 0); ++it) {
             // r = b - A x (x = 0 on the first pass)
             PLR_HIP(hipMemcpy2DAsync(q->r, (size_t)N * sizeof(double), B, (size_t)ldb * sizeof(double),
@@ -783,7 +812,7 @@ static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int 
             hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, q->r, N, N,
                                q->pmax);
             if (it < iters) {
-                rc = lp_solve_v2_any(p, q, nc, it == 0, X, ldx, st);
+                rc = lp_solve_v2_any(p, q, nc, it == 0, X, ldx, q->r, N, q->pmax, LP_SCALE_CHUNKS, st);
                 if (rc < 0) return rc;
                 if (rc == 0) continue;
                 rc = 0;
@@ -833,7 +862,7 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
 //              z = r / max|r| per column, z|_leaf = Dinv z|_leaf, then every
 //              level d >= d0 (segments inside the span) with z in LDS, and
 //              finally T of level d0 - 1 (the span is one of its segments).
-//   k_lp_level one work-group per (256 rows, 64 right-hand sides) of level
+//   k_lp_level one work-group per (128 rows, 64 right-hand sides) of level
 //              d < d0: the rows of inv(K) for its segment (recomputed per
 //              work-group), z += Ut_d W_d, then the partial sums of
 //              V_{d-1}^T z over its rows (summed by k_lp_finish), or, at
@@ -843,6 +872,7 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
 static constexpr int LT_THREADS = 512; // eight waves
 static constexpr int LT_MAXT = 4;      // 16-row leaf tiles per wave (span <= 512)
 static constexpr int LV_RB = 256, LV_CW = 64, LV_THREADS = 512;
+static constexpr int LV_RT = LV_RB / 128; // 16-row tiles per wave
 
 struct LpTailTab
 {
@@ -948,11 +978,23 @@ __device__ __forceinline__ void lp_cap_tiles(const float *const *K32, const doub
         for (int v = 0; v < 4; ++v) w[p][v] = -s[p][v] * acc[p][v];
 }
 
-// LDS of k_lp_tail: z [CW][S+8], T [tsegs][RT][CW+4], W [tsegs][CW][RT+8],
+// Eight consecutive floats of LDS as two 16-byte reads (rows are padded to
+// 16 bytes, not 32).
+__device__ __forceinline__ f8 lds8(const float *p)
+{
+    const f4 lo = *(const f4 *)p, hi = *(const f4 *)(p + 4);
+    return f8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+}
+
+// LDS row strides (floats) are chosen for the access patterns: rows read
+// by 16 lanes at once with 16-byte accesses have an odd number of 16-byte
+// units (S + 4, R + 4); rows read one float per lane by four lane groups
+// are 16 mod 64 (CW + 16, 64 + 16), so no two lanes of a group share a bank.
+// LDS of k_lp_tail: z [CW][S+4], T [tsegs][RT][CW+16], W [tsegs][CW][RT+4],
 // smax [tsegs][CW] (floats).
 static size_t lpt_lds(int S, int CW, int RT, int tsegs)
 {
-    return ((size_t)CW * (S + 8) + (size_t)tsegs * RT * (CW + 4) + (size_t)tsegs * CW * (RT + 8) +
+    return ((size_t)CW * (S + 4) + (size_t)tsegs * RT * (CW + 16) + (size_t)tsegs * CW * (RT + 4) +
             (size_t)tsegs * CW) * sizeof(float);
 }
 
@@ -980,7 +1022,7 @@ __device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, co
         for (int q = 0; q < 8; ++q) {
             const int e = e0 + q, grp = e / nstep, rs = e - grp * nstep;
             const int kt = grp / NCT, ct = grp - kt * NCT;
-            const f8 zv = *(const f8 *)(z + (ct * 16 + m) * zs + rs * 32 + 8 * g);
+            const f8 zv = lds8(z + (ct * 16 + m) * zs + rs * 32 + 8 * g);
             acc = Lp<T16>::mfma(av[q], cvt8<T16>(zv, 1.f), acc);
             if ((rs + 1) % spq == 0 && spq <= 8) {
                 emit(rs / spq, kt, ct, acc);
@@ -992,10 +1034,11 @@ __device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, co
     }
 }
 
+// pmax: pm_n partial maxima of |r| per column (column stride pm_n).
 template <typename T16, int CW>
 __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const double *__restrict__ r,
                                                         int ldr, const double *__restrict__ pmax,
-                                                        double *__restrict__ scale,
+                                                        int pm_n, double *__restrict__ scale,
                                                         float *__restrict__ Z, int ldz,
                                                         float *__restrict__ Tout, size_t tstride,
                                                         int ncols)
@@ -1004,9 +1047,9 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
     constexpr int NCT = CW / 16;
     extern __shared__ float lds[];
     __shared__ double cinv[CW];
-    const int S = a.span, zs = S + 8, RT = a.RT, nsm = a.tsegs;
-    const int ts = CW + 4, tsz = RT * ts;   // T[s][k][c]
-    const int wsk = RT + 8, wsz = CW * wsk; // W[s][c][k]
+    const int S = a.span, zs = S + 4, RT = a.RT, nsm = a.tsegs;
+    const int ts = CW + 16, tsz = RT * ts;  // T[s][k][c]
+    const int wsk = RT + 4, wsz = CW * wsk; // W[s][c][k]
     float *z = lds;
     float *T = z + CW * zs;
     float *W = T + nsm * tsz;
@@ -1015,19 +1058,21 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
     const int row0 = blockIdx.x * S, c0 = blockIdx.y * CW, N = a.N, b = a.b;
     const int nc = min(CW, ncols - c0);
 
-    // 1. column scales (max |r| from the 64 chunk maxima, eight threads per
+    // 1. column scales (max |r| from the partial maxima, eight threads per
     // column), then z = r / max|r|
     {
         double *cp = (double *)T; // [CW][8], T is free until the first level
         const int c = tid >> 3, part = tid & 7;
         if (c < CW) {
-            const double *pm = pmax + (size_t)(c0 + min(c, nc - 1)) * LP_SCALE_CHUNKS + part * 8;
-            double pv[8];
-#pragma unroll
-            for (int u = 0; u < 8; ++u) pv[u] = pm[u];
+            const double *pm = pmax + (size_t)(c0 + min(c, nc - 1)) * pm_n;
             double mv = 0.0;
+            for (int j0 = part * 8; j0 < pm_n; j0 += 64) {
+                double pv[8];
 #pragma unroll
-            for (int u = 0; u < 8; ++u) mv = fmax(mv, pv[u]);
+                for (int u = 0; u < 8; ++u) pv[u] = pm[min(j0 + u, pm_n - 1)];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) mv = fmax(mv, pv[u]);
+            }
             cp[c * 8 + part] = mv;
         }
         __syncthreads();
@@ -1091,8 +1136,8 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                 const float *zb = z + m * zs + lf[q] * b + 8 * g + j0;
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct) {
-                    const f8 x0 = *(const f8 *)(zb + ct * 16 * zs);
-                    const f8 x1 = *(const f8 *)(zb + ct * 16 * zs + 32);
+                    const f8 x0 = lds8(zb + ct * 16 * zs);
+                    const f8 x1 = lds8(zb + ct * 16 * zs + 32);
                     acc[q][ct] = Lp<T16>::mfma(a0[q], cvt8<T16>(x0, 1.f), acc[q][ct]);
                     acc[q][ct] = Lp<T16>::mfma(a1[q], cvt8<T16>(x1, 1.f), acc[q][ct]);
                 }
@@ -1180,7 +1225,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                     const int e = e0 + q, ks = e % nks, grp = e / nks, it = grp / NCT, ct = grp - it * NCT;
                     const int s = (it * 16) / seg;
                     const unsigned *sm = smax + s * CW + ct * 16;
-                    const f8 w8 = *(const f8 *)(W + s * wsz + (ct * 16 + m) * wsk + ks * 32 + 8 * g);
+                    const f8 w8 = lds8(W + s * wsz + (ct * 16 + m) * wsk + ks * 32 + 8 * g);
                     acc = Lp<T16>::mfma(cvt8<T16>(w8, 1.f / lp_pow2(sm[m])), bq[q], acc);
                     if (ks == nks - 1) {
 #pragma unroll
@@ -1233,18 +1278,18 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
     }
 }
 
-// LDS of k_lp_level: rhs [2R][CW+4], W [CW][R+8] (floats), zt [CW][RB+8] (16-bit)
+// LDS of k_lp_level: rhs [2R][CW+16], W [CW][R+4] (floats), zt [CW][RB+8] (16-bit)
 static size_t lpl_lds(int R)
 {
-    return ((size_t)2 * R * (LV_CW + 4) + (size_t)LV_CW * (R + 8)) * sizeof(float) +
+    return ((size_t)2 * R * (LV_CW + 16) + (size_t)LV_CW * (R + 4)) * sizeof(float) +
            (size_t)LV_CW * (LV_RB + 8) * 2;
 }
 
 // Every global load of the work-group (old z, Ut, V of level d-1, T, K, sig)
-// is issued at the start, so the kernel waits on memory about once.
-static constexpr int LV_MAXR = 64;
-
-template <typename T16>
+// is issued at the start, so the work-group waits on memory about once.  Eight
+// waves of LV_RT 16-row tiles each; RM bounds the ranks of levels d and d-1
+// (32 or 64), which sizes the register buffers.
+template <typename T16, int RM>
 __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float *__restrict__ Tin,
                                                          size_t tstride, float *__restrict__ Z,
                                                          int ldz, float *__restrict__ part,
@@ -1253,50 +1298,49 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
                                                          int ncols)
 {
     typedef typename Lp<T16>::v8 v8;
-    constexpr int KS = LV_MAXR / 32;   // expansion k-steps (R <= 64)
-    constexpr int NV = LV_MAXR / 32;   // reduction tasks per wave (Rn <= 64)
+    constexpr int KS = RM / 32;    // expansion k-steps
+    constexpr int NP = RM / 32;    // capacitance and reduction tasks per wave
+    constexpr int NU = LV_RB / 32; // reduction k-steps (rows)
     extern __shared__ float lds[];
     __shared__ unsigned smax[LV_CW];
-    const int R = a.R, n2 = 2 * R, rs = LV_CW + 4, wsk = R + 8, zts = LV_RB + 8;
+    const int R = a.R, n2 = 2 * R, rs = LV_CW + 16, wsk = R + 4, zts = LV_RB + 8;
     float *rhs = lds;                   // [n2][rs]
     float *W = rhs + n2 * rs;           // [CW][wsk]
     T16 *zt = (T16 *)(W + LV_CW * wsk); // [CW][zts]
     const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wv = tid >> 6;
     const int r0 = blockIdx.x * LV_RB, c0 = blockIdx.y * LV_CW, N = a.N;
+    const int ri = r0 + wv * 16 * LV_RT; // this wave's rows
     const int nc = min(LV_CW, ncols - c0);
     const int s = r0 / a.seg, istop = !(s & 1), top = s & ~1, bot = top + 1;
     const int nks = R / 32, nkt = R / 16, nktn = a.Rn / 16;
 
-    // old z (wave wv owns rows r0 + 32 wv .. + 31, columns as the expansion's D)
-    float zo[2][4][4];
+    // old z (columns as the expansion's D: c = 16ct + 4g + v, row ri + 16rt + m)
+    float zo[LV_RT][4][4];
 #pragma unroll
-    for (int rt = 0; rt < 2; ++rt)
+    for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = ct * 16 + 4 * g + v;
-                zo[rt][ct][v] = Z[(size_t)(c0 + min(c, nc - 1)) * ldz + r0 + wv * 32 + rt * 16 + m];
-            }
+            for (int v = 0; v < 4; ++v)
+                zo[rt][ct][v] = Z[(size_t)(c0 + min(ct * 16 + 4 * g + v, nc - 1)) * ldz + ri + 16 * rt + m];
     // Ut rows of the expansion
     const T16 *U = (const T16 *)a.Ut;
-    v8 bq[2][KS];
+    v8 bq[LV_RT][KS];
     if (R > 0) // a level of rank 0 has no Ut rows
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
             for (int ks = 0; ks < KS; ++ks)
-                bq[rt][ks] = *(const v8 *)(U + (size_t)(r0 + wv * 32 + rt * 16 + m) * R +
-                                           min(ks, nks - 1) * 32 + 8 * g);
+                bq[rt][ks] = *(const v8 *)(U + (size_t)(ri + 16 * rt + m) * R + min(ks, nks - 1) * 32 + 8 * g);
     // V of level d-1 for this wave's reduction tasks (kt, ct) = wv + 8p
-    v8 av[NV][8];
+    v8 av[NP][NU];
     if (a.Rn > 0) {
         const T16 *Vn = (const T16 *)a.Vn;
 #pragma unroll
-        for (int p = 0; p < NV; ++p) {
+        for (int p = 0; p < NP; ++p) {
             const int task = min(wv + 8 * p, nktn * 4 - 1), kt = task >> 2;
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int u = 0; u < NU; ++u)
                 av[p][u] = *(const v8 *)(Vn + (size_t)(kt * 16 + m) * N + r0 + 32 * u + 8 * g);
         }
     }
@@ -1323,15 +1367,15 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
     if (tid < LV_CW) smax[tid] = 0u;
     __syncthreads();
     // 2. W_s = -S_s .* (rows of inv(K) for segment s) rhs, largest |W| per
-    // column: tasks (kt, ct), wave wv takes wv and wv + 8
+    // column: tasks (kt, ct), wave wv takes wv + 8p
     if (R > 0) {
         const size_t ko = (size_t)(s >> 1) * n2 * n2;
         const int ntask = nkt * 4;
-        const float *k32[2], *lo[2], *hi[2], *sv[2];
-        const double *k64[2];
-        int i0[2], ct[2], kk[2];
+        const float *k32[NP], *lo[NP], *hi[NP], *sv[NP];
+        const double *k64[NP];
+        int i0[NP], ct[NP], kk[NP];
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < NP; ++p) {
             const int task = min(wv + 8 * p, ntask - 1);
             ct[p] = task & 3;
             kk[p] = (task >> 2) * 16;
@@ -1343,13 +1387,10 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
             kk[p] += 4 * g;
             sv[p] = a.S + (size_t)s * R + kk[p];
         }
-        f4 w[2];
-        if (ntask > 8)
-            lp_cap_tiles<2>(k32, k64, n2, R, i0, lo, hi, rs, sv, m, g, w);
-        else
-            lp_cap_tiles<1>(k32, k64, n2, R, i0, lo, hi, rs, sv, m, g, w);
+        f4 w[NP];
+        lp_cap_tiles<NP>(k32, k64, n2, R, i0, lo, hi, rs, sv, m, g, w);
 #pragma unroll
-        for (int p = 0; p < 2; ++p) {
+        for (int p = 0; p < NP; ++p) {
             if (wv + 8 * p >= ntask) continue;
             *(f4 *)(W + (ct[p] * 16 + m) * wsk + kk[p]) = w[p];
             const float mx = fmaxf(fmaxf(fabsf(w[p][0]), fabsf(w[p][1])), fmaxf(fabsf(w[p][2]), fabsf(w[p][3])));
@@ -1357,7 +1398,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
         }
     }
     __syncthreads();
-    // 3. z += Ut W (M = columns, N = rows): two 16-row tiles per wave
+    // 3. z += Ut W (M = columns, N = rows): LV_RT 16-row tiles per wave
     {
         float isc[4], sc[4][4];
 #pragma unroll
@@ -1366,61 +1407,63 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
 #pragma unroll
             for (int v = 0; v < 4; ++v) sc[ct][v] = lp_pow2(smax[ct * 16 + 4 * g + v]);
         }
-        f4 acc[2][4];
+        f4 acc[LV_RT][4];
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int ks = 0; ks < KS; ++ks) {
-            if (ks >= nks) break;
+            if (ks < nks) {
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                const v8 aw = cvt8<T16>(*(const f8 *)(W + (ct * 16 + m) * wsk + ks * 32 + 8 * g), isc[ct]);
+                for (int ct = 0; ct < 4; ++ct) {
+                    const v8 aw = cvt8<T16>(lds8(W + (ct * 16 + m) * wsk + ks * 32 + 8 * g), isc[ct]);
 #pragma unroll
-                for (int rt = 0; rt < 2; ++rt) acc[rt][ct] = Lp<T16>::mfma(aw, bq[rt][ks], acc[rt][ct]);
+                    for (int rt = 0; rt < LV_RT; ++rt) acc[rt][ct] = Lp<T16>::mfma(aw, bq[rt][ks], acc[rt][ct]);
+                }
             }
         }
-        // lane holds dz[c = ct*16 + 4g + v][i = 32 wv + 16 rt + m]
+        // lane holds dz[c = ct*16 + 4g + v][i = 16 (LV_RT wv + rt) + m]
 #pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
+        for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
+        for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) {
-                    const int c = ct * 16 + 4 * g + v, i = wv * 32 + rt * 16 + m;
-                    const float zn = zo[rt][ct][v] * (c < nc ? 1.f : 0.f) + acc[rt][ct][v] * sc[ct][v];
-                    if (X) {
-                        if (c < nc) {
-                            double *xp = X + (size_t)(c0 + c) * ldx + r0 + i;
-                            const double sr = scale[c0 + c] > 0.0 ? scale[c0 + c] : 1.0;
-                            *xp = (first ? 0.0 : *xp) + sr * (double)zn;
-                        }
-                    } else {
-                        if (c < nc) Z[(size_t)(c0 + c) * ldz + r0 + i] = zn;
-                        zt[c * zts + i] = (T16)zn;
+            for (int v = 0; v < 4; ++v) {
+                const int c = ct * 16 + 4 * g + v, i = (wv * LV_RT + rt) * 16 + m;
+                const float zn = zo[rt][ct][v] * (c < nc ? 1.f : 0.f) + acc[rt][ct][v] * sc[ct][v];
+                if (X) {
+                    if (c < nc) {
+                        double *xp = X + (size_t)(c0 + c) * ldx + r0 + i;
+                        const double sr = scale[c0 + c] > 0.0 ? scale[c0 + c] : 1.0;
+                        *xp = (first ? 0.0 : *xp) + sr * (double)zn;
                     }
+                } else {
+                    if (c < nc) Z[(size_t)(c0 + c) * ldz + r0 + i] = zn;
+                    zt[c * zts + i] = (T16)zn;
                 }
+            }
     }
     if (a.Rn == 0) return;
     __syncthreads();
-    // 4. partial V_{d-1}^T z over the 256 rows, in k_lp_finish's layout
-    // (tiles of 256 rows of the segment of level d-1)
+    // 4. partial V_{d-1}^T z over the work-group's rows, in k_lp_finish's
+    // layout (tiles of LV_RB rows of the segment of level d-1)
     {
         const int sn = r0 / a.segn, tile = (r0 - sn * a.segn) / LV_RB, st = sn * a.tps + tile;
 #pragma unroll
-        for (int p = 0; p < NV; ++p) {
+        for (int p = 0; p < NP; ++p) {
             const int task = wv + 8 * p;
-            if (task >= nktn * 4) break;
-            const int ct = task & 3, kt = task >> 2;
-            const T16 *pz = zt + (ct * 16 + m) * zts + 8 * g;
-            f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+            if (task < nktn * 4) {
+                const int ct = task & 3, kt = task >> 2;
+                const T16 *pz = zt + (ct * 16 + m) * zts + 8 * g;
+                f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-            for (int u = 0; u < 8; ++u) acc = Lp<T16>::mfma(av[p][u], *(const v8 *)(pz + 32 * u), acc);
-            const int k = kt * 16 + 4 * g, kg = k / LP_KG, kl = k - kg * LP_KG;
-            float *pp = part + ((size_t)(st * a.nkg + kg) * a.ncg + blockIdx.y) * LP_BLK +
-                        (ct * 16 + m) * LP_KG + kl;
-            *(f4 *)pp = acc;
+                for (int u = 0; u < NU; ++u) acc = Lp<T16>::mfma(av[p][u], *(const v8 *)(pz + 32 * u), acc);
+                const int k = kt * 16 + 4 * g, kg = k / LP_KG, kl = k - kg * LP_KG;
+                float *pp = part + ((size_t)(st * a.nkg + kg) * a.ncg + blockIdx.y) * LP_BLK +
+                            (ct * 16 + m) * LP_KG + kl;
+                *(f4 *)pp = acc;
+            }
         }
     }
 }
@@ -1444,7 +1487,7 @@ static lp_v2 lp_v2_shape(const dplrh_plan *p, const plr_lp *q)
         int d0 = 0;
         while ((N >> d0) > S) ++d0;
         if ((N >> d0) != S || d0 > L) continue;
-        if (S < N && S < LV_RB) continue; // the upper-level pass takes 256-row blocks
+        if (S < N && S < 2 * LV_RB) continue; // the upper-level pass: whole row blocks per segment
         int RT = 32;
         for (int d = d0 > 0 ? d0 - 1 : 0; d < L; ++d) RT = q->Rp[d] > RT ? q->Rp[d] : RT;
         const int tsegs = S / b > 2 ? S / b : 2;
@@ -1462,6 +1505,8 @@ static lp_v2 lp_v2_shape(const dplrh_plan *p, const plr_lp *q)
     return v;
 }
 
+static bool lp_v2_ok(const dplrh_plan *p, const plr_lp *q) { return lp_v2_shape(p, q).ok != 0; }
+
 // partial blocks the upper levels need (k_lp_level writes, k_lp_finish sums)
 static size_t lp_v2_part(const dplrh_plan *p, const plr_lp *q, const lp_v2 &v, int n)
 {
@@ -1476,7 +1521,7 @@ static size_t lp_v2_part(const dplrh_plan *p, const plr_lp *q, const lp_v2 &v, i
 
 template <typename T16, int CW>
 static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_lp *q, int n,
-                          hipStream_t st)
+                          const double *r, int ldr, const double *pm, int pm_n, hipStream_t st)
 {
     static bool attr = false;
     if (!attr) {
@@ -1485,17 +1530,19 @@ static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         attr = true;
     }
-    hipLaunchKernelGGL((k_lp_tail<T16, CW>), grid, dim3(LT_THREADS), lds, st, tt, q->r, tt.N,
-                       q->pmax, q->scale, q->z32, tt.N, q->T, q->t_per_col, n);
+    hipLaunchKernelGGL((k_lp_tail<T16, CW>), grid, dim3(LT_THREADS), lds, st, tt, r, ldr, pm, pm_n,
+                       q->scale, q->z32, tt.N, q->T, q->t_per_col, n);
     PLR_HIP(hipGetLastError());
     return 0;
 }
 
-// One low-precision solve of the scaled residual q->r (column maxima in
-// q->pmax) added into X: X = (first ? 0 : X) + max|r| * solve_lp(r / max|r|).
+// One low-precision solve of the residual r (ldr; pm_n partial maxima of
+// |r| per column in pm) added into X:
+// X = (first ? 0 : X) + max|r| * solve_lp(r / max|r|).
 template <typename T16>
 static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, int first,
-                       double *X, int ldx, hipStream_t st)
+                       double *X, int ldx, const double *r, int ldr, const double *pm, int pm_n,
+                       hipStream_t st)
 {
     const int N = p->N, L = p->L;
     const T16 *Vh = (const T16 *)q->Vh, *Uh = (const T16 *)q->Uh;
@@ -1521,8 +1568,8 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
     tt.D = q->Dh;
     const size_t lds = lpt_lds(v.S, v.cw, v.RT, v.tsegs);
     const dim3 grid(N / v.S, (n + v.cw - 1) / v.cw);
-    rc = v.cw == 64 ? lp_launch_tail<T16, 64>(tt, lds, grid, q, n, st)
-                    : lp_launch_tail<T16, 32>(tt, lds, grid, q, n, st);
+    rc = v.cw == 64 ? lp_launch_tail<T16, 64>(tt, lds, grid, q, n, r, ldr, pm, pm_n, st)
+                    : lp_launch_tail<T16, 32>(tt, lds, grid, q, n, r, ldr, pm, pm_n, st);
     if (rc) return rc;
     if (v.d0 == 0) {
         const size_t ne = (size_t)N * n;
@@ -1533,7 +1580,9 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
     }
     static bool attr = false;
     if (!attr) {
-        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16>,
+        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 32>,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 64>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         attr = true;
     }
@@ -1554,9 +1603,14 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
         lt.S = q->Sf + q->soff[d];
         lt.K32 = d >= 2 ? q->K32 + q->koff[d] : nullptr;
         lt.K64 = d < 2 ? q->K64 + q->koff[d] : nullptr;
-        hipLaunchKernelGGL(k_lp_level<T16>, dim3(N / LV_RB, ncg), dim3(LV_THREADS), lpl_lds(lt.R),
-                           st, lt, q->T, q->t_per_col, q->z32, N, q->part, q->scale,
-                           d == 0 ? X : nullptr, ldx, first, n);
+        if (lt.R <= 32 && lt.Rn <= 32)
+            hipLaunchKernelGGL((k_lp_level<T16, 32>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
+                               lpl_lds(lt.R), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
+                               q->scale, d == 0 ? X : nullptr, ldx, first, n);
+        else
+            hipLaunchKernelGGL((k_lp_level<T16, 64>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
+                               lpl_lds(lt.R), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
+                               q->scale, d == 0 ? X : nullptr, ldx, first, n);
         PLR_HIP(hipGetLastError());
         if (d > 0 && lt.Rn > 0) {
             LpTab t;
@@ -1570,10 +1624,10 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
 }
 
 static int lp_solve_v2_any(const dplrh_plan *p, plr_lp *q, int n, int first, double *X, int ldx,
-                           hipStream_t st)
+                           const double *r, int ldr, const double *pm, int pm_n, hipStream_t st)
 {
     const lp_v2 v = lp_v2_shape(p, q);
     if (!v.ok) return 1; // shape not covered: the caller takes the first form
-    return q->type == 1 ? lp_solve_v2<_Float16>(p, q, v, n, first, X, ldx, st)
-                        : lp_solve_v2<__bf16>(p, q, v, n, first, X, ldx, st);
+    return q->type == 1 ? lp_solve_v2<_Float16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, st)
+                        : lp_solve_v2<__bf16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, st);
 }

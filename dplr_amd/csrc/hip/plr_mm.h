@@ -197,6 +197,9 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
 // summed by mm_finish_body.
 static constexpr int MMF_RB = 256, MMF_NW = 4, MMF_TPW = 3;
 
+// WT: coefficients rank-major, w[(w_off[d] + s*R + k) * wstride + c] (wstride
+// = the padded column count), and partial blocks [k_local][c_local].
+template <bool WT = false>
 __device__ __forceinline__ void mm_reduce_fused_body(const MmTab &a, int rbk, int cg,
                                                      const double *__restrict__ X, int ldx,
                                                      double alpha, int trans, int raw,
@@ -288,11 +291,20 @@ __device__ __forceinline__ void mm_reduce_fused_body(const MmTab &a, int rbk, in
                 for (int ct = 0; ct < 4; ++ct) {
                     const int c = c0 + ct * 16 + m;
                     if (c >= a.ncols) continue;
-                    double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+                    if (WT) {
+                        double *wc = w + ((size_t)a.w_off[d] + (size_t)sp * R) * wstride + c;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) {
-                        const int k = tk[j] + g + 4 * v;
-                        if (k < R) wc[k] = f * acc[j][ct][v];
+                        for (int v = 0; v < 4; ++v) {
+                            const int k = tk[j] + g + 4 * v;
+                            if (k < R) wc[(size_t)k * wstride] = f * acc[j][ct][v];
+                        }
+                    } else {
+                        double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
+#pragma unroll
+                        for (int v = 0; v < 4; ++v) {
+                            const int k = tk[j] + g + 4 * v;
+                            if (k < R) wc[k] = f * acc[j][ct][v];
+                        }
                     }
                 }
             } else {
@@ -302,8 +314,10 @@ __device__ __forceinline__ void mm_reduce_fused_body(const MmTab &a, int rbk, in
 #pragma unroll
                 for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-                    for (int v = 0; v < 4; ++v)
-                        pp[(ct * 16 + m) * MM_KG + (tk[j] % MM_KG) + g + 4 * v] = acc[j][ct][v];
+                    for (int v = 0; v < 4; ++v) {
+                        const int kl = (tk[j] % MM_KG) + g + 4 * v, cl = ct * 16 + m;
+                        pp[WT ? kl * MM_CG + cl : cl * MM_KG + kl] = acc[j][ct][v];
+                    }
             }
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) acc[j][ct] = d4{0.0, 0.0, 0.0, 0.0};
@@ -346,6 +360,7 @@ __device__ __forceinline__ void mm_reduce_fused_body(const MmTab &a, int rbk, in
 // tile with eight loads in flight, then combine through LDS.
 static constexpr int MM_FCH = MM_BLK / 64; // chunks per block
 
+template <bool WT = false>
 __device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double alpha, int trans,
                                                int raw, const double *__restrict__ part,
                                                double *__restrict__ w, size_t wstride)
@@ -374,12 +389,14 @@ __device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double al
     __syncthreads();
     if (ph != 0) return;
     const double t = (red[0][el] + red[1][el]) + (red[2][el] + red[3][el]);
-    const int kl = e % MM_KG, cl = e / MM_KG;
+    const int kl = WT ? e / MM_CG : e % MM_KG, cl = WT ? e % MM_CG : e / MM_KG;
     const int k = kg * MM_KG + kl, c = cg * MM_CG + cl;
     if (k >= R || c >= a.ncols) return;
     const int sp = raw ? s : (s ^ 1);
-    double *wc = w + (size_t)c * wstride + a.w_off[d] + (size_t)sp * R;
-    wc[k] = raw ? t : alpha * t;
+    if (WT)
+        w[((size_t)a.w_off[d] + (size_t)sp * R + k) * wstride + c] = raw ? t : alpha * t;
+    else
+        w[(size_t)c * wstride + a.w_off[d] + (size_t)sp * R + k] = raw ? t : alpha * t;
 }
 
 
@@ -506,6 +523,191 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
 }
 
 
+
+// Y^T (64 columns x 32 rows per wavefront) = beta Y^T + alpha X_leaf^T op(D)^T
+// + sum_d W_d^T (sig .* Q_d^T), with W rank-major (as written by the WT
+// reduction): w[(w_off[d] + s*R + k) * wtc + c].  Four waves per work-group
+// cover 128 rows; X of the leaves is staged in LDS (64 leaf columns per
+// stage, the only barriers), then every wave runs on its own: W^T comes
+// straight from global memory (16 consecutive columns per lane group), the
+// ranks of all levels in chunks of 16, the next chunk's loads in flight
+// while the current one is multiplied (two register buffers, no copies).
+static constexpr int ME_RB = 128;
+
+// The beta term reads C (which may be Y itself); with bmax != null the
+// largest |Y| of every (column, 128-row block) goes to bmax[c * (N/128) + bx].
+__device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
+                                                const double *__restrict__ D,
+                                                const double *__restrict__ X, int ldx,
+                                                double alpha, double beta, int trans,
+                                                const double *__restrict__ w, int wtc,
+                                                const double *C, int ldc, double *Y, int ldy,
+                                                double *__restrict__ bmax, double *xs)
+{
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const int wv = threadIdx.x >> 6;
+    const int rbk = bx * ME_RB, r0 = rbk + wv * 32;
+    const int c0 = by * MM_CG;
+    const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
+    d4 acc[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+
+    if (D) {
+        // leaves of the work-group: one (b >= 128) or two (b = 64)
+        const int lf0 = rbk / b, nl = b >= ME_RB ? 1 : ME_RB / b;
+        const int li = r0 / b - lf0, il = r0 - (r0 / b) * b;
+        const size_t ds_i = trans ? b : 1, ds_j = trans ? 1 : b; // element (row, col) of op(D)
+        const double *Dl = D + (size_t)(r0 / b) * b * b;
+        for (int j0 = 0; j0 < b; j0 += 64) {
+            double bq[16][2];
+#pragma unroll
+            for (int u = 0; u < 16; ++u)
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+                    bq[u][rt] = Dl[(size_t)(il + 16 * rt + m) * ds_i + (size_t)(j0 + 4 * u + g) * ds_j];
+            __syncthreads(); // the previous stage is consumed
+            // xs[l][c][jj] = X[c0 + c][(lf0 + l) * b + j0 + jj]
+            for (int e0 = 0; e0 < nl * 64 * 64; e0 += 8 * 256) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = min(e0 + u * 256 + (int)threadIdx.x, nl * 64 * 64 - 1);
+                    const int l = e >> 12, c = (e >> 6) & 63, jj = e & 63;
+                    v[u] = X[(size_t)(c0 + min(c, nc - 1)) * ldx + (size_t)(lf0 + l) * b + j0 + jj] *
+                           (double)(c < nc);
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const int e = e0 + u * 256 + (int)threadIdx.x;
+                    if (e < nl * 64 * 64) xs[((e >> 12) * 64 + ((e >> 6) & 63)) * 65 + (e & 63)] = v[u];
+                }
+            }
+            __syncthreads();
+            const double *xl = xs + li * 64 * 65;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                double av[4];
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) av[ct] = xl[(16 * ct + m) * 65 + 4 * u + g];
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = mfma64(av[ct], bq[u][rt], acc[rt][ct]);
+            }
+        }
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] *= alpha;
+    }
+
+    // levels: chunks of 16 ranks over all levels
+    int nch = 0;
+    for (int d = 0; d < a.L; ++d) nch += (a.R[d] + 15) / 16;
+    if (nch > 0) {
+        struct Buf
+        {
+            double wa[4][4]; // W^T[c0 + 16ct + m][k0 + 4u + g]
+            double qb[4][2]; // Q[k0 + 4u + g][r0 + 16rt + m]
+            double sg[4];    // sig of rank k0 + 4u + g
+            int kn;          // real ranks in the chunk (0 past the last chunk)
+        };
+        // every load unconditional (clamped indices), masks computed, not loaded
+        auto load = [&](Buf &B, int c) {
+            const int cc = min(c, nch - 1);
+            int d = 0, cum = 0;
+            while (cc >= cum + (a.R[d] + 15) / 16) cum += (a.R[d++] + 15) / 16;
+            const int R = a.R[d], k0 = (cc - cum) * 16, s = r0 / a.seglen[d];
+            const double *Q = a.Q[d] + r0 + m;
+            const double *sg = a.S[d] + (size_t)(trans ? (s ^ 1) : s) * R;
+            const double *wr = w + ((size_t)a.w_off[d] + (size_t)s * R) * wtc + c0 + m;
+            B.kn = c < nch ? R - k0 : 0;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int k = k0 + 4 * u + g, kc = min(k, R - 1);
+                B.sg[u] = sg[kc];
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) B.wa[u][ct] = wr[(size_t)kc * wtc + 16 * ct];
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) B.qb[u][rt] = Q[(size_t)kc * N + 16 * rt];
+            }
+        };
+        auto compute = [&](const Buf &B) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double f = B.sg[u] * (double)(4 * u + g < B.kn);
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    const double bv = B.qb[u][rt] * f;
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = mfma64(B.wa[u][ct], bv, acc[rt][ct]);
+                }
+            }
+        };
+        Buf bA, bB;
+        load(bA, 0);
+#pragma unroll 1
+        for (int c = 0; c < nch; c += 2) {
+            load(bB, c + 1);
+            compute(bA);
+            load(bA, c + 2);
+            compute(bB);
+        }
+    }
+
+    // lane holds Y^T[c0 + 16ct + g + 4v][r0 + 16rt + m]
+    double yo[2][4][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
+                    yo[rt][ct][v] = C[(size_t)c * ldc + r0 + 16 * rt + m];
+                }
+    }
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = c0 + ct * 16 + g + 4 * v;
+                acc[rt][ct][v] += beta == 0.0 ? 0.0 : beta * yo[rt][ct][v];
+                if (c < a.ncols) Y[(size_t)c * ldy + r0 + 16 * rt + m] = acc[rt][ct][v];
+            }
+    if (bmax) {
+        // column maxima: over the two row tiles, the 16 lanes of a group
+        // (rows), then the four waves
+        double cm[4][4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double x = fmax(fabs(acc[0][ct][v]), fabs(acc[1][ct][v]));
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) x = fmax(x, __shfl_xor(x, o, 64));
+                cm[ct][v] = x;
+            }
+        __syncthreads(); // xs is free again
+        if (m == 0)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) xs[wv * 64 + ct * 16 + g + 4 * v] = cm[ct][v];
+        __syncthreads();
+        if (threadIdx.x < MM_CG && c0 + (int)threadIdx.x < a.ncols) {
+            const int c = threadIdx.x;
+            const double x = fmax(fmax(xs[c], xs[64 + c]), fmax(xs[128 + c], xs[192 + c]));
+            bmax[(size_t)(c0 + c) * (N / ME_RB) + bx] = x;
+        }
+    }
+}
 
 // Table for levels [d0, d1) with operands P (reduced), Q (expanded) and S.
 static inline void mm_tab(const dplrh_plan *p, int d0, int d1, const double *const *P,

@@ -29,19 +29,33 @@ __global__ __launch_bounds__(64 * NW) void k_mm_reduce(MmTab a, const double *__
     mm_reduce_body<NW>(a, blockIdx.x, X, ldx, alpha, trans, raw, part, w, wstride);
 }
 
+template <bool WT>
 __global__ __launch_bounds__(256) void k_mm_reduce_fused(MmTab a, const double *__restrict__ X,
                                                          int ldx, double alpha, int trans, int raw,
                                                          double *__restrict__ part,
                                                          double *__restrict__ w, size_t wstride)
 {
-    mm_reduce_fused_body(a, blockIdx.x, blockIdx.y, X, ldx, alpha, trans, raw, part, w, wstride);
+    mm_reduce_fused_body<WT>(a, blockIdx.x, blockIdx.y, X, ldx, alpha, trans, raw, part, w, wstride);
 }
 
+template <bool WT>
 __global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int trans, int raw,
                                                    const double *__restrict__ part,
                                                    double *__restrict__ w, size_t wstride)
 {
-    mm_finish_body(a, blockIdx.x, alpha, trans, raw, part, w, wstride);
+    mm_finish_body<WT>(a, blockIdx.x, alpha, trans, raw, part, w, wstride);
+}
+
+__global__ __launch_bounds__(256, 2) void k_mm_expand2(MmTab a, const double *__restrict__ D,
+                                                    const double *__restrict__ X, int ldx,
+                                                    double alpha, double beta, int trans,
+                                                    const double *__restrict__ w, int wtc,
+                                                    const double *C, int ldc, double *Y, int ldy,
+                                                    double *__restrict__ bmax)
+{
+    extern __shared__ double xs_dyn[];
+    mm_expand2_body(a, blockIdx.x, blockIdx.y, D, X, ldx, alpha, beta, trans, w, wtc, C, ldc, Y,
+                    ldy, bmax, xs_dyn);
 }
 
 __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__restrict__ D,
@@ -82,6 +96,18 @@ int plr_mm_apply(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
                  const double *X, int ldx, double *Y, int ldy, double *w, size_t wstride,
                  hipStream_t st)
 {
+    return plr_mm_apply_c(p, trans, ncols, alpha, beta, X, ldx, nullptr, 0, Y, ldy, w, wstride,
+                          nullptr, st);
+}
+
+// As plr_mm_apply with the beta term read from C (ldc) instead of Y, and,
+// on the fused path, the per-(column, 128-row block) maxima of |Y| in bmax.
+// Returns 1, launching nothing, when C or bmax is given and the shape does
+// not take the fused path.
+int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
+                   const double *X, int ldx, const double *C, int ldc, double *Y, int ldy,
+                   double *w, size_t wstride, double *bmax, hipStream_t st)
+{
     const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
     for (int d = 0; d < p->L; ++d) {
         const double *Ud = p->U + p->uoff[d], *Vd = p->V + p->uoff[d];
@@ -93,18 +119,44 @@ int plr_mm_apply(dplrh_plan *p, int trans, int ncols, double alpha, double beta,
     mm_tab(p, 0, p->L, P, Q, S, ncols, 0, &t);
     int rc = mm_part(p, (size_t)t.part_off[t.L]);
     if (rc) return rc;
+    // the fused path keeps the coefficients rank-major (w holds soff[L] rows
+    // of t.ncg * 64 columns: the caller sizes it for whole 64-column groups)
+    const bool fused = p->N % MMF_RB == 0 && ldx % 2 == 0 && !p->force_valu && w == p->w &&
+                       t.ncg * MM_CG <= p->wk_cols;
+    if (!fused && (C || bmax)) return 1;
+    if (fused) {
+        const int wtc = t.ncg * MM_CG;
+        if (t.task_off[t.L] > 0) {
+            hipLaunchKernelGGL(k_mm_reduce_fused<true>, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF_NW),
+                               0, st, t, X, ldx, alpha, trans, 0, p->tmp, w, (size_t)wtc);
+            PLR_HIP(hipGetLastError());
+        }
+        if (t.fin_off[t.L] > 0) {
+            hipLaunchKernelGGL(k_mm_finish<true>, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t,
+                               alpha, trans, 0, p->tmp, w, (size_t)wtc);
+            PLR_HIP(hipGetLastError());
+        }
+        const int nl = p->b >= ME_RB ? 1 : ME_RB / p->b;
+        const size_t lds = (size_t)nl * 64 * 65 * sizeof(double);
+        static bool attr = false;
+        if (!attr) {
+            PLR_HIP(hipFuncSetAttribute((const void *)k_mm_expand2,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+            attr = true;
+        }
+        hipLaunchKernelGGL(k_mm_expand2, dim3(p->N / ME_RB, t.ncg), dim3(256), lds, st, t, p->D, X,
+                           ldx, alpha, beta, trans, w, wtc, C ? C : Y, C ? ldc : ldy, Y, ldy, bmax);
+        PLR_HIP(hipGetLastError());
+        return 0;
+    }
     if (t.task_off[t.L] > 0) {
-        if (p->N % MMF_RB == 0 && ldx % 2 == 0 && !p->force_valu)
-            hipLaunchKernelGGL(k_mm_reduce_fused, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF_NW), 0, st, t, X,
-                               ldx, alpha, trans, 0, p->tmp, w, wstride);
-        else
-            hipLaunchKernelGGL(k_mm_reduce<1>, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx,
-                               alpha, trans, 0, p->tmp, w, wstride);
+        hipLaunchKernelGGL(k_mm_reduce<1>, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx, alpha,
+                           trans, 0, p->tmp, w, wstride);
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[t.L] > 0) {
-        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t, alpha, trans, 0,
-                           p->tmp, w, wstride);
+        hipLaunchKernelGGL(k_mm_finish<false>, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t, alpha,
+                           trans, 0, p->tmp, w, wstride);
         PLR_HIP(hipGetLastError());
     }
     dim3 grid((p->N + 63) / 64, t.ncg);
@@ -134,7 +186,7 @@ int plr_mm_reduce1(dplrh_plan *p, int d, const double *P, const double *X, int l
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[1] > 0) {
-        hipLaunchKernelGGL(k_mm_finish, dim3(t.fin_off[1] * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1, p->tmp,
+        hipLaunchKernelGGL(k_mm_finish<false>, dim3(t.fin_off[1] * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1, p->tmp,
                            T, tstride);
         PLR_HIP(hipGetLastError());
     }

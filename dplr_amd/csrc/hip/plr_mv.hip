@@ -631,7 +631,9 @@ int plr_gemm_direct(dplrh_plan *p, int trans, int n, double alpha, double beta,
     if (alpha == 0.0) return dplrh_scale(p->N, n, beta, dC, ldc, st);
     if (n >= 2 && plr_mm_ok(p) && ldb % 2 == 0 && !p->force_valu) {
         const int chunk = n < kMmChunk ? n : kMmChunk;
-        int rc = plr_workspace(p, chunk);
+        // whole 64-column groups: the multi-RHS product keeps its
+        // coefficients rank-major over padded columns
+        int rc = plr_workspace(p, (chunk + 63) & ~63);
         if (rc) return rc;
         for (int c0 = 0; c0 < n; c0 += chunk) {
             const int nc = n - c0 < chunk ? n - c0 : chunk;

@@ -112,3 +112,29 @@ def test_full_size_matvec(cuda, orc):
     x = np.random.default_rng(6).standard_normal(m)
     for trans in ("N", "T"):
         assert rel(D.gepmv(trans, A, x), D.gepmv(trans, A, x, api=orc)) < TOL
+
+
+@pytest.mark.parametrize("m,nlvl,rank,n", [
+    (4096, 6, 32, 70),    # leaf 64, two column groups
+    (4096, 5, 24, 16),    # leaf 128
+    (2048, 3, 40, 33),    # leaf 256, rank not a multiple of 16
+    (8192, 7, 20, 64),
+])
+@pytest.mark.parametrize("trans", ["N", "T"])
+def test_multi_rhs_product_fused_path(cuda, orc, m, nlvl, rank, n, trans):
+    """Even leading dimensions and N a multiple of 256: the all-level
+    reduction, rank-major coefficients and the barrier-free expansion."""
+    import torch
+    print(f"fused product m={m} L={nlvl} r={rank} n={n} {trans}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=25)
+    P = D.Plan(A)
+    rng = np.random.default_rng(7)
+    B = rng.standard_normal((m, n))
+    C0 = rng.standard_normal((m, n))
+    Bd = torch.tensor(B.T.copy(), device=cuda)
+    Cd = torch.tensor(C0.T.copy(), device=cuda)
+    P.gepdmm(trans, 0.75, Bd, -1.5, Cd)
+    torch.cuda.synchronize()
+    ref = D.gepdmm(trans, "N", A, B, n, 0.75, -1.5, C0, api=orc)
+    assert rel(Cd.cpu().numpy().T, ref) < TOL
+    P.close()

@@ -89,3 +89,28 @@ def test_refined_solve_shapes(cuda, m, nlvl, rank, n, dtype):
     for c in range(n):
         assert rel(X[c].cpu().numpy(), ref[c]) <= 1e-10
     P.close()
+
+
+@pytest.mark.parametrize("pad", [6, 3])
+def test_refined_solve_strided_blocks(cuda, pad):
+    """B and X inside larger arrays (leading dimension m + pad): an even
+    leading dimension reads B in place, an odd one goes through a copy."""
+    import torch
+    m, n = 4096, 20
+    print(f"mixed strided pad={pad}", flush=True)
+    A = PlrTree.random(m, 6, 32, seed=44)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    assert P.factor_mixed("f16") == 0
+    big = torch.tensor(np.random.default_rng(8).standard_normal((n, m + pad)), device=cuda)
+    B = big[:, :m]
+    Xbig = torch.full((n, m + pad), 7.0, dtype=torch.float64, device=cuda)
+    X = Xbig[:, :m]
+    P.solve_refined(B, X, 8)
+    X64 = torch.zeros((n, m), dtype=torch.float64, device=cuda)
+    P.solve("N", B.contiguous(), X64)
+    torch.cuda.synchronize()
+    assert rel(X.cpu().numpy(), X64.cpu().numpy()) <= 1e-10
+    assert torch.all(Xbig[:, m:] == 7.0)     # padding untouched
+    assert torch.equal(big[:, :m], B)        # B is not written
+    P.close()
