@@ -894,10 +894,6 @@ struct LpLvlTab
     const double *K64;
 };
 
-__device__ __forceinline__ f4 mfma_f32_4(float a, float b, f4 c)
-{
-    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
-}
 
 // smallest power of two >= v (1 for v = 0); v >= 0 as the bits of a float
 __device__ __forceinline__ float lp_pow2(unsigned bits)
@@ -912,10 +908,13 @@ __device__ __forceinline__ float lp_pow2(unsigned bits)
 // Capacitance rows for P tasks at once: task p computes the 16 x 16 tile
 // c = inv(K)[i0[p] .. +15, :] rhs[:, 16 columns], where row j of rhs is
 // lo[p][j * rs + m] for j < R and hi[p][(j - R) * rs + m] above (LDS), and
-// multiplies row i by -sv[p][v] (sig of that row).  Every K and sig load of
-// the P tasks is issued before the first MFMA.  K is column-major n2 x n2:
-// FP64 (FP64 MFMA) when K64 != null, else FP32 (FP32 MFMA).  Lane result:
-// w[p][v] = -sig * c[i0[p] + 4g + v][m].
+// multiplies row i by its sig (sv[p] points at the sig of row i0[p] + g;
+// row + 4v is sv[p][4v]).  Every K and sig load of the P tasks is issued
+// before the first MFMA.  K is column-major n2 x n2, stored in FP64 (K64)
+// or FP32 (K32); the arithmetic is FP64 MFMA either way (an FP32 product
+// loses digits to cancellation in inv(K) rhs that the refinement then has
+// to win back).  Lane result, in the FP64 MFMA's D layout:
+// w[p][v] = -sig * c[i0[p] + g + 4v][m].
 template <int P>
 __device__ __forceinline__ void lp_cap_tiles(const float *const *K32, const double *const *K64,
                                              int n2, int R, const int *i0,
@@ -927,11 +926,11 @@ __device__ __forceinline__ void lp_cap_tiles(const float *const *K32, const doub
 #pragma unroll
     for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) s[p][v] = sv[p][v];
-    if (K64[0]) {
-        plr_d4 acc[P];
+        for (int v = 0; v < 4; ++v) s[p][v] = sv[p][4 * v];
+    plr_d4 acc[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) acc[p] = plr_d4{0.0, 0.0, 0.0, 0.0};
+    for (int p = 0; p < P; ++p) acc[p] = plr_d4{0.0, 0.0, 0.0, 0.0};
+    if (K64[0]) {
         for (int j0 = 0; j0 < n2; j0 += 64) {
             double av[P][16];
 #pragma unroll
@@ -948,34 +947,29 @@ __device__ __forceinline__ void lp_cap_tiles(const float *const *K32, const doub
                     acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[p][u], (double)row[m], acc[p], 0, 0, 0);
                 }
         }
+    } else {
+        for (int j0 = 0; j0 < n2; j0 += 64) {
+            float av[P][16];
 #pragma unroll
-        for (int p = 0; p < P; ++p)
+            for (int p = 0; p < P; ++p)
 #pragma unroll
-            for (int v = 0; v < 4; ++v) w[p][v] = (float)(-(double)s[p][v] * acc[p][v]);
-        return;
-    }
-    f4 acc[P];
+                for (int u = 0; u < 16; ++u)
+                    av[p][u] = K32[p][(size_t)(j0 + 4 * u + g) * n2 + i0[p] + m];
 #pragma unroll
-    for (int p = 0; p < P; ++p) acc[p] = f4{0.f, 0.f, 0.f, 0.f};
-    for (int j0 = 0; j0 < n2; j0 += 64) {
-        float av[P][16];
+            for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int u = 0; u < 16; ++u) av[p][u] = K32[p][(size_t)(j0 + 4 * u + g) * n2 + i0[p] + m];
-#pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                const int j = j0 + 4 * u + g;
-                const float *row = j < R ? lo[p] + j * rs : hi[p] + (j - R) * rs;
-                acc[p] = mfma_f32_4(av[p][u], row[m], acc[p]);
-            }
+                for (int u = 0; u < 16; ++u) {
+                    const int j = j0 + 4 * u + g;
+                    const float *row = j < R ? lo[p] + j * rs : hi[p] + (j - R) * rs;
+                    acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64((double)av[p][u], (double)row[m],
+                                                                  acc[p], 0, 0, 0);
+                }
+        }
     }
 #pragma unroll
     for (int p = 0; p < P; ++p)
 #pragma unroll
-        for (int v = 0; v < 4; ++v) w[p][v] = -s[p][v] * acc[p][v];
+        for (int v = 0; v < 4; ++v) w[p][v] = (float)(-(double)s[p][v] * acc[p][v]);
 }
 
 // Eight consecutive floats of LDS as two 16-byte reads (rows are padded to
@@ -1189,8 +1183,9 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                     i0[p] = it * 16;
                     lo[p] = T + bot * tsz + ct[p] * 16;
                     hi[p] = T + top * tsz + ct[p] * 16;
-                    // the four rows of a lane lie in one half
-                    const int i = it * 16 + 4 * g, h = i >= R;
+                    // a lane's rows it*16 + g + 4v lie in one half (R is a
+                    // multiple of 16)
+                    const int i = it * 16 + g, h = it * 16 >= R;
                     kk[p] = h ? i - R : i;
                     sl[p] = h ? bot : top;
                     sv[p] = a.S[d] + (size_t)(sg0 + sl[p]) * R + kk[p];
@@ -1200,7 +1195,9 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
 #pragma unroll
                 for (int p = 0; p < 2; ++p) {
                     if (t0 + p >= ntask) continue;
-                    *(f4 *)(W + sl[p] * wsz + (ct[p] * 16 + m) * wsk + kk[p]) = w[p];
+                    float *wr = W + sl[p] * wsz + (ct[p] * 16 + m) * wsk + kk[p];
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) wr[4 * v] = w[p][v];
                     const float mx = fmaxf(fmaxf(fabsf(w[p][0]), fabsf(w[p][1])),
                                            fmaxf(fabsf(w[p][2]), fabsf(w[p][3])));
                     atomicMax(&smax[sl[p] * CW + ct[p] * 16 + m], __float_as_uint(mx));
@@ -1384,7 +1381,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
             k64[p] = a.K64 ? a.K64 + ko : nullptr;
             lo[p] = rhs + ct[p] * 16;
             hi[p] = rhs + R * rs + ct[p] * 16;
-            kk[p] += 4 * g;
+            kk[p] += g; // rows kk + 4v (the FP64 MFMA's D layout)
             sv[p] = a.S + (size_t)s * R + kk[p];
         }
         f4 w[NP];
@@ -1392,7 +1389,8 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             if (wv + 8 * p >= ntask) continue;
-            *(f4 *)(W + (ct[p] * 16 + m) * wsk + kk[p]) = w[p];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) W[(ct[p] * 16 + m) * wsk + kk[p] + 4 * v] = w[p][v];
             const float mx = fmaxf(fmaxf(fabsf(w[p][0]), fabsf(w[p][1])), fmaxf(fabsf(w[p][2]), fabsf(w[p][3])));
             atomicMax(&smax[ct[p] * 16 + m], __float_as_uint(mx));
         }

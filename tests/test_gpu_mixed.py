@@ -35,7 +35,13 @@ def test_refined_solve_reaches_1e10(cuda, orc, dtype, iters):
         torch.cuda.synchronize()
         errs.append(rel(Xd.cpu().numpy(), X64.cpu().numpy()))
     print(dtype, ["%.1e" % e for e in errs], flush=True)
-    assert errs[0] < 5e-2                      # one low-precision solve
+    # one low-precision solve, and the contraction of the refinement: a
+    # wrong capacitance row or level term shows up here long before the
+    # refined result misses its target (measured: f16 3.0e-4, 1.1e-7,
+    # 4.6e-11; bf16 2.4e-3, 7.0e-6, 2.3e-8, 8.3e-11)
+    first, by = (1e-3, 2) if dtype == "f16" else (1e-2, 3)
+    assert errs[0] < first
+    assert errs[by] < 1e-9
     assert errs[-1] <= 1e-10                   # refined to the target
     # residual through the oracle's FP64 matvec
     X = Xd.cpu().numpy().T
@@ -83,7 +89,7 @@ def test_refined_solve_shapes(cuda, m, nlvl, rank, n, dtype):
     P.solve_refined(B, X, 10)
     e10 = rel(X.cpu().numpy(), ref)
     print(f"one solve {e1:.1e}, ten {e10:.1e}", flush=True)
-    assert e1 < 5e-2
+    assert e1 < (2e-3 if dtype == "f16" else 1e-2)
     assert e10 <= 1e-10
     # every column on its own reaches the target too
     for c in range(n):
