@@ -844,15 +844,9 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
         plr_set_error("solve before a successful mixed-precision factorization");
         return -4;
     }
-    if (res) // reads results back to the host: not capturable
-        return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, res, (hipStream_t)stream);
-    const unsigned long long key[10] = {4, (unsigned long long)n, (unsigned long long)dB,
-                                        (unsigned long long)ldb, (unsigned long long)dX,
-                                        (unsigned long long)ldx, (unsigned long long)iters,
-                                        (unsigned long long)q};
-    return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
-        return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, nullptr, s);
-    });
+    // Launched directly: with kernels of 5-100 us the stream stays fed, and
+    // measured, the graph's own start cost more than it saved here.
+    return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, res, (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------

@@ -598,13 +598,6 @@ int plr_mv_launch(dplrh_plan *p, int trans, int ncols, double alpha, double beta
     return 0;
 }
 
-static unsigned long long dbits(double v)
-{
-    unsigned long long u;
-    memcpy(&u, &v, sizeof(u));
-    return u;
-}
-
 static int gemv_body(dplrh_plan *p, int trans, double alpha, double beta, const double *dx,
                      double *dy, hipStream_t st)
 {
@@ -616,11 +609,9 @@ extern "C" int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
                           const double *dx, double *dy, void *stream)
 {
     if (p->N == 0) return 0;
-    const unsigned long long key[10] = {1, (unsigned long long)trans, dbits(alpha), dbits(beta),
-                                        (unsigned long long)dx, (unsigned long long)dy};
-    return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
-        return gemv_body(p, trans, alpha, beta, dx, dy, s);
-    });
+    // Launched directly: three kernels finish sooner than a graph launch of
+    // them (the graph's own start costs about as much as a kernel boundary).
+    return gemv_body(p, trans, alpha, beta, dx, dy, (hipStream_t)stream);
 }
 
 static constexpr int kMmChunk = 128;
@@ -650,11 +641,6 @@ extern "C" int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double 
                           const double *dB, int ldb, double *dC, int ldc, void *stream)
 {
     if (p->N == 0 || n == 0) return 0;
-    const unsigned long long key[10] = {2, (unsigned long long)trans, (unsigned long long)n,
-                                        dbits(alpha), dbits(beta), (unsigned long long)dB,
-                                        (unsigned long long)ldb, (unsigned long long)dC,
-                                        (unsigned long long)ldc};
-    return plr_run_cached(p, key, (hipStream_t)stream, [&](hipStream_t s) {
-        return plr_gemm_direct(p, trans, n, alpha, beta, dB, ldb, dC, ldc, s);
-    });
+    // launched directly, as the matvec (three kernels per 128 columns)
+    return plr_gemm_direct(p, trans, n, alpha, beta, dB, ldb, dC, ldc, (hipStream_t)stream);
 }

@@ -113,3 +113,27 @@ def test_solve_bottom_levels_in_one_kernel(cuda, orc, m, nlvl, n):
     assert np.linalg.norm(X.cpu().numpy() - ref) / np.linalg.norm(ref) < 1e-12
     P.close()
     A.free()
+
+
+def test_solve_graph_replays_match_eager(cuda):
+    """The FP64 solve runs eagerly on its first call, is captured on the
+    second and replayed after that; other GPU work and synchronizations sit
+    between the calls, and every result must equal the eager one."""
+    import torch
+    m, n = 8192, 24
+    A = PlrTree.random(m, 7, 32, seed=31)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    B = torch.randn(n, m, dtype=torch.float64, device=cuda)
+    X0 = torch.zeros_like(B)
+    P.solve("N", B, X0)
+    torch.cuda.synchronize()
+    ref = X0.clone()
+    scratch = torch.randn(2048, 2048, device=cuda)
+    for rep in range(4):
+        X = torch.full_like(B, float(rep))
+        P.solve("N", B, X)
+        scratch = scratch @ scratch.T * 1e-3          # eager work between calls
+        torch.cuda.synchronize()
+        assert torch.equal(X, ref), rep
+    P.close()
