@@ -160,6 +160,10 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
                    double *w, size_t wstride, double *bmax, hipStream_t st);
 int plr_mm_reduce1(dplrh_plan *p, int d, const double *P, const double *X, int ldx, int ncols,
                    double *T, size_t tstride, hipStream_t st);
+int plr_mm_part(dplrh_plan *p, size_t blocks);
+int plr_mm_finish_raw(dplrh_plan *p, int d, int ncols, int tmax, double *T, size_t tstride,
+                      hipStream_t st);
+size_t plr_mm_part_blocks(const dplrh_plan *p, int d0, int d1, int ncols, int tmax);
 int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,
                    const double *D, int trans, const double *X, int ldx, double alpha, double beta,
                    double *Y, int ldy, int ncols, hipStream_t st);

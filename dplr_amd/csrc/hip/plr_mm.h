@@ -712,7 +712,7 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
 // Table for levels [d0, d1) with operands P (reduced), Q (expanded) and S.
 static inline void mm_tab(const dplrh_plan *p, int d0, int d1, const double *const *P,
                    const double *const *Q, const double *const *S, int ncols, int w_local,
-                   MmTab *t)
+                   MmTab *t, int tmax = MM_TMAX)
 {
     memset(t, 0, sizeof(*t));
     t->L = d1 - d0;
@@ -725,7 +725,7 @@ static inline void mm_tab(const dplrh_plan *p, int d0, int d1, const double *con
     for (int i = 0; i < t->L; ++i) {
         const int d = d0 + i;
         const int seg = p->N >> (d + 1), nseg = 2 << d, R = p->R[d];
-        const int T = seg < MM_TMAX ? seg : MM_TMAX;
+        const int T = seg < tmax ? seg : tmax;
         const int tps = seg / T;
         const int nkg = (R + MM_KG - 1) / MM_KG;
         t->R[i] = R;

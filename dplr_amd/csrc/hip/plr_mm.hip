@@ -211,3 +211,34 @@ int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_
     PLR_HIP(hipGetLastError());
     return 0;
 }
+
+// T[c][s*R_d + k] = sum of the raw partial blocks in p->tmp that a caller
+// wrote in mm_reduce_body's layout with tiles of `tmax` rows (level d,
+// ncols columns).
+int plr_mm_finish_raw(dplrh_plan *p, int d, int ncols, int tmax, double *T, size_t tstride,
+                      hipStream_t st)
+{
+    const double *P = nullptr, *Q = nullptr, *S = nullptr;
+    MmTab t;
+    mm_tab(p, d, d + 1, &P, &Q, &S, ncols, 1, &t, tmax);
+    if (t.fin_off[1] > 0) {
+        hipLaunchKernelGGL(k_mm_finish<false>, dim3(t.fin_off[1] * MM_FCH), dim3(256), 0, st, t, 1.0,
+                           0, 1, p->tmp, T, tstride);
+        PLR_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// Partial blocks (of MM_BLK doubles) the levels [d0, d1) need with tiles of
+// `tmax` rows and ncols columns.
+size_t plr_mm_part_blocks(const dplrh_plan *p, int d0, int d1, int ncols, int tmax)
+{
+    size_t need = 0;
+    for (int d = d0; d < d1; ++d) {
+        const double *P = nullptr, *Q = nullptr, *S = nullptr;
+        MmTab t;
+        mm_tab(p, d, d + 1, &P, &Q, &S, ncols, 1, &t, tmax);
+        if ((size_t)t.part_off[1] > need) need = t.part_off[1];
+    }
+    return need;
+}

@@ -501,6 +501,8 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
 }
 
 // X := op(A)^{-1} B for n device columns (ldb, ldx >= N), after dplrh_factor.
+int plr_solve_fast(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
+                   hipStream_t st);
 int plr_solve_tail(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
                    hipStream_t st);
 
@@ -522,6 +524,12 @@ static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb
         double *Z = X;
         int ldz = ldx;
         int dstop = L; // levels [dstop, L) and the leaves done by the tail kernel
+        if (!trans && mm) {
+            rc = plr_solve_fast(p, B, ldb, X, ldx, nc, st);
+            if (rc < 0) return rc;
+            if (rc == 0) continue;
+            rc = 0;
+        }
         if (!trans) {
             const int d0 = mm ? plr_solve_tail(p, B, ldb, X, ldx, nc, st) : -1;
             if (d0 >= 0)

@@ -10,10 +10,12 @@
 //     reduce: T_s = V_d|_s^T z|_s                          (A = V from HBM, B = z in LDS)
 //     cap:    c = inv(K_node) [T_bot; T_top], W_top = -S_top .* c1, W_bot = -S_bot .* c2
 //     expand: z|_s += Ut_d|_s W_s                         (A = Ut from HBM, B = W in LDS)
-// The levels above d0 keep their per-level kernels.
+// When Rn > 0 the kernel also leaves T = V_{d0-1}^T z of its span (one
+// segment of level d0 - 1) for the upper levels (k_fp_level).
 #include <cstring>
 
 #include "plr_common.h"
+#include "plr_mm.h"
 
 typedef double tl_d4 __attribute__((ext_vector_type(4)));
 
@@ -33,6 +35,9 @@ struct TailTab
     int R[PLR_MAXL];
     const double *V[PLR_MAXL], *Ut[PLR_MAXL], *S[PLR_MAXL], *Kinv[PLR_MAXL];
     const double *Dinv;
+    int Rn;           // rank of level d0 - 1 when its T is wanted (multiple of 16), else 0
+    double *Tout;     // T[c][s * Rn + k] of level d0 - 1, column stride tstride
+    size_t tstride;
 };
 
 // LDS: z[c][row] (row stride span + 1), T and W as [seg][k][c] (k stride 17)
@@ -174,17 +179,282 @@ __global__ __launch_bounds__(64 * TL_NW) void k_solve_tail(TailTab a, const doub
         __syncthreads();
     }
 
-    // 4. write the span back
+    // 4. T of level d0 - 1 over the span: tasks (16-rank tile, 32-row chunk),
+    // chunk partials combined in a fixed order through the (free) T region
+    if (a.Rn > 0) {
+        const int Rn = a.Rn, nkt = Rn / 16, nch = span / 32, ntask = nkt * nch;
+        const double *Vn = a.V[a.d0 - 1];
+        double *red = T; // [task][lane][4]
+        for (int task = wv; task < ntask; task += TL_NW) {
+            const int kt = task / nch, ch = task - kt * nch;
+            const double *pv = Vn + (size_t)(kt * 16 + m) * N + row0 + ch * 32 + g;
+            double av[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) av[u] = pv[4 * u];
+            tl_d4 acc = tl_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = tl_mfma(av[u], z[m * zs + ch * 32 + 4 * u + g], acc);
+            // lane holds T[kt*16 + g + 4v][column m]
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[(task * 64 + lane) * 4 + v] = acc[v];
+        }
+        __syncthreads();
+        const int sg = row0 / span;
+        for (int kt = wv; kt < nkt; kt += TL_NW) {
+            double t[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int ch = 0; ch < nch; ++ch)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) t[v] += red[((kt * nch + ch) * 64 + lane) * 4 + v];
+            if (m < nc)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    a.Tout[(size_t)(c0 + m) * a.tstride + (size_t)sg * Rn + kt * 16 + g + 4 * v] = t[v];
+        }
+    }
+
+    // 5. write the span back
     for (int e = threadIdx.x; e < TL_CW * span; e += 64 * TL_NW) {
         const int c = e / span, r = e - c * span;
         if (c < nc) X[(size_t)(c0 + c) * ldx + row0 + r] = z[c * zs + r];
     }
 }
 
-// The host side: handles the leaves and the levels d >= d0, returns d0 (the
-// caller runs the levels above), or -1 when the shape does not fit.
+// ---------------------------------------------------------------------------
+// One pass per level above the tail (d = d0-1 .. 0), FP64 MFMA throughout.
+// A work-group owns 128 rows x 64 right-hand sides, four waves of 32 rows:
+//   1. rhs = [T_bot; T_top] of its node into LDS
+//   2. W = -S_s .* (the rows of inv(K) for its segment s) rhs   (tasks (kt, ct))
+//   3. z^T += W^T Ut_d^T: a lane holds z[column g + 4v][row m] of each tile,
+//      so z is read and written 16 consecutive rows (128 bytes) per lane group
+//   4. z^T into LDS, then the partial V_{d-1}^T z over the 128 rows: wave w
+//      takes rank tile w/2 and two column tiles; in MFMA step t of a 64-row
+//      block lane group g contributes row 16g + t, so each lane streams 16
+//      consecutive rows of one V column.  Written in mm_reduce_body's layout
+//      for k_mm_finish (tiles of 128 rows).
+// The old z and the Ut rows are loaded at the start, the V rows while the
+// expansion runs.  Ranks R and Rn are multiples of 16 and at most 32.
+static constexpr int FL_RB = 128, FL_CW = 64, FL_ZS = FL_RB + 1;
+
+struct FpLvlTab
+{
+    int N, R, Rn, seg, segn, tps, ncg;
+    const double *Ut, *Vn, *S, *Kinv;
+};
+
+static size_t fl_lds(int R)
+{
+    const size_t cap = (size_t)3 * R * (FL_CW + 1), zt = (size_t)FL_CW * FL_ZS;
+    return (cap > zt ? cap : zt) * sizeof(double);
+}
+
+__global__ __launch_bounds__(256, 2) void k_fp_level(FpLvlTab a, const double *__restrict__ Tin,
+                                                     size_t tstride, double *__restrict__ Z, int ldz,
+                                                     double *__restrict__ part, int ncols)
+{
+    extern __shared__ double lds[];
+    const int R = a.R, n2 = 2 * R, rs = FL_CW + 1;
+    double *rhs = lds;         // [n2][rs]
+    double *W = lds + n2 * rs; // [R][rs]: W[k][column]
+    double *zt = lds;          // after the expansion: [column][FL_ZS]
+    const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wv = tid >> 6;
+    const int r0 = blockIdx.x * FL_RB, c0 = blockIdx.y * FL_CW, N = a.N;
+    const int rw = 32 * wv; // this wave's rows (local)
+    const int nc = min(FL_CW, ncols - c0);
+    const int s = r0 / a.seg, istop = !(s & 1), top = s & ~1, bot = top + 1;
+    const int nk4 = R / 4, nktn = a.Rn / 16;
+
+    // old z: zo[rt][ct][v] = z[r0 + rw + 16rt + m][c0 + 16ct + g + 4v]
+    double zo[2][4][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                zo[rt][ct][v] = Z[(size_t)(c0 + min(16 * ct + g + 4 * v, nc - 1)) * ldz + r0 + rw + 16 * rt + m];
+    // Ut of the expansion: ua[rt][u] = Ut[4u + g][r0 + rw + 16rt + m]
+    double ua[2][8];
+    if (R > 0) // a level of rank 0 has no Ut rows
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                ua[rt][u] = a.Ut[(size_t)min(4 * u + g, R - 1) * N + r0 + rw + 16 * rt + m];
+    // 1. rhs = [T_bot; T_top] of the node
+    for (int e0 = 0; e0 < n2 * FL_CW; e0 += 8 * 256) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = min(e0 + u * 256 + tid, n2 * FL_CW - 1);
+            const int cl = e / n2, j = e - cl * n2, hi = j >= R;
+            const int sg = hi ? top : bot, kk = hi ? j - R : j;
+            v[u] = Tin[(size_t)(c0 + min(cl, nc - 1)) * tstride + (size_t)sg * R + kk] *
+                   (double)(cl < nc);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 256 + tid;
+            if (e < n2 * FL_CW) {
+                const int cl = e / n2, j = e - cl * n2;
+                rhs[j * rs + cl] = v[u];
+            }
+        }
+    }
+    __syncthreads();
+    // 2. W[k][c] = -S_s[k] * sum_j inv(K)[row(k)][j] rhs[j][c]; row(k) = k (top) or R + k
+    {
+        const double *K = a.Kinv + (size_t)(s >> 1) * n2 * n2 + (istop ? 0 : R);
+        const int ntask = (R / 16) * 4;
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int task = wv + 4 * p;
+            if (task < ntask) {
+                const int kt = task >> 2, ct = task & 3;
+                double kv[16];
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    kv[u] = K[(size_t)min(4 * u + g, n2 - 1) * n2 + kt * 16 + m];
+                tl_d4 acc = tl_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int u = 0; u < 16; ++u)
+                    if (4 * u < n2) acc = tl_mfma(kv[u], rhs[(4 * u + g) * rs + ct * 16 + m], acc);
+                // lane holds c[kt*16 + g + 4v][ct*16 + m]
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = kt * 16 + g + 4 * v;
+                    W[k * rs + ct * 16 + m] = -a.S[(size_t)s * R + k] * acc[v];
+                }
+            }
+        }
+    }
+    __syncthreads();
+    // V rows of the reduction (level d-1), in flight during the expansion:
+    // wave w takes rank tile kt = w / 2; va[blk][h] = rows 64blk + 16g + 2h, +1
+    const int kt = wv >> 1;
+    double2 va[2][8];
+    if (kt < nktn) {
+        const double *pv = a.Vn + (size_t)(kt * 16 + m) * N + r0 + 16 * g;
+#pragma unroll
+        for (int blk = 0; blk < 2; ++blk)
+#pragma unroll
+            for (int h = 0; h < 8; ++h) va[blk][h] = *(const double2 *)(pv + 64 * blk + 2 * h);
+    }
+    // 3. z^T tiles: zn = zo + W^T Ut^T
+    tl_d4 za[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) za[rt][ct] = tl_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+        if (u < nk4) {
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const double w = W[(4 * u + g) * rs + ct * 16 + m];
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) za[rt][ct] = tl_mfma(w, ua[rt][u], za[rt][ct]);
+            }
+        }
+    }
+    if (nktn > 0) __syncthreads(); // W is read by every wave before zt overwrites it
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = 16 * ct + g + 4 * v, i = rw + 16 * rt + m;
+                const double zn = zo[rt][ct][v] + za[rt][ct][v];
+                if (c < nc) Z[(size_t)(c0 + c) * ldz + r0 + i] = zn;
+                if (nktn > 0) zt[c * FL_ZS + i] = zn;
+            }
+    if (nktn == 0) return;
+    __syncthreads();
+    // 4. partial V_{d-1}^T z over the 128 rows: tasks (kt, ct) with ct = 2(w&1), +1
+    if (kt < nktn) {
+        const int ct0 = 2 * (wv & 1);
+        tl_d4 ta[2] = {tl_d4{0.0, 0.0, 0.0, 0.0}, tl_d4{0.0, 0.0, 0.0, 0.0}};
+#pragma unroll
+        for (int blk = 0; blk < 2; ++blk)
+#pragma unroll
+            for (int t = 0; t < 16; ++t) {
+                const double av = (t & 1) ? va[blk][t >> 1].y : va[blk][t >> 1].x;
+                const int row = 64 * blk + 16 * g + t;
+#pragma unroll
+                for (int q = 0; q < 2; ++q)
+                    ta[q] = tl_mfma(av, zt[((ct0 + q) * 16 + m) * FL_ZS + row], ta[q]);
+            }
+        // lane holds T[kt*16 + g + 4v][(ct0+q)*16 + m]
+        const int sn = r0 / a.segn, tile = (r0 - sn * a.segn) / FL_RB, st = sn * a.tps + tile;
+        double *pp = part + ((size_t)st * a.ncg + blockIdx.y) * MM_BLK; // one rank group (Rn <= 32)
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                pp[((ct0 + q) * 16 + m) * MM_KG + kt * 16 + g + 4 * v] = ta[q][v];
+    }
+}
+
+static int solve_tail_t(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
+                        int want_t, hipStream_t st);
+
 int plr_solve_tail(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
                    hipStream_t st)
+{
+    return solve_tail_t(p, Bm, ldb, Xm, ldx, ncols, 0, st);
+}
+
+// The whole non-transposed FP64 solve for ncols columns: k_solve_tail (with T
+// of the first level above it), then per level k_fp_level and the finishing
+// sums.  Returns 1, launching nothing, when the shape is not covered (leaf
+// 64, N a multiple of 256, ranks above the tail multiples of 16 and at most
+// 32, the workspace T sized by the caller for ncols).
+int plr_solve_fast(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
+                   hipStream_t st)
+{
+    const int N = p->N, L = p->L;
+    if (L == 0 || ncols <= 0 || p->b != 64 || N % 256 != 0) return 1;
+    int d0 = 0;
+    while ((N >> d0) > TL_MAXSPAN) ++d0;
+    if (d0 == 0 || d0 > L - 1) return 1;
+    for (int d = 0; d < d0; ++d)
+        if (p->R[d] % 16 != 0 || p->R[d] > 32) return 1;
+    int rc = plr_mm_part(p, plr_mm_part_blocks(p, 0, d0 - 1, ncols, FL_RB));
+    if (rc) return rc;
+    rc = solve_tail_t(p, Bm, ldb, Xm, ldx, ncols, 1, st);
+    if (rc < 0) return 1; // the tail does not fit: nothing launched
+    static bool attr = false;
+    if (!attr) {
+        PLR_HIP(hipFuncSetAttribute((const void *)k_fp_level,
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 80 * 1024));
+        attr = true;
+    }
+    const int ncg = (ncols + FL_CW - 1) / FL_CW;
+    for (int d = d0 - 1; d >= 0; --d) {
+        FpLvlTab t;
+        memset(&t, 0, sizeof(t));
+        t.N = N;
+        t.R = p->R[d];
+        t.Rn = d > 0 ? p->R[d - 1] : 0;
+        t.seg = N >> (d + 1);
+        t.segn = N >> d;
+        t.tps = t.segn / FL_RB;
+        t.ncg = ncg;
+        t.Ut = p->f.Ut + p->uoff[d];
+        t.Vn = d > 0 ? p->V + p->uoff[d - 1] : nullptr;
+        t.S = p->S + p->soff[d];
+        t.Kinv = p->f.Kinv + p->f.koff[d];
+        hipLaunchKernelGGL(k_fp_level, dim3(N / FL_RB, ncg), dim3(256), fl_lds(t.R), st, t, p->sw.T,
+                           p->sw.t_per_col, Xm, ldx, p->tmp, ncols);
+        PLR_HIP(hipGetLastError());
+        if (t.Rn > 0 && (rc = plr_mm_finish_raw(p, d - 1, ncols, FL_RB, p->sw.T, p->sw.t_per_col, st)))
+            return rc;
+    }
+    return 0;
+}
+
+static int solve_tail_t(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
+                        int want_t, hipStream_t st)
 {
     const int N = p->N, L = p->L, b = p->b;
     if (L == 0 || ncols <= 0 || b != 64) return -1;
@@ -194,7 +464,7 @@ int plr_solve_tail(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx
     if (d0 > L - 1) return -1;
     const int span = N >> d0;
     int RT = 16;
-    for (int d = d0; d < L; ++d) {
+    for (int d = d0 > 0 && want_t ? d0 - 1 : d0; d < L; ++d) {
         if (p->R[d] > TL_MAXR) return -1;
         RT = ((p->R[d] + 15) / 16) * 16 > RT ? ((p->R[d] + 15) / 16) * 16 : RT;
     }
@@ -214,6 +484,11 @@ int plr_solve_tail(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx
         t.Kinv[d] = p->f.Kinv + p->f.koff[d];
     }
     t.Dinv = p->f.Dinv;
+    if (want_t && d0 > 0) {
+        t.Rn = p->R[d0 - 1];
+        t.Tout = p->sw.T;
+        t.tstride = p->sw.t_per_col;
+    }
     const size_t tw = (size_t)(span / b) * RT * (TL_CW + 1);
     const size_t lds = ((size_t)TL_CW * (span + 1) + 2 * tw) * sizeof(double);
     static bool attr = false;

@@ -137,3 +137,25 @@ def test_solve_graph_replays_match_eager(cuda):
         torch.cuda.synchronize()
         assert torch.equal(X, ref), rep
     P.close()
+
+
+@pytest.mark.parametrize("m,nlvl,rank,n", [(8192, 7, 32, 64), (16384, 8, 16, 70), (4096, 6, 32, 5)])
+def test_solve_upper_levels_fused(cuda, orc, m, nlvl, rank, n):
+    """Leaf 64, N a multiple of 256, ranks 16 or 32: the bottom kernel, then one
+    fused pass per level above it (capacitance rows, expansion, the next
+    level's partial reduction); against the oracle's inverse applied to B."""
+    import torch
+    print(f"fused solve m={m} L={nlvl} r={rank} n={n}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=32)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    rng = np.random.default_rng(9)
+    B = rng.standard_normal((m, n))
+    Bd = torch.tensor(B.T.copy(), device=cuda)
+    Xd = torch.zeros_like(Bd)
+    P.solve("N", Bd, Xd)
+    torch.cuda.synchronize()
+    X = Xd.cpu().numpy().T
+    r = D.gepdmm("N", "N", A, X, n, api=orc) - B
+    assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-12
+    P.close()
