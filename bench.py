@@ -172,6 +172,11 @@ def batch_leg(args, torch, D, log):
     Y = [torch.zeros_like(x) for x in X]
     Z = [torch.zeros_like(x) for x in X]
 
+    # autotuner: the tile shape (rows per reduction task) for this batch
+    tile, tile_ms = B.tune(X, Y, reps=3)
+    log(f"batch: tuner picked {tile}-row tiles ("
+        + ", ".join(f"{k}: {v:.2f} ms" for k, v in tile_ms.items()) + ")")
+
     def both(_):
         B.gepdmm(X, Y)
         B.solve(X, Z)
@@ -201,6 +206,7 @@ def batch_leg(args, torch, D, log):
            "rhs_s": total / (ms * 1e-3), "ms_per_batch": ms,
            "apply_ms": ms_apply, "solve_ms": ms_solve,
            "max_rel_residual": err, "dtype": "f64",
+           "tile_rows": tile, "tuner_ms": tile_ms,
            "per_matrix_ms_p50": float(np.percentile(lat, 50)),
            "per_matrix_ms_p99": float(np.percentile(lat, 99)),
            "one_at_a_time_rhs_s": total / (lat.sum() * 1e-3),

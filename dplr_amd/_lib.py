@@ -50,6 +50,9 @@ def _load():
                                        POINTER(c_int), P]),
         "dplrx_batch_solve": (c_int, [P, c_int, POINTER(P), POINTER(c_int), POINTER(P),
                                       POINTER(c_int), P]),
+        "dplrx_batch_set_tile": (c_int, [P, c_int]),
+        "dplrx_batch_tune": (c_int, [P, c_int, POINTER(P), POINTER(c_int), POINTER(P),
+                                     POINTER(c_int), c_int, POINTER(c_int), PD, P]),
         "dplrx_gen_random": (c_int, [c_int, c_int, c_int, c_double, c_ulonglong, PM]),
         "dplrx_gen_normal": (c_int, [c_size_t, c_ulonglong, c_uint, PD]),
     }

@@ -74,6 +74,10 @@ int dplrh_batch_gemm(dplrh_batch *B, int nrhs, const double *const *X, const int
                      double *const *Y, const int *ldy, void *stream);
 int dplrh_batch_solve(dplrh_batch *B, int nrhs, const double *const *X, const int *ldx,
                       double *const *Y, const int *ldy, void *stream);
+int dplrh_batch_set_tile(dplrh_batch *B, int tmax);
+int dplrh_batch_tune(dplrh_batch *B, int nrhs, const double *const *X, const int *ldx,
+                     double *const *Y, const int *ldy, int reps, int *tmax_out, double *ms,
+                     void *stream);
 int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
                const double *dB, int ldb, double *dC, int ldc, void *stream);
 

@@ -11,6 +11,11 @@
 // The whole sequence is captured once into a HIP graph per (operation,
 // RHS count) and replayed; the tables are rebuilt when a plan's device
 // memory changes (its workspace generation).
+//
+// Tile shape: the rows per reduction task (tmax) trades parallelism (more,
+// shorter tasks) against partial sums (more work for the finishing pass).
+// dplrh_batch_tune times the apply + solve of the caller's batch for each
+// candidate and keeps the fastest; without it the batch uses MM_TMAX.
 #include <cstring>
 #include <vector>
 
@@ -144,6 +149,7 @@ struct dplrh_batch
     size_t cap_lds;
     hipStream_t cap;
     std::vector<batch_graph> graphs;
+    int tmax; // rows per reduction task (tile shape)
 };
 
 static void batch_free_tables(dplrh_batch *B)
@@ -185,6 +191,7 @@ extern "C" int dplrh_batch_create(int count, dplrh_plan *const *plans, dplrh_bat
     B->Lmax = 0;
     for (auto *p : B->plans) B->Lmax = p->L > B->Lmax ? p->L : B->Lmax;
     B->tsl = B->Lmax + 2;
+    B->tmax = MM_TMAX;
     if (hipStreamCreateWithFlags(&B->cap, hipStreamNonBlocking) != hipSuccess) {
         delete B;
         plr_set_error("stream creation failed");
@@ -231,14 +238,14 @@ static int batch_build(dplrh_batch *B, int kind, int nrhs)
             S[d] = p->S + p->soff[d];
         }
         MmTab *ti = &tabs[(size_t)i * tsl];
-        mm_tab(p, 0, p->L, P, Q, S, nrhs, 0, &ti[0]);
-        mm_tab(p, 0, 0, P, Q, S, nrhs, 1, &ti[1]);
+        mm_tab(p, 0, p->L, P, Q, S, nrhs, 0, &ti[0], B->tmax);
+        mm_tab(p, 0, 0, P, Q, S, nrhs, 1, &ti[1], B->tmax);
         size_t maxpart = (size_t)ti[0].part_off[ti[0].L];
         for (int d = 0; d < p->L; ++d) {
             const double *Pd = p->V + p->uoff[d];
             const double *Qd = p->f.done ? p->f.Ut + p->uoff[d] : p->U + p->uoff[d];
             const double *Sd = p->S + p->soff[d];
-            mm_tab(p, d, d + 1, &Pd, &Qd, &Sd, nrhs, 1, &ti[2 + d]);
+            mm_tab(p, d, d + 1, &Pd, &Qd, &Sd, nrhs, 1, &ti[2 + d], B->tmax);
             if ((size_t)ti[2 + d].part_off[1] > maxpart) maxpart = ti[2 + d].part_off[1];
         }
         int rc = plr_mm_part(p, maxpart);
@@ -412,4 +419,82 @@ extern "C" int dplrh_batch_solve(dplrh_batch *B, int nrhs, const double *const *
                                  double *const *Y, const int *ldy, void *stream)
 {
     return batch_run(B, 1, nrhs, X, ldx, Y, ldy, (hipStream_t)stream);
+}
+
+// Autotuner: for each candidate tile (rows per reduction task) rebuild the
+// tables, run the batch's apply and solve eagerly (one warm-up, then `reps`
+// timed with events) and keep the fastest.  ms[i] receives the time of
+// candidate i (apply + solve, per repetition) when not null.  The tables
+// and graphs are rebuilt for the chosen tile on the next call.
+static const int kTuneTiles[] = {128, 256, 512, 1024};
+
+extern "C" int dplrh_batch_tune(dplrh_batch *B, int nrhs, const double *const *X, const int *ldx,
+                                double *const *Y, const int *ldy, int reps, int *tmax_out,
+                                double *ms, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (nrhs <= 0 || reps <= 0) {
+        plr_set_error("tuning needs nrhs > 0 and reps > 0");
+        return -1;
+    }
+    for (int i = 0; i < B->count; ++i)
+        if (!B->plans[i]->f.done) {
+            plr_set_error("tuning runs the solve: factor every plan first");
+            return -4;
+        }
+    std::vector<BIo> hio(B->count);
+    for (int i = 0; i < B->count; ++i) hio[i] = BIo{X[i], Y[i], ldx[i], ldy[i]};
+    BIo *dio = nullptr;
+    PLR_HIP(hipMalloc(&dio, hio.size() * sizeof(BIo)));
+    PLR_HIP(hipMemcpy(dio, hio.data(), hio.size() * sizeof(BIo), hipMemcpyHostToDevice));
+    hipEvent_t e0, e1;
+    PLR_HIP(hipEventCreate(&e0));
+    PLR_HIP(hipEventCreate(&e1));
+    int best = B->tmax, rc = 0;
+    float best_ms = 0.f;
+    const int ncand = (int)(sizeof(kTuneTiles) / sizeof(kTuneTiles[0]));
+    for (int c = 0; c < ncand && !rc; ++c) {
+        B->tmax = kTuneTiles[c];
+        if ((rc = batch_build(B, 1, nrhs))) break;
+        for (int i = 0; i < B->count; ++i) B->gen[i] = B->plans[i]->ws_gen;
+        float t = 0.f;
+        for (int r = 0; r <= reps && !rc; ++r) {
+            if (r == 1) PLR_HIP(hipEventRecord(e0, st));
+            if (!(rc = batch_launch(B, 0, nrhs, dio, st))) rc = batch_launch(B, 1, nrhs, dio, st);
+        }
+        if (rc) break;
+        PLR_HIP(hipEventRecord(e1, st));
+        PLR_HIP(hipEventSynchronize(e1));
+        PLR_HIP(hipEventElapsedTime(&t, e0, e1));
+        t /= reps;
+        if (ms) ms[c] = t;
+        if (c == 0 || t < best_ms) {
+            best_ms = t;
+            best = kTuneTiles[c];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(dio);
+    B->tmax = best;
+    B->built_nrhs = -1; // rebuild (and recapture) with the chosen tile
+    batch_free_tables(B);
+    if (tmax_out) *tmax_out = best;
+    return rc;
+}
+
+extern "C" int dplrh_batch_set_tile(dplrh_batch *B, int tmax)
+{
+    bool ok = false;
+    for (int t : kTuneTiles) ok = ok || t == tmax;
+    if (!ok) {
+        plr_set_error("tile %d is not one of 128, 256, 512, 1024", tmax);
+        return -1;
+    }
+    if (B->tmax != tmax) {
+        B->tmax = tmax;
+        B->built_nrhs = -1;
+        batch_free_tables(B);
+    }
+    return 0;
 }

@@ -524,7 +524,9 @@ static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb
         double *Z = X;
         int ldz = ldx;
         int dstop = L; // levels [dstop, L) and the leaves done by the tail kernel
-        if (!trans && mm) {
+        // the fused path reads B and X with scalar loads: any leading
+        // dimension and a single column are fine
+        if (!trans && plr_mm_ok(p) && !p->force_valu) {
             rc = plr_solve_fast(p, B, ldb, X, ldx, nc, st);
             if (rc < 0) return rc;
             if (rc == 0) continue;

@@ -244,6 +244,15 @@ int dplrx_batch_solve(dplrx_batch *batch, int nrhs, const double *const *dB, con
     return dplrh_batch_solve(batch, nrhs, dB, ldb, dX, ldx, stream);
 }
 
+int dplrx_batch_tune(dplrx_batch *batch, int nrhs, const double *const *dX, const int *ldx,
+                     double *const *dY, const int *ldy, int reps, int *tile, double *ms,
+                     void *stream)
+{
+    return dplrh_batch_tune(batch, nrhs, dX, ldx, dY, ldy, reps, tile, ms, stream);
+}
+
+int dplrx_batch_set_tile(dplrx_batch *batch, int tile) { return dplrh_batch_set_tile(batch, tile); }
+
 /* ---- adjoints ----------------------------------------------------------- */
 
 int dplrx_grad_outer(dplrx_plan *plan, int n, double alpha, const double *dGa, int lda,

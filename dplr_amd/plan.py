@@ -178,3 +178,23 @@ class Batch:
         check(lib.dplrx_batch_solve(self.b, nrhs, bs, lb, xs, lx, _stream(stream)),
               "dplrx_batch_solve")
         return X
+
+    TILES = (128, 256, 512, 1024)
+
+    def set_tile(self, tile):
+        """Rows per reduction task (one of TILES), without timing."""
+        check(lib.dplrx_batch_set_tile(self.b, tile), "dplrx_batch_set_tile")
+
+    def tune(self, X, Y, reps=3, stream=None):
+        """Autotuner: times apply + solve of this batch (X read, Y written) for
+        each candidate tile shape (rows per reduction task) and keeps the
+        fastest.  Returns (tile, {tile: ms per repetition})."""
+        import numpy as np
+        nrhs = X[0].shape[0]
+        xs, lx, ys, ly = self._io(X, Y)
+        tile = c_int(0)
+        ms = np.zeros(len(self.TILES))
+        check(lib.dplrx_batch_tune(self.b, nrhs, xs, lx, ys, ly, reps, byref(tile),
+                                   ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                   _stream(stream)), "dplrx_batch_tune")
+        return tile.value, dict(zip(self.TILES, ms.tolist()))

@@ -94,6 +94,17 @@ int dplrx_batch_gepdmm(dplrx_batch *batch, int nrhs, const double *const *dX, co
 /* X[i] := A_i^{-1} B[i], every plan factored with dplrx_factor. */
 int dplrx_batch_solve(dplrx_batch *batch, int nrhs, const double *const *dB, const int *ldb,
                       double *const *dX, const int *ldx, void *stream);
+/* Autotuner: times the batch's apply + solve (nrhs columns, the caller's
+ * buffers X and Y, `reps` repetitions after a warm-up) for each candidate
+ * tile shape (rows per reduction task: 128, 256, 512, 1024) and keeps the
+ * fastest for later calls.  *tile receives the choice; ms[0..3] (optional)
+ * the time per repetition of each candidate.  No reference counterpart:
+ * the reference has no batched path. */
+int dplrx_batch_tune(dplrx_batch *batch, int nrhs, const double *const *dX, const int *ldx,
+                     double *const *dY, const int *ldy, int reps, int *tile, double *ms,
+                     void *stream);
+/* Sets the tile shape (one of the tuner's candidates) without timing. */
+int dplrx_batch_set_tile(dplrx_batch *batch, int tile);
 
 const char *dplrx_last_error(void);
 

@@ -96,3 +96,48 @@ def test_batch_rebuilds_after_plan_workspace_changes(batch):
         back = torch.zeros_like(x)
         P.gepdmm("N", 1.0, z, 0.0, back)
         assert rel(back, x) < 1e-12, SPECS[k]
+
+
+@pytest.mark.parametrize("tile", [128, 512, 1024])
+def test_batched_tiles_match_single(batch, tile):
+    """Every tile shape the tuner can pick gives the single-plan results."""
+    import torch
+    trees, plans, B, X = batch
+    print(f"batched tile {tile}", flush=True)
+    for P in plans:
+        assert P.factor() == 0
+    B.set_tile(tile)
+    Y = [torch.zeros_like(x) for x in X]
+    Z = [torch.zeros_like(x) for x in X]
+    B.gepdmm(X, Y)
+    B.solve(X, Z)
+    torch.cuda.synchronize()
+    for k, (P, x) in enumerate(zip(plans, X)):
+        y1 = torch.zeros_like(x)
+        z1 = torch.zeros_like(x)
+        P.gepdmm("N", 1.0, x, 0.0, y1)
+        P.solve("N", x, z1)
+        assert rel(Y[k], y1) < 1e-13, (tile, SPECS[k])
+        assert rel(Z[k], z1) < 1e-12, (tile, SPECS[k])
+    B.set_tile(256)
+
+
+def test_batch_autotuner_picks_a_measured_tile(batch):
+    import torch
+    trees, plans, B, X = batch
+    for P in plans:
+        assert P.factor() == 0
+    Y = [torch.zeros_like(x) for x in X]
+    tile, ms = B.tune(X, Y, reps=2)
+    print("tuner", tile, ms, flush=True)
+    assert tile in D.Batch.TILES
+    assert all(v > 0 for v in ms.values())
+    assert ms[tile] == min(ms.values())
+    # the batch keeps working with the chosen tile
+    Z = [torch.zeros_like(x) for x in X]
+    B.solve(X, Z)
+    torch.cuda.synchronize()
+    for k, (P, x) in enumerate(zip(plans, X)):
+        z1 = torch.zeros_like(x)
+        P.solve("N", x, z1)
+        assert rel(Z[k], z1) < 1e-12, SPECS[k]

@@ -139,7 +139,8 @@ def test_solve_graph_replays_match_eager(cuda):
     P.close()
 
 
-@pytest.mark.parametrize("m,nlvl,rank,n", [(8192, 7, 32, 64), (16384, 8, 16, 70), (4096, 6, 32, 5)])
+@pytest.mark.parametrize("m,nlvl,rank,n", [(8192, 7, 32, 64), (16384, 8, 16, 70), (4096, 6, 32, 5),
+                                            (8192, 7, 32, 1)])
 def test_solve_upper_levels_fused(cuda, orc, m, nlvl, rank, n):
     """Leaf 64, N a multiple of 256, ranks 16 or 32: the bottom kernel, then one
     fused pass per level above it (capacitance rows, expansion, the next
