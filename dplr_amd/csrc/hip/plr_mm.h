@@ -354,6 +354,178 @@ __device__ __forceinline__ void mm_reduce_fused_body(const MmTab &a, int rbk, in
     }
 }
 
+// All-level reduction, second form (rank-major output only): a work-group
+// of 8 waves owns 256 rows x 64 columns; X of its rows is staged in LDS once
+// (one barrier), then every wave streams the P rows of its 16-rank tiles
+// (tiles w, w+8, w+16 of all levels: 20 at rank 32 x 10 levels, more in
+// further passes) with the next quarter-block's loads in flight, and never
+// waits on another wave.  A wave runs only its valid tiles (the pass is
+// specialised on their number), so waves w and w+4, which share a SIMD,
+// together carry 5 of 20 tiles.  Ranks past R in a tile read a duplicate of
+// row R-1: they only feed output rows k >= R, which are never stored.
+// Every level's segment length must be 64 or 128 or a multiple of 256, and
+// the task tile 256 rows where it is longer.
+static constexpr int MMF2_NW = 8, MMF2_TPW = 3, MMF2_XS = 72;
+
+template <int NT>
+__device__ __forceinline__ void mm_fused2_pass(const MmTab &a, int rb0, int cg, int t0, int wv,
+                                               double alpha, double *__restrict__ part,
+                                               double *__restrict__ w, size_t wstride,
+                                               const double *xs)
+{
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const int c0 = cg * MM_CG, N = a.N;
+    int td[NT], tk[NT];
+    const double *pa[NT];
+    d4 acc[NT][4];
+#pragma unroll
+    for (int j = 0; j < NT; ++j) {
+        const int t = t0 + wv + MMF2_NW * j;
+        int d = 0, cum = 0;
+        while (t >= cum + (a.R[d] + 15) / 16) cum += (a.R[d++] + 15) / 16;
+        td[j] = d;
+        tk[j] = (t - cum) * 16;
+        pa[j] = a.P[d] + (size_t)min(tk[j] + m, a.R[d] - 1) * N;
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[j][ct] = d4{0.0, 0.0, 0.0, 0.0};
+    }
+    double2 pA[NT][2], pB[NT][2];
+    // quarter-block k: rows rb0 + 16k + 8h + 2g + {0,1} (h = 0, 1): the four
+    // lane groups of one load read 64 contiguous bytes of a rank row, the
+    // two loads its whole 128-byte line
+    auto pload = [&](double2 (*pv)[2], int k) {
+        const int r = rb0 + 16 * k + 2 * g;
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) pv[j][h] = *(const double2 *)(pa[j] + r + 8 * h);
+    };
+    auto compute = [&](double2 (*pv)[2], int k) {
+#pragma unroll
+        for (int tt = 0; tt < 4; ++tt) {
+            // step tt: k-index g is row 16k + 8(tt/2) + 2g + tt%2
+            const int rl = 16 * k + 8 * (tt >> 1) + 2 * g + (tt & 1);
+            double bq[4];
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) bq[ct] = xs[rl * MMF2_XS + 16 * ct + m];
+#pragma unroll
+            for (int j = 0; j < NT; ++j) {
+                const double av = (tt & 1) ? pv[j][tt >> 1].y : pv[j][tt >> 1].x;
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) acc[j][ct] = mfma64(av, bq[ct], acc[j][ct]);
+            }
+        }
+    };
+    // tile j's output for the segment ending at row rend of its level
+    auto flush = [&](int j, int rend) {
+        const int d = td[j], R = a.R[d], seg = a.seglen[d];
+        const int s = (rend - 1) / seg;
+        const int tps = a.tps[d], T = a.T[d];
+        if (tps == 1) {
+            // stores only: a load here would drain the prefetch in flight
+            const int sp = s ^ 1;
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+                const int c = c0 + ct * 16 + m;
+                if (c >= a.ncols) continue;
+                double *wc = w + ((size_t)a.w_off[d] + (size_t)sp * R) * wstride + c;
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int k = tk[j] + g + 4 * v;
+                    if (k < R) wc[(size_t)k * wstride] = alpha * acc[j][ct][v];
+                }
+            }
+        } else {
+            const int tile = (rend - T - s * seg) / T;
+            const int st = s * tps + tile, kg = tk[j] / MM_KG;
+            double *pp = part + ((size_t)a.part_off[d] + ((size_t)st * a.nkg[d] + kg) * a.ncg + cg) * MM_BLK;
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    pp[((tk[j] % MM_KG) + g + 4 * v) * MM_CG + ct * 16 + m] = acc[j][ct][v];
+        }
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[j][ct] = d4{0.0, 0.0, 0.0, 0.0};
+    };
+    auto segment_ends = [&](int rend) {
+#pragma unroll
+        for (int j = 0; j < NT; ++j)
+            if ((rend - rb0) % min(a.seglen[td[j]], MMF_RB) == 0) flush(j, rend);
+    };
+    // quarter-blocks 0..15: the next one's loads in flight while the
+    // current one is used (the two buffers swap roles, nothing is copied;
+    // no load is conditional, so no wait covers more than the one buffer)
+    pload(pA, 0);
+#pragma unroll 1
+    for (int k = 0; k < 12; k += 4) {
+        pload(pB, k + 1);
+        compute(pA, k);
+        pload(pA, k + 2);
+        compute(pB, k + 1);
+        pload(pB, k + 3);
+        compute(pA, k + 2);
+        pload(pA, k + 4);
+        compute(pB, k + 3);
+        segment_ends(rb0 + 16 * (k + 4));
+    }
+    pload(pB, 13);
+    compute(pA, 12);
+    pload(pA, 14);
+    compute(pB, 13);
+    pload(pB, 15);
+    compute(pA, 14);
+    compute(pB, 15);
+    segment_ends(rb0 + MMF_RB);
+}
+
+__device__ __forceinline__ void mm_reduce_fused2_body(const MmTab &a, int rbk, int cg,
+                                                      const double *__restrict__ X, int ldx,
+                                                      double alpha, double *__restrict__ part,
+                                                      double *__restrict__ w, size_t wstride,
+                                                      double *xs)
+{
+    // wave-uniform (scalar) so that the per-tile table reads are scalar
+    // loads: a vector load there would wait for the P loads in flight
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rb0 = rbk * MMF_RB, c0 = cg * MM_CG;
+    int ntile = 0;
+    for (int d = 0; d < a.L; ++d) ntile += (a.R[d] + 15) / 16;
+    // X of all 256 rows, row-major in LDS (xs[row * 72 + col]: the B reads
+    // of one half-wave, 16 columns x 2 rows 2 apart, hit 64 distinct banks),
+    // columns c0.. (zero past ncols), then one barrier.  Element pair e:
+    // rows 2pr, 2pr+1 of column c, pr = 4*(e>>8) + (e&3), c = 16*((e>>6)&3)
+    // + ((e>>2)&15): four lanes read 64 contiguous bytes of one column.
+    for (int e0 = 0; e0 < 64 * (MMF_RB / 2); e0 += 8 * 64 * MMF2_NW) {
+        double2 xr[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 64 * MMF2_NW + (int)threadIdx.x;
+            const int pr = 4 * (e >> 8) + (e & 3), c = 16 * ((e >> 6) & 3) + ((e >> 2) & 15);
+            xr[u] = *(const double2 *)(X + (size_t)min(c0 + c, a.ncols - 1) * ldx + rb0 + 2 * pr);
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 64 * MMF2_NW + (int)threadIdx.x;
+            const int pr = 4 * (e >> 8) + (e & 3), c = 16 * ((e >> 6) & 3) + ((e >> 2) & 15);
+            const double on = c0 + c < a.ncols ? 1.0 : 0.0;
+            xs[(2 * pr) * MMF2_XS + c] = xr[u].x * on;
+            xs[(2 * pr + 1) * MMF2_XS + c] = xr[u].y * on;
+        }
+    }
+    __syncthreads();
+    for (int t0 = 0; t0 < ntile; t0 += MMF2_NW * MMF2_TPW) {
+        const int rest = ntile - t0 - wv; // tiles t0 + wv + 8j exist for 8j < rest
+        const int nt = rest <= 0 ? 0 : min(MMF2_TPW, (rest + MMF2_NW - 1) / MMF2_NW);
+        if (nt == 3)
+            mm_fused2_pass<3>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs);
+        else if (nt == 2)
+            mm_fused2_pass<2>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs);
+        else if (nt == 1)
+            mm_fused2_pass<1>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs);
+    }
+}
+
 // Sum the partial blocks of every segment (fixed order) into the
 // coefficients.  One work-group per (level, segment, rank group, column
 // group, 64-element chunk of the block): its four waves sum every fourth

@@ -38,6 +38,15 @@ __global__ __launch_bounds__(256) void k_mm_reduce_fused(MmTab a, const double *
     mm_reduce_fused_body<WT>(a, blockIdx.x, blockIdx.y, X, ldx, alpha, trans, raw, part, w, wstride);
 }
 
+__global__ __launch_bounds__(64 * MMF2_NW) void k_mm_reduce_fused2(MmTab a, const double *__restrict__ X,
+                                                                int ldx, double alpha,
+                                                                double *__restrict__ part,
+                                                                double *__restrict__ w, size_t wstride)
+{
+    extern __shared__ double xs_dyn[];
+    mm_reduce_fused2_body(a, blockIdx.x, blockIdx.y, X, ldx, alpha, part, w, wstride, xs_dyn);
+}
+
 template <bool WT>
 __global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int trans, int raw,
                                                    const double *__restrict__ part,
@@ -121,14 +130,24 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
     if (rc) return rc;
     // the fused path keeps the coefficients rank-major (w holds soff[L] rows
     // of t.ncg * 64 columns: the caller sizes it for whole 64-column groups)
-    const bool fused = p->N % MMF_RB == 0 && ldx % 2 == 0 && !p->force_valu && w == p->w &&
-                       t.ncg * MM_CG <= p->wk_cols;
+    // and needs leaves of a power of two >= 64 rows (whole 64-row steps in
+    // every segment, no leaf across a 128-row expansion block)
+    const bool leaves_ok = p->L == 0 || (p->b >= 64 && (p->b & (p->b - 1)) == 0);
+    const bool fused = p->N % MMF_RB == 0 && leaves_ok && ldx % 2 == 0 && !p->force_valu &&
+                       w == p->w && t.ncg * MM_CG <= p->wk_cols;
     if (!fused && (C || bmax)) return 1;
     if (fused) {
         const int wtc = t.ncg * MM_CG;
         if (t.task_off[t.L] > 0) {
-            hipLaunchKernelGGL(k_mm_reduce_fused<true>, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF_NW),
-                               0, st, t, X, ldx, alpha, trans, 0, p->tmp, w, (size_t)wtc);
+            static bool attr2 = false;
+            if (!attr2) {
+                PLR_HIP(hipFuncSetAttribute((const void *)k_mm_reduce_fused2,
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+                attr2 = true;
+            }
+            hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF2_NW),
+                               (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, alpha, p->tmp,
+                               w, (size_t)wtc);
             PLR_HIP(hipGetLastError());
         }
         if (t.fin_off[t.L] > 0) {

@@ -119,6 +119,9 @@ def test_full_size_matvec(cuda, orc):
     (4096, 5, 24, 16),    # leaf 128
     (2048, 3, 40, 33),    # leaf 256, rank not a multiple of 16
     (8192, 7, 20, 64),
+    (8192, 7, 64, 40),    # rank 64: 28 rank tiles, two passes over the staged X
+    (2048, 6, 16, 8),     # leaf 32: the general path
+    (3072, 5, 16, 8),     # leaf 96: the general path
 ])
 @pytest.mark.parametrize("trans", ["N", "T"])
 def test_multi_rhs_product_fused_path(cuda, orc, m, nlvl, rank, n, trans):
