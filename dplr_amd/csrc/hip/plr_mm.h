@@ -699,25 +699,34 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
 // Y^T (64 columns x 32 rows per wavefront) = beta Y^T + alpha X_leaf^T op(D)^T
 // + sum_d W_d^T (sig .* Q_d^T), with W rank-major (as written by the WT
 // reduction): w[(w_off[d] + s*R + k) * wtc + c].  Four waves per work-group
-// cover 128 rows; X of the leaves is staged in LDS (64 leaf columns per
-// stage, the only barriers), then every wave runs on its own: W^T comes
-// straight from global memory (16 consecutive columns per lane group), the
-// ranks of all levels in chunks of 16, the next chunk's loads in flight
-// while the current one is multiplied (two register buffers, no copies).
+// cover 128 rows and never wait on each other: the leaf columns (16 per
+// chunk, A = X^T, B = op(D)^T, the sum scaled by alpha) and the ranks of all levels
+// (16 per chunk, A = W^T, B = sig .* Q^T) form one stream of chunks, the
+// next chunk's loads in flight while the current one is multiplied (two
+// register buffers, no copies).  Every load is 16 bytes per lane: a lane
+// holds two adjacent columns (A tiles ct = 2cp, 2cp+1 are the even and odd
+// columns of a 32-column half) and two adjacent rows (row tiles rt = 0, 1
+// are the even and odd rows of the wave's 32), or for X of the leaf two
+// adjacent leaf columns j (MFMA steps u = 2h, 2h+1 take j = 8h + 2g + {0,1}).
 static constexpr int ME_RB = 128;
 
 // The beta term reads C (which may be Y itself); with bmax != null the
 // largest |Y| of every (column, 128-row block) goes to bmax[c * (N/128) + bx].
+// X, W, Q and D must be 16-byte aligned with even leading dimensions.
+template <int TR>
 __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
                                                 const double *__restrict__ D,
                                                 const double *__restrict__ X, int ldx,
-                                                double alpha, double beta, int trans,
+                                                double alpha, double beta,
                                                 const double *__restrict__ w, int wtc,
                                                 const double *C, int ldc, double *Y, int ldy,
-                                                double *__restrict__ bmax, double *xs)
+                                                double *__restrict__ bmax)
 {
+    __shared__ double cmx[4 * MM_CG];
     const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
-    const int wv = threadIdx.x >> 6;
+    // wave-uniform, so that table reads are scalar loads (a vector load
+    // would wait for the loads in flight)
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rbk = bx * ME_RB, r0 = rbk + wv * 32;
     const int c0 = by * MM_CG;
     const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
@@ -727,110 +736,123 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = d4{0.0, 0.0, 0.0, 0.0};
 
-    if (D) {
-        // leaves of the work-group: one (b >= 128) or two (b = 64)
-        const int lf0 = rbk / b, nl = b >= ME_RB ? 1 : ME_RB / b;
-        const int li = r0 / b - lf0, il = r0 - (r0 / b) * b;
-        const size_t ds_i = trans ? b : 1, ds_j = trans ? 1 : b; // element (row, col) of op(D)
-        const double *Dl = D + (size_t)(r0 / b) * b * b;
-        for (int j0 = 0; j0 < b; j0 += 64) {
-            double bq[16][2];
-#pragma unroll
-            for (int u = 0; u < 16; ++u)
-#pragma unroll
-                for (int rt = 0; rt < 2; ++rt)
-                    bq[u][rt] = Dl[(size_t)(il + 16 * rt + m) * ds_i + (size_t)(j0 + 4 * u + g) * ds_j];
-            __syncthreads(); // the previous stage is consumed
-            // xs[l][c][jj] = X[c0 + c][(lf0 + l) * b + j0 + jj]
-            for (int e0 = 0; e0 < nl * 64 * 64; e0 += 8 * 256) {
-                double v[8];
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int e = min(e0 + u * 256 + (int)threadIdx.x, nl * 64 * 64 - 1);
-                    const int l = e >> 12, c = (e >> 6) & 63, jj = e & 63;
-                    v[u] = X[(size_t)(c0 + min(c, nc - 1)) * ldx + (size_t)(lf0 + l) * b + j0 + jj] *
-                           (double)(c < nc);
-                }
-#pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    const int e = e0 + u * 256 + (int)threadIdx.x;
-                    if (e < nl * 64 * 64) xs[((e >> 12) * 64 + ((e >> 6) & 63)) * 65 + (e & 63)] = v[u];
-                }
-            }
-            __syncthreads();
-            const double *xl = xs + li * 64 * 65;
-#pragma unroll
-            for (int u = 0; u < 16; ++u) {
-                double av[4];
-#pragma unroll
-                for (int ct = 0; ct < 4; ++ct) av[ct] = xl[(16 * ct + m) * 65 + 4 * u + g];
-#pragma unroll
-                for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-                    for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = mfma64(av[ct], bq[u][rt], acc[rt][ct]);
-            }
-        }
-#pragma unroll
-        for (int rt = 0; rt < 2; ++rt)
-#pragma unroll
-            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] *= alpha;
-    }
-
-    // levels: chunks of 16 ranks over all levels
+    // chunks: b/16 of the leaf (when D is given), then 16 ranks at a time
+    // over all levels
+    const int nlc = D ? b / 16 : 0;
     int nch = 0;
     for (int d = 0; d < a.L; ++d) nch += (a.R[d] + 15) / 16;
-    if (nch > 0) {
+    const int ntot = nlc + nch;
+    if (ntot > 0) {
         struct Buf
         {
-            double wa[4][4]; // W^T[c0 + 16ct + m][k0 + 4u + g]
-            double qb[4][2]; // Q[k0 + 4u + g][r0 + 16rt + m]
-            double sg[4];    // sig of rank k0 + 4u + g
+            // level: A[2u + cp] = W^T[c0 + 32cp + 2m + {0,1}][k0 + 4u + g]
+            // leaf:  A[4h + q]  = X^T[c0 + 32(q>>1) + 2m + (q&1)][8h + 2g + {0,1}]
+            double2 A[8];
+            // level, leaf (TR = 0): Bq[u] = B[k(u)][r0 + 2m + {0,1}]
+            // leaf (TR = 1): Bq[2rt + h] = op(D)[r0 + 2m + rt][j0 + 8h + 2g + {0,1}]
+            double2 Bq[4];
+            double sg[4];    // sig of rank k0 + 4u + g (level chunks)
             int kn;          // real ranks in the chunk (0 past the last chunk)
+            bool lf;         // leaf chunk
         };
-        // every load unconditional (clamped indices), masks computed, not loaded
+        const int leaf = r0 / b, il = r0 - leaf * b;
+        const double *Dl = D ? D + (size_t)leaf * b * b : nullptr;
+        const double *Xl = X + (size_t)leaf * b;
+        size_t xcol[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            xcol[q] = (size_t)(c0 + min(32 * (q >> 1) + 2 * m + (q & 1), nc - 1)) * ldx;
+        // both kinds of chunk load through the same 16 instructions, with
+        // addresses selected per chunk (a branch around the loads would make
+        // every wait cover the loads in flight)
         auto load = [&](Buf &B, int c) {
-            const int cc = min(c, nch - 1);
+            const int cc = min(c, ntot - 1);
+            const bool lf = cc < nlc;
+            // level chunk (a valid one stands in while the leaf is loaded)
+            const int lc = lf ? 0 : cc - nlc;
             int d = 0, cum = 0;
-            while (cc >= cum + (a.R[d] + 15) / 16) cum += (a.R[d++] + 15) / 16;
-            const int R = a.R[d], k0 = (cc - cum) * 16, s = r0 / a.seglen[d];
-            const double *Q = a.Q[d] + r0 + m;
-            const double *sg = a.S[d] + (size_t)(trans ? (s ^ 1) : s) * R;
-            const double *wr = w + ((size_t)a.w_off[d] + (size_t)s * R) * wtc + c0 + m;
-            B.kn = c < nch ? R - k0 : 0;
+            if (nch > 0)
+                while (lc >= cum + (a.R[d] + 15) / 16) cum += (a.R[d++] + 15) / 16;
+            const int R = a.R[d], k0 = (lc - cum) * 16, s = r0 / a.seglen[d];
+            const double *Wb = w + ((size_t)a.w_off[d] + (size_t)s * R) * wtc + c0 + 2 * m;
+            const double *Qb = a.Q[d] + r0 + 2 * m;
+            const double *Sb = lf ? Xl : a.S[d] + (size_t)(TR ? (s ^ 1) : s) * R;
+            const int j0 = 16 * cc;
+            B.kn = c >= ntot ? 0 : lf ? 16 : R - k0;
+            B.lf = lf;
+            int kc[4];
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const int k = k0 + 4 * u + g, kc = min(k, R - 1);
-                B.sg[u] = sg[kc];
+            for (int u = 0; u < 4; ++u) kc[u] = min(k0 + 4 * u + g, R - 1);
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct) B.wa[u][ct] = wr[(size_t)kc * wtc + 16 * ct];
+            for (int i = 0; i < 8; ++i) {
+                const double *pl = Wb + (size_t)kc[i >> 1] * wtc + 32 * (i & 1);
+                const double *pf = Xl + xcol[i & 3] + j0 + 8 * (i >> 2) + 2 * g;
+                B.A[i] = *(const double2 *)(lf ? pf : pl);
+            }
 #pragma unroll
-                for (int rt = 0; rt < 2; ++rt) B.qb[u][rt] = Q[(size_t)kc * N + 16 * rt];
+            for (int i = 0; i < 4; ++i) {
+                const double *pl = Qb + (size_t)kc[i] * N;
+                const double *pf = TR ? Dl + (size_t)(il + 2 * m + (i >> 1)) * b + j0 + 8 * (i & 1) + 2 * g
+                                      : Dl + (size_t)(j0 + 8 * (i >> 1) + 2 * g + (i & 1)) * b + il + 2 * m;
+                B.Bq[i] = *(const double2 *)(lf ? pf : pl);
+                B.sg[i] = Sb[lf ? 0 : kc[i]];
             }
         };
         auto compute = [&](const Buf &B) {
+            if (B.lf) {
 #pragma unroll
-            for (int u = 0; u < 4; ++u) {
-                const double f = B.sg[u] * (double)(4 * u + g < B.kn);
+                for (int u = 0; u < 4; ++u) {
+                    double bv[2];
 #pragma unroll
-                for (int rt = 0; rt < 2; ++rt) {
-                    const double bv = B.qb[u][rt] * f;
+                    for (int rt = 0; rt < 2; ++rt) {
+                        const double2 q2 = TR ? B.Bq[2 * rt + (u >> 1)] : B.Bq[u];
+                        bv[rt] = TR ? ((u & 1) ? q2.y : q2.x) : (rt ? q2.y : q2.x);
+                    }
 #pragma unroll
-                    for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = mfma64(B.wa[u][ct], bv, acc[rt][ct]);
+                    for (int ct = 0; ct < 4; ++ct) {
+                        const double2 a2 = B.A[4 * (u >> 1) + ct];
+                        const double av = (u & 1) ? a2.y : a2.x;
+#pragma unroll
+                        for (int rt = 0; rt < 2; ++rt) acc[rt][ct] = mfma64(av, bv[rt], acc[rt][ct]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double f = 4 * u + g < B.kn ? B.sg[u] : 0.0;
+                    const double bv[2] = {B.Bq[u].x * f, B.Bq[u].y * f};
+#pragma unroll
+                    for (int ct = 0; ct < 4; ++ct) {
+                        const double2 a2 = B.A[2 * u + (ct >> 1)];
+                        const double av = (ct & 1) ? a2.y : a2.x;
+#pragma unroll
+                        for (int rt = 0; rt < 2; ++rt) acc[rt][ct] = mfma64(av, bv[rt], acc[rt][ct]);
+                    }
                 }
             }
+        };
+        // the leaf term (chunks 0 .. nlc-1, nlc even) is scaled by alpha
+        // once it is complete; W of the levels carries alpha already
+        auto scale = [&]() {
+#pragma unroll
+            for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) acc[rt][ct] *= alpha;
         };
         Buf bA, bB;
         load(bA, 0);
 #pragma unroll 1
-        for (int c = 0; c < nch; c += 2) {
+        for (int c = 0; c < ntot; c += 2) {
             load(bB, c + 1);
+            if (nlc > 0 && c == nlc) scale();
             compute(bA);
             load(bA, c + 2);
             compute(bB);
         }
+        if (nlc > 0 && nch == 0) scale();
     }
 
-    // lane holds Y^T[c0 + 16ct + g + 4v][r0 + 16rt + m]
+    // lane holds Y^T[c0 + 32(ct>>1) + 2(g + 4v) + (ct&1)][r0 + 2m + rt]
     double yo[2][4][4];
     if (beta != 0.0) {
 #pragma unroll
@@ -839,8 +861,8 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
             for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
-                    yo[rt][ct][v] = C[(size_t)c * ldc + r0 + 16 * rt + m];
+                    const int c = min(c0 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1), a.ncols - 1);
+                    yo[rt][ct][v] = C[(size_t)c * ldc + r0 + 2 * m + rt];
                 }
     }
 #pragma unroll
@@ -849,9 +871,9 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
         for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
             for (int v = 0; v < 4; ++v) {
-                const int c = c0 + ct * 16 + g + 4 * v;
+                const int c = c0 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1);
                 acc[rt][ct][v] += beta == 0.0 ? 0.0 : beta * yo[rt][ct][v];
-                if (c < a.ncols) Y[(size_t)c * ldy + r0 + 16 * rt + m] = acc[rt][ct][v];
+                if (c < a.ncols) Y[(size_t)c * ldy + r0 + 2 * m + rt] = acc[rt][ct][v];
             }
     if (bmax) {
         // column maxima: over the two row tiles, the 16 lanes of a group
@@ -866,16 +888,16 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
                 for (int o = 1; o < 16; o <<= 1) x = fmax(x, __shfl_xor(x, o, 64));
                 cm[ct][v] = x;
             }
-        __syncthreads(); // xs is free again
         if (m == 0)
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
-                for (int v = 0; v < 4; ++v) xs[wv * 64 + ct * 16 + g + 4 * v] = cm[ct][v];
+                for (int v = 0; v < 4; ++v)
+                    cmx[wv * 64 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1)] = cm[ct][v];
         __syncthreads();
         if (threadIdx.x < MM_CG && c0 + (int)threadIdx.x < a.ncols) {
             const int c = threadIdx.x;
-            const double x = fmax(fmax(xs[c], xs[64 + c]), fmax(xs[128 + c], xs[192 + c]));
+            const double x = fmax(fmax(cmx[c], cmx[64 + c]), fmax(cmx[128 + c], cmx[192 + c]));
             bmax[(size_t)(c0 + c) * (N / ME_RB) + bx] = x;
         }
     }

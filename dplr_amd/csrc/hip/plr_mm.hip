@@ -55,16 +55,16 @@ __global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int tr
     mm_finish_body<WT>(a, blockIdx.x, alpha, trans, raw, part, w, wstride);
 }
 
+template <int TR>
 __global__ __launch_bounds__(256, 2) void k_mm_expand2(MmTab a, const double *__restrict__ D,
                                                     const double *__restrict__ X, int ldx,
-                                                    double alpha, double beta, int trans,
+                                                    double alpha, double beta,
                                                     const double *__restrict__ w, int wtc,
                                                     const double *C, int ldc, double *Y, int ldy,
                                                     double *__restrict__ bmax)
 {
-    extern __shared__ double xs_dyn[];
-    mm_expand2_body(a, blockIdx.x, blockIdx.y, D, X, ldx, alpha, beta, trans, w, wtc, C, ldc, Y,
-                    ldy, bmax, xs_dyn);
+    mm_expand2_body<TR>(a, blockIdx.x, blockIdx.y, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy,
+                        bmax);
 }
 
 __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__restrict__ D,
@@ -133,8 +133,9 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
     // and needs leaves of a power of two >= 64 rows (whole 64-row steps in
     // every segment, no leaf across a 128-row expansion block)
     const bool leaves_ok = p->L == 0 || (p->b >= 64 && (p->b & (p->b - 1)) == 0);
-    const bool fused = p->N % MMF_RB == 0 && leaves_ok && ldx % 2 == 0 && !p->force_valu &&
-                       w == p->w && t.ncg * MM_CG <= p->wk_cols;
+    const bool fused = p->N % MMF_RB == 0 && leaves_ok && ldx % 2 == 0 &&
+                       ((uintptr_t)X & 15) == 0 && !p->force_valu && w == p->w &&
+                       t.ncg * MM_CG <= p->wk_cols;
     if (!fused && (C || bmax)) return 1;
     if (fused) {
         const int wtc = t.ncg * MM_CG;
@@ -155,16 +156,9 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
                                alpha, trans, 0, p->tmp, w, (size_t)wtc);
             PLR_HIP(hipGetLastError());
         }
-        const int nl = p->b >= ME_RB ? 1 : ME_RB / p->b;
-        const size_t lds = (size_t)nl * 64 * 65 * sizeof(double);
-        static bool attr = false;
-        if (!attr) {
-            PLR_HIP(hipFuncSetAttribute((const void *)k_mm_expand2,
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
-            attr = true;
-        }
-        hipLaunchKernelGGL(k_mm_expand2, dim3(p->N / ME_RB, t.ncg), dim3(256), lds, st, t, p->D, X,
-                           ldx, alpha, beta, trans, w, wtc, C ? C : Y, C ? ldc : ldy, Y, ldy, bmax);
+        hipLaunchKernelGGL(trans ? k_mm_expand2<1> : k_mm_expand2<0>, dim3(p->N / ME_RB, t.ncg),
+                           dim3(256), 0, st, t, p->D, X, ldx, alpha, beta, w, wtc, C ? C : Y,
+                           C ? ldc : ldy, Y, ldy, bmax);
         PLR_HIP(hipGetLastError());
         return 0;
     }
