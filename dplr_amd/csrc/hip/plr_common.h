@@ -43,6 +43,11 @@ struct plr_fact
     size_t koff[PLR_MAXL + 1];     // element offset of level d in Kinv
     int *dinfo;                    // per-matrix LU status
     double *work;                  // LU scratch (leaves, capacitances)
+    // reassociated upper solve: G_{d,e}[s] = V_d|_s^T Ut_e|_s for d < e < up_d0,
+    // per segment s of level e, R_d x R_e column-major at goff[d][e] + s*R_d*R_e
+    double *G;
+    int up_d0;
+    size_t goff[PLR_MAXL][PLR_MAXL];
 };
 
 // Workspace of the per-level solve kernels, sized for `cols` columns.
@@ -53,6 +58,10 @@ struct plr_solve_ws
     double *T, *W, *part, *tmp;
     int *cnt;
     size_t tmp_cap;
+    // reassociated upper solve: T0 and W (rows soff[d] + s*R + k), the
+    // ping-pong subtree sums A, all with row stride ucols (64-column groups)
+    int ucols;
+    double *uT0, *uW, *uA[2];
 };
 
 struct dplrh_plan
@@ -164,6 +173,14 @@ int plr_mm_part(dplrh_plan *p, size_t blocks);
 int plr_mm_finish_raw(dplrh_plan *p, int d, int ncols, int tmax, double *T, size_t tstride,
                       hipStream_t st);
 size_t plr_mm_part_blocks(const dplrh_plan *p, int d0, int d1, int ncols, int tmax);
+int plr_mm_reduce_raw(dplrh_plan *p, int d1, const double *X, int ldx, int ncols, double *T,
+                      size_t ws, hipStream_t st);
+int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W, size_t ws,
+                    double *Y, int ldy, int ncols, hipStream_t st);
+// Reassociated upper levels of the FP64 solve (plr_up.hip).
+int plr_up_factor(dplrh_plan *p, hipStream_t st);
+int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream_t st);
+int plr_tail_d0(const dplrh_plan *p);
 int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,
                    const double *D, int trans, const double *X, int ldx, double alpha, double beta,
                    double *Y, int ldy, int ncols, hipStream_t st);

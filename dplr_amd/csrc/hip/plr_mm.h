@@ -27,6 +27,8 @@ struct MmTab
     int w_off[PLR_MAXL + 1];
     int fin_off[PLR_MAXL + 1];  // finishing tasks (segments x groups of levels with tps > 1)
     int sx; // the reduction stores alpha * t; the expansion multiplies by sig when staging
+    int raw;   // second-form reduction: T of the segment itself, unscaled
+    int nosig; // second-form expansion: no sig factor (W is complete)
     const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
 };
 
@@ -354,7 +356,9 @@ __device__ __forceinline__ void mm_reduce_fused_body(const MmTab &a, int rbk, in
     }
 }
 
-// All-level reduction, second form (rank-major output only): a work-group
+// All-level reduction, second form (rank-major output only; with a.raw the
+// coefficients of segment s are its own unscaled T, else alpha T goes to
+// the partner segment s^1 as the expansion needs it): a work-group
 // of 8 waves owns 256 rows x 64 columns; X of its rows is staged in LDS once
 // (one barrier), then every wave streams the P rows of its 16-rank tiles
 // (tiles w, w+8, w+16 of all levels: 20 at rank 32 x 10 levels, more in
@@ -423,7 +427,8 @@ __device__ __forceinline__ void mm_fused2_pass(const MmTab &a, int rb0, int cg, 
         const int tps = a.tps[d], T = a.T[d];
         if (tps == 1) {
             // stores only: a load here would drain the prefetch in flight
-            const int sp = s ^ 1;
+            const int sp = a.raw ? s : s ^ 1;
+            const double f = a.raw ? 1.0 : alpha;
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
                 const int c = c0 + ct * 16 + m;
@@ -432,7 +437,7 @@ __device__ __forceinline__ void mm_fused2_pass(const MmTab &a, int rb0, int cg, 
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
                     const int k = tk[j] + g + 4 * v;
-                    if (k < R) wc[(size_t)k * wstride] = alpha * acc[j][ct][v];
+                    if (k < R) wc[(size_t)k * wstride] = f * acc[j][ct][v];
                 }
             }
         } else {
@@ -697,8 +702,8 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
 
 
 // Y^T (64 columns x 32 rows per wavefront) = beta Y^T + alpha X_leaf^T op(D)^T
-// + sum_d W_d^T (sig .* Q_d^T), with W rank-major (as written by the WT
-// reduction): w[(w_off[d] + s*R + k) * wtc + c].  Four waves per work-group
+// + sum_d W_d^T (sig .* Q_d^T) (sig = 1 with a.nosig), with W rank-major (as
+// written by the WT reduction): w[(w_off[d] + s*R + k) * wtc + c].  Four waves per work-group
 // cover 128 rows and never wait on each other: the leaf columns (16 per
 // chunk, A = X^T, B = op(D)^T, the sum scaled by alpha) and the ranks of all levels
 // (16 per chunk, A = W^T, B = sig .* Q^T) form one stream of chunks, the
@@ -776,7 +781,7 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
             const int R = a.R[d], k0 = (lc - cum) * 16, s = r0 / a.seglen[d];
             const double *Wb = w + ((size_t)a.w_off[d] + (size_t)s * R) * wtc + c0 + 2 * m;
             const double *Qb = a.Q[d] + r0 + 2 * m;
-            const double *Sb = lf ? Xl : a.S[d] + (size_t)(TR ? (s ^ 1) : s) * R;
+            const double *Sb = lf || a.nosig ? Xl : a.S[d] + (size_t)(TR ? (s ^ 1) : s) * R;
             const int j0 = 16 * cc;
             B.kn = c >= ntot ? 0 : lf ? 16 : R - k0;
             B.lf = lf;
@@ -819,7 +824,7 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
             } else {
 #pragma unroll
                 for (int u = 0; u < 4; ++u) {
-                    const double f = 4 * u + g < B.kn ? B.sg[u] : 0.0;
+                    const double f = 4 * u + g < B.kn ? (a.nosig ? 1.0 : B.sg[u]) : 0.0;
                     const double bv[2] = {B.Bq[u].x * f, B.Bq[u].y * f};
 #pragma unroll
                     for (int ct = 0; ct < 4; ++ct) {

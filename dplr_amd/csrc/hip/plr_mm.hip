@@ -225,6 +225,71 @@ int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_
     return 0;
 }
 
+// T0 of levels [0, d1) over X for the reassociated upper solve:
+// T[(soff[d] + s*R_d + k) * ws + c] = V_d|_s^T X|_s (raw), through the
+// second-form reduction and its finishing sums.  Returns 1, launching
+// nothing, when X or the shape does not suit that kernel.
+int plr_mm_reduce_raw(dplrh_plan *p, int d1, const double *X, int ldx, int ncols, double *T,
+                      size_t ws, hipStream_t st)
+{
+    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+    for (int d = 0; d < d1; ++d) {
+        P[d] = p->V + p->uoff[d];
+        Q[d] = p->U + p->uoff[d];
+        S[d] = p->S + p->soff[d];
+    }
+    MmTab t;
+    mm_tab(p, 0, d1, P, Q, S, ncols, 0, &t);
+    t.raw = 1;
+    const bool leaves_ok = p->b >= 64 && (p->b & (p->b - 1)) == 0;
+    if (p->N % MMF_RB != 0 || !leaves_ok || ldx % 2 != 0 || ((uintptr_t)X & 15) != 0 ||
+        (size_t)t.ncg * MM_CG > ws)
+        return 1;
+    int rc = mm_part(p, (size_t)t.part_off[t.L]);
+    if (rc) return rc;
+    if (t.task_off[t.L] > 0) {
+        static bool attr2 = false;
+        if (!attr2) {
+            PLR_HIP(hipFuncSetAttribute((const void *)k_mm_reduce_fused2,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            attr2 = true;
+        }
+        hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF2_NW),
+                           (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, 1.0, p->tmp, T,
+                           ws);
+        PLR_HIP(hipGetLastError());
+    }
+    if (t.fin_off[t.L] > 0) {
+        hipLaunchKernelGGL(k_mm_finish<true>, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t, 1.0,
+                           0, 1, p->tmp, T, ws);
+        PLR_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// Y += sum_{d < d1} Q_d|_s W_d[s] with W rank-major (rows soff[d] + s*R_d + k,
+// stride ws) and no sig factor: the expansion of the reassociated upper
+// solve (Q = Ut).  Y must be 16-byte aligned with an even ldy.
+int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W, size_t ws,
+                    double *Y, int ldy, int ncols, hipStream_t st)
+{
+    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+    for (int d = 0; d < d1; ++d) {
+        P[d] = p->V + p->uoff[d];
+        Q[d] = Qbase + p->uoff[d];
+        S[d] = p->S + p->soff[d];
+    }
+    MmTab t;
+    mm_tab(p, 0, d1, P, Q, S, ncols, 0, &t);
+    t.nosig = 1;
+    if (p->N % ME_RB != 0 || (size_t)t.ncg * MM_CG > ws) return 1;
+    hipLaunchKernelGGL(k_mm_expand2<0>, dim3(p->N / ME_RB, t.ncg), dim3(256), 0, st, t,
+                       (const double *)nullptr, (const double *)Y, ldy, 1.0, 1.0, W, (int)ws,
+                       (const double *)Y, ldy, Y, ldy, (double *)nullptr);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
 // T[c][s*R_d + k] = sum of the raw partial blocks in p->tmp that a caller
 // wrote in mm_reduce_body's layout with tiles of `tmax` rows (level d,
 // ncols columns).

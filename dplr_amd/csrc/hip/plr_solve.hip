@@ -298,8 +298,9 @@ static size_t col_off(const dplrh_plan *p, int d) { return p->N ? p->uoff[d] / p
 void plr_free_fact(dplrh_plan *p)
 {
     ++p->ws_gen;
-    void *bufs[] = {p->f.Dinv, p->f.Ut, p->f.Kinv, p->f.dinfo, p->f.work,
-                    p->sw.T, p->sw.W, p->sw.part, p->sw.tmp, p->sw.cnt};
+    void *bufs[] = {p->f.Dinv, p->f.Ut, p->f.Kinv, p->f.dinfo, p->f.work, p->f.G,
+                    p->sw.T, p->sw.W, p->sw.part, p->sw.tmp, p->sw.cnt,
+                    p->sw.uT0, p->sw.uW, p->sw.uA[0], p->sw.uA[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
     memset(&p->f, 0, sizeof(p->f));
@@ -485,6 +486,11 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
                                        p->sw.t_per_col, f.Ut, N, off, st);
             if (rc) return rc;
         }
+    }
+    // the cross-level blocks of the reassociated upper solve
+    if (fails.empty()) {
+        const int rc = plr_up_factor(p, st);
+        if (rc) return rc;
     }
     PLR_HIP(hipStreamSynchronize(st));
     if (!fails.empty()) {

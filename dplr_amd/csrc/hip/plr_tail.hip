@@ -409,6 +409,13 @@ int plr_solve_tail(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx
 // sums.  Returns 1, launching nothing, when the shape is not covered (leaf
 // 64, N a multiple of 256, ranks above the tail multiples of 16 and at most
 // 32, the workspace T sized by the caller for ncols).
+int plr_tail_d0(const dplrh_plan *p)
+{
+    int d0 = 0;
+    while ((p->N >> d0) > TL_MAXSPAN) ++d0;
+    return d0;
+}
+
 int plr_solve_fast(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
                    hipStream_t st)
 {
@@ -423,6 +430,9 @@ int plr_solve_fast(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx
     if (rc) return rc;
     rc = solve_tail_t(p, Bm, ldb, Xm, ldx, ncols, 1, st);
     if (rc < 0) return 1; // the tail does not fit: nothing launched
+    // the reassociated upper levels when the factorization prepared them
+    rc = plr_up_solve(p, d0, Xm, ldx, ncols, st);
+    if (rc <= 0) return rc;
     static bool attr = false;
     if (!attr) {
         PLR_HIP(hipFuncSetAttribute((const void *)k_fp_level,
