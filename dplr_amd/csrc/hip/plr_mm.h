@@ -908,6 +908,284 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
     }
 }
 
+// Expansion, third form: as mm_expand2_body on 256 rows x 64 columns per
+// work-group (eight waves of 32 rows), with W^T and sig of the levels whose
+// segments hold the whole work-group ("shared" levels, segments of 256 rows
+// or more, a prefix of the levels) staged in LDS once, behind the one
+// barrier, so that every wave takes them from LDS instead of each reading
+// them from global memory.  The stream then runs in three phases, each
+// with its own two register buffers: the leaf (A = X^T, B = op(D)^T, from
+// global memory), the shared levels (A from LDS, B = Q from global memory),
+// the other levels (as in the second form).  bmax covers the work-group's
+// two 128-row blocks.
+static constexpr int ME3_RB = 256, ME3_NW = 8, ME3_CS = 65;
+
+// LDS doubles of the third form for `rows` staged ranks (W^T rows + sig).
+__host__ __device__ inline size_t me3_lds(int rows) { return (size_t)rows * (ME3_CS + 1); }
+
+template <int TR>
+__device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
+                                                const double *__restrict__ D,
+                                                const double *__restrict__ X, int ldx,
+                                                double alpha, double beta,
+                                                const double *__restrict__ w, int wtc,
+                                                const double *C, int ldc, double *Y, int ldy,
+                                                double *__restrict__ bmax, double *lds)
+{
+    __shared__ double cmx[ME3_NW * MM_CG];
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int rb0 = bx * ME3_RB, r0 = rb0 + wv * 32;
+    const int c0 = by * MM_CG;
+    const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
+
+    // shared levels [0, dsh) and their staged rows
+    int dsh = 0, rows = 0;
+    while (dsh < a.L && a.seglen[dsh] >= ME3_RB) rows += a.R[dsh++];
+    double *Ws = lds;                    // [rows][ME3_CS]: W^T[c0 + c][k] at Ws[k * CS + c]
+    double *Ss = lds + (size_t)rows * ME3_CS; // sig of the staged ranks
+    for (int e = threadIdx.x; e < rows * 32; e += 64 * ME3_NW) {
+        const int row = e >> 5, cp = e & 31;
+        int d = 0, k = row;
+        while (k >= a.R[d]) k -= a.R[d++];
+        const int R = a.R[d], s = rb0 / a.seglen[d];
+        const double2 v = *(const double2 *)(w + ((size_t)a.w_off[d] + (size_t)s * R + k) * wtc + c0 + 2 * cp);
+        Ws[row * ME3_CS + 2 * cp] = v.x;
+        Ws[row * ME3_CS + 2 * cp + 1] = v.y;
+        if (cp == 0) Ss[row] = a.nosig ? 1.0 : a.S[d][(size_t)(TR ? (s ^ 1) : s) * R + k];
+    }
+    __syncthreads();
+
+    d4 acc[2][4];
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = d4{0.0, 0.0, 0.0, 0.0};
+
+    // phase 1: the leaf, b/16 chunks of 16 leaf columns j (steps u = 2h, 2h+1
+    // take j = 8h + 2g + {0,1}), A = X^T pairs, B = op(D)^T
+    if (D) {
+        struct LBuf
+        {
+            double2 A[8];  // A[4h + q] = X^T[c0 + 32(q>>1) + 2m + (q&1)][j0 + 8h + 2g + {0,1}]
+            double2 Bq[4]; // TR = 0: Bq[u] = op(D)[r0 + 2m + {0,1}][j(u)]; TR = 1: Bq[2rt + h]
+        };
+        const int leaf = r0 / b, il = r0 - leaf * b;
+        const double *Dl = D + (size_t)leaf * b * b;
+        const double *Xl = X + (size_t)leaf * b;
+        size_t xcol[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+            xcol[q] = (size_t)(c0 + min(32 * (q >> 1) + 2 * m + (q & 1), nc - 1)) * ldx;
+        const int nlc = b / 16;
+        auto load = [&](LBuf &B, int c) {
+            const int j0 = 16 * min(c, nlc - 1);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                B.A[i] = *(const double2 *)(Xl + xcol[i & 3] + j0 + 8 * (i >> 2) + 2 * g);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+                B.Bq[i] = *(const double2 *)(TR ? Dl + (size_t)(il + 2 * m + (i >> 1)) * b + j0 + 8 * (i & 1) + 2 * g
+                                                : Dl + (size_t)(j0 + 8 * (i >> 1) + 2 * g + (i & 1)) * b + il + 2 * m);
+        };
+        auto compute = [&](const LBuf &B, int c) {
+            if (c >= nlc) return;
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                double bv[2];
+#pragma unroll
+                for (int rt = 0; rt < 2; ++rt) {
+                    const double2 q2 = TR ? B.Bq[2 * rt + (u >> 1)] : B.Bq[u];
+                    bv[rt] = TR ? ((u & 1) ? q2.y : q2.x) : (rt ? q2.y : q2.x);
+                }
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) {
+                    const double2 a2 = B.A[4 * (u >> 1) + ct];
+                    const double av = (u & 1) ? a2.y : a2.x;
+#pragma unroll
+                    for (int rt = 0; rt < 2; ++rt) acc[rt][ct] = mfma64(av, bv[rt], acc[rt][ct]);
+                }
+            }
+        };
+        LBuf bA, bB;
+        load(bA, 0);
+#pragma unroll 1
+        for (int c = 0; c < nlc; c += 2) {
+            load(bB, c + 1);
+            compute(bA, c);
+            load(bA, c + 2);
+            compute(bB, c + 1);
+        }
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] *= alpha;
+    }
+
+    // phase 2: the shared levels, 16 ranks per chunk; A (W^T) and sig from LDS
+    int nsc = 0;
+    for (int d = 0; d < dsh; ++d) nsc += (a.R[d] + 15) / 16;
+    if (nsc > 0) {
+        struct SBuf
+        {
+            double2 Bq[4]; // Q[k(u)][r0 + 2m + {0,1}]
+            int row0, kn;  // first staged row of the chunk, real ranks in it
+        };
+        auto load = [&](SBuf &B, int c) {
+            const int cc = min(c, nsc - 1);
+            int d = 0, cum = 0, pre = 0;
+            while (cc >= cum + (a.R[d] + 15) / 16) {
+                cum += (a.R[d] + 15) / 16;
+                pre += a.R[d++];
+            }
+            const int R = a.R[d], k0 = (cc - cum) * 16;
+            B.row0 = pre + k0;
+            B.kn = c < nsc ? R - k0 : 0;
+            const double *Qb = a.Q[d] + r0 + 2 * m;
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                B.Bq[u] = *(const double2 *)(Qb + (size_t)min(k0 + 4 * u + g, R - 1) * N);
+        };
+        auto compute = [&](const SBuf &B) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int kk = 4 * u + g, row = B.row0 + min(kk, max(B.kn - 1, 0));
+                const double f = kk < B.kn ? Ss[row] : 0.0;
+                const double bv[2] = {B.Bq[u].x * f, B.Bq[u].y * f};
+                const double *wr = Ws + (size_t)row * ME3_CS + 2 * m;
+#pragma unroll
+                for (int cp = 0; cp < 2; ++cp) {
+                    const double a0 = wr[32 * cp], a1 = wr[32 * cp + 1];
+#pragma unroll
+                    for (int rt = 0; rt < 2; ++rt) {
+                        acc[rt][2 * cp] = mfma64(a0, bv[rt], acc[rt][2 * cp]);
+                        acc[rt][2 * cp + 1] = mfma64(a1, bv[rt], acc[rt][2 * cp + 1]);
+                    }
+                }
+            }
+        };
+        SBuf bA, bB;
+        load(bA, 0);
+#pragma unroll 1
+        for (int c = 0; c < nsc; c += 2) {
+            load(bB, c + 1);
+            compute(bA);
+            load(bA, c + 2);
+            compute(bB);
+        }
+    }
+
+    // phase 3: the other levels, as in the second form
+    int nch = 0;
+    for (int d = dsh; d < a.L; ++d) nch += (a.R[d] + 15) / 16;
+    if (nch > 0) {
+        struct Buf
+        {
+            double2 A[8];  // A[2u + cp] = W^T[c0 + 32cp + 2m + {0,1}][k0 + 4u + g]
+            double2 Bq[4]; // Q[k(u)][r0 + 2m + {0,1}]
+            double sg[4];
+            int kn;
+        };
+        auto load = [&](Buf &B, int c) {
+            const int cc = min(c, nch - 1);
+            int d = dsh, cum = 0;
+            while (cc >= cum + (a.R[d] + 15) / 16) cum += (a.R[d++] + 15) / 16;
+            const int R = a.R[d], k0 = (cc - cum) * 16, s = r0 / a.seglen[d];
+            const double *Wb = w + ((size_t)a.w_off[d] + (size_t)s * R) * wtc + c0 + 2 * m;
+            const double *Qb = a.Q[d] + r0 + 2 * m;
+            const double *Sb = a.nosig ? Qb : a.S[d] + (size_t)(TR ? (s ^ 1) : s) * R;
+            B.kn = c < nch ? R - k0 : 0;
+            int kc[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) kc[u] = min(k0 + 4 * u + g, R - 1);
+#pragma unroll
+            for (int i = 0; i < 8; ++i)
+                B.A[i] = *(const double2 *)(Wb + (size_t)kc[i >> 1] * wtc + 32 * (i & 1));
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                B.Bq[i] = *(const double2 *)(Qb + (size_t)kc[i] * N);
+                B.sg[i] = Sb[a.nosig ? 0 : kc[i]];
+            }
+        };
+        auto compute = [&](const Buf &B) {
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const double f = 4 * u + g < B.kn ? (a.nosig ? 1.0 : B.sg[u]) : 0.0;
+                const double bv[2] = {B.Bq[u].x * f, B.Bq[u].y * f};
+#pragma unroll
+                for (int ct = 0; ct < 4; ++ct) {
+                    const double2 a2 = B.A[2 * u + (ct >> 1)];
+                    const double av = (ct & 1) ? a2.y : a2.x;
+#pragma unroll
+                    for (int rt = 0; rt < 2; ++rt) acc[rt][ct] = mfma64(av, bv[rt], acc[rt][ct]);
+                }
+            }
+        };
+        Buf bA, bB;
+        load(bA, 0);
+#pragma unroll 1
+        for (int c = 0; c < nch; c += 2) {
+            load(bB, c + 1);
+            compute(bA);
+            load(bA, c + 2);
+            compute(bB);
+        }
+    }
+
+    // lane holds Y^T[c0 + 32(ct>>1) + 2(g + 4v) + (ct&1)][r0 + 2m + rt]
+    double yo[2][4][4];
+    if (beta != 0.0) {
+#pragma unroll
+        for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const int c = min(c0 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1), a.ncols - 1);
+                    yo[rt][ct][v] = C[(size_t)c * ldc + r0 + 2 * m + rt];
+                }
+    }
+#pragma unroll
+    for (int rt = 0; rt < 2; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int c = c0 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1);
+                acc[rt][ct][v] += beta == 0.0 ? 0.0 : beta * yo[rt][ct][v];
+                if (c < a.ncols) Y[(size_t)c * ldy + r0 + 2 * m + rt] = acc[rt][ct][v];
+            }
+    if (bmax) {
+        // column maxima per 128-row block: the two row tiles, the 16 lanes of
+        // a group, then the four waves of the block
+        double cm[4][4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                double x = fmax(fabs(acc[0][ct][v]), fabs(acc[1][ct][v]));
+#pragma unroll
+                for (int o = 1; o < 16; o <<= 1) x = fmax(x, __shfl_xor(x, o, 64));
+                cm[ct][v] = x;
+            }
+        if (m == 0)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    cmx[wv * 64 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1)] = cm[ct][v];
+        __syncthreads();
+        if (threadIdx.x < 2 * MM_CG) {
+            const int c = threadIdx.x & 63, h = threadIdx.x >> 6;
+            if (c0 + c < a.ncols) {
+                const double *q = cmx + h * 4 * 64 + c;
+                const double x = fmax(fmax(q[0], q[64]), fmax(q[128], q[192]));
+                bmax[(size_t)(c0 + c) * (N / ME_RB) + 2 * bx + h] = x;
+            }
+        }
+    }
+}
+
 // Table for levels [d0, d1) with operands P (reduced), Q (expanded) and S.
 static inline void mm_tab(const dplrh_plan *p, int d0, int d1, const double *const *P,
                    const double *const *Q, const double *const *S, int ncols, int w_local,

@@ -61,6 +61,57 @@ __global__ __launch_bounds__(256, 2) void k_mm_expand2(MmTab a, const double *__
                                                     double alpha, double beta,
                                                     const double *__restrict__ w, int wtc,
                                                     const double *C, int ldc, double *Y, int ldy,
+                                                    double *__restrict__ bmax);
+
+template <int TR>
+__global__ __launch_bounds__(64 * ME3_NW) void k_mm_expand3(MmTab a, const double *__restrict__ D,
+                                                        const double *__restrict__ X, int ldx,
+                                                        double alpha, double beta,
+                                                        const double *__restrict__ w, int wtc,
+                                                        const double *C, int ldc, double *Y,
+                                                        int ldy, double *__restrict__ bmax)
+{
+    extern __shared__ double lds_dyn[];
+    mm_expand3_body<TR>(a, blockIdx.x, blockIdx.y, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy,
+                        bmax, lds_dyn);
+}
+
+// The expansion over the table's levels (and the leaf when D is given):
+// the third form when its staged ranks fit in LDS, else the second.
+static constexpr size_t ME3_LDS_MAX = 150 * 1024;
+
+template <int TR>
+static int mm_expand_launch(dplrh_plan *p, const MmTab &t, const double *D, const double *X,
+                            int ldx, double alpha, double beta, const double *w, int wtc,
+                            const double *C, int ldc, double *Y, int ldy, double *bmax,
+                            hipStream_t st)
+{
+    int rows = 0;
+    for (int d = 0; d < t.L && t.seglen[d] >= ME3_RB; ++d) rows += t.R[d];
+    const size_t lds = me3_lds(rows) * sizeof(double);
+    if (p->N % ME3_RB == 0 && lds <= ME3_LDS_MAX) {
+        static bool attr = false;
+        if (!attr) {
+            PLR_HIP(hipFuncSetAttribute((const void *)k_mm_expand3<TR>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, (int)ME3_LDS_MAX));
+            attr = true;
+        }
+        hipLaunchKernelGGL(k_mm_expand3<TR>, dim3(p->N / ME3_RB, t.ncg), dim3(64 * ME3_NW), lds, st,
+                           t, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy, bmax);
+    } else {
+        hipLaunchKernelGGL(k_mm_expand2<TR>, dim3(p->N / ME_RB, t.ncg), dim3(256), 0, st, t, D, X,
+                           ldx, alpha, beta, w, wtc, C, ldc, Y, ldy, bmax);
+    }
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+template <int TR>
+__global__ __launch_bounds__(256, 2) void k_mm_expand2(MmTab a, const double *__restrict__ D,
+                                                    const double *__restrict__ X, int ldx,
+                                                    double alpha, double beta,
+                                                    const double *__restrict__ w, int wtc,
+                                                    const double *C, int ldc, double *Y, int ldy,
                                                     double *__restrict__ bmax)
 {
     mm_expand2_body<TR>(a, blockIdx.x, blockIdx.y, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy,
@@ -156,11 +207,10 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
                                alpha, trans, 0, p->tmp, w, (size_t)wtc);
             PLR_HIP(hipGetLastError());
         }
-        hipLaunchKernelGGL(trans ? k_mm_expand2<1> : k_mm_expand2<0>, dim3(p->N / ME_RB, t.ncg),
-                           dim3(256), 0, st, t, p->D, X, ldx, alpha, beta, w, wtc, C ? C : Y,
-                           C ? ldc : ldy, Y, ldy, bmax);
-        PLR_HIP(hipGetLastError());
-        return 0;
+        return trans ? mm_expand_launch<1>(p, t, p->D, X, ldx, alpha, beta, w, wtc, C ? C : Y,
+                                           C ? ldc : ldy, Y, ldy, bmax, st)
+                     : mm_expand_launch<0>(p, t, p->D, X, ldx, alpha, beta, w, wtc, C ? C : Y,
+                                           C ? ldc : ldy, Y, ldy, bmax, st);
     }
     if (t.task_off[t.L] > 0) {
         hipLaunchKernelGGL(k_mm_reduce<1>, dim3(t.task_off[t.L]), dim3(64), 0, st, t, X, ldx, alpha,
@@ -283,11 +333,8 @@ int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W,
     mm_tab(p, 0, d1, P, Q, S, ncols, 0, &t);
     t.nosig = 1;
     if (p->N % ME_RB != 0 || (size_t)t.ncg * MM_CG > ws) return 1;
-    hipLaunchKernelGGL(k_mm_expand2<0>, dim3(p->N / ME_RB, t.ncg), dim3(256), 0, st, t,
-                       (const double *)nullptr, (const double *)Y, ldy, 1.0, 1.0, W, (int)ws,
-                       (const double *)Y, ldy, Y, ldy, (double *)nullptr);
-    PLR_HIP(hipGetLastError());
-    return 0;
+    return mm_expand_launch<0>(p, t, nullptr, Y, ldy, 1.0, 1.0, W, (int)ws, Y, ldy, Y, ldy,
+                               nullptr, st);
 }
 
 // T[c][s*R_d + k] = sum of the raw partial blocks in p->tmp that a caller
