@@ -220,6 +220,230 @@ __global__ __launch_bounds__(64 * TL_NW) void k_solve_tail(TailTab a, const doub
 }
 
 // ---------------------------------------------------------------------------
+// The same bottom solve with its global loads issued a phase ahead, for
+// ranks that are multiples of 16 and at most 32 (RT <= 32), with NW = span/32
+// waves (two per 64-row chunk).  Work of a phase is spread over all waves:
+//   leaves:  tasks (leaf, pair of 16-row tiles), the leaf's z read by every
+//            task before any of them writes it back (a barrier between);
+//   reduce:  tasks (64-row chunk of the span, rank tile): partial T per
+//            chunk in the T region, [chunk][k][c], then summed per segment
+//            (in chunk order) into the segment's first chunk; each lane
+//            group takes rows 8h + 2g + {0,1} of a 16-row step, so one load
+//            of the four groups covers 64 contiguous bytes of a V row;
+//   cap:     tasks (node, 16-row tile of [c1; c2]);
+//   expand:  tasks (16-row tile of the span), two per wave.
+// The V rows of a level's reduction are loaded during the previous phase,
+// inv(K) and sig during the reduction, Ut during the capacitance solve.
+template <int NW>
+__global__ __launch_bounds__(64 * NW) void k_solve_tail2(TailTab a, const double *__restrict__ B,
+                                                     int ldb, double *__restrict__ X, int ldx,
+                                                     int ncols)
+{
+    extern __shared__ double lds[];
+    const int span = a.span, zs = span + 1, RT = a.RT, ts = TL_CW + 1;
+    const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int blk = blockIdx.x, c0 = blockIdx.y * TL_CW;
+    const int row0 = blk * span, N = a.N;
+    const int nc = min(TL_CW, ncols - c0);
+    const int nch = span / 64; // 64-row chunks (= leaves)
+    double *z = lds;                  // [c][row], stride zs
+    double *T = z + TL_CW * zs;       // [chunk][k][c], RT x ts per chunk
+    double *W = T + nch * RT * ts;    // [segment][k][c]
+
+    // leaf operands: task wv = (leaf wv/2, tiles 2(wv&1) + i); Dinv column-major 64 x 64
+    const int lf = wv >> 1, qh = wv & 1;
+    const bool lfw = lf < nch;
+    double dv[2][16];
+    {
+        const double *Dl = a.Dinv + (size_t)(row0 / 64 + min(lf, nch - 1)) * 64 * 64;
+#pragma unroll
+        for (int i = 0; i < 2; ++i)
+#pragma unroll
+            for (int u = 0; u < 16; ++u) dv[i][u] = Dl[(size_t)(4 * u + g) * 64 + 16 * (2 * qh + i) + m];
+    }
+    // b of the span (columns c0.., zero padding columns)
+    for (int e = threadIdx.x; e < TL_CW * span; e += 64 * NW) {
+        const int c = e / span, r = e - c * span;
+        const double v = B[(size_t)(c0 + min(c, nc - 1)) * ldb + row0 + r];
+        z[c * zs + r] = c < nc ? v : 0.0;
+    }
+    __syncthreads();
+    // leaves: z|_leaf = Dinv_leaf b|_leaf
+    {
+        tl_d4 acc[2] = {tl_d4{0.0, 0.0, 0.0, 0.0}, tl_d4{0.0, 0.0, 0.0, 0.0}};
+        if (lfw)
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const double bv = z[m * zs + lf * 64 + 4 * u + g];
+#pragma unroll
+                for (int i = 0; i < 2; ++i) acc[i] = tl_mfma(dv[i][u], bv, acc[i]);
+            }
+        __syncthreads(); // every read of the leaves is done
+        if (lfw)
+#pragma unroll
+            for (int i = 0; i < 2; ++i)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) z[m * zs + lf * 64 + 16 * (2 * qh + i) + g + 4 * v] = acc[i][v];
+    }
+
+    // reduction operands of level d: task wv = (chunk wv/2, rank tile wv&1);
+    // va[q][h] = V[kt*16 + m][row0 + 64ch + 16q + 8h + 2g + {0,1}]
+    const int rch = wv >> 1, rkt = wv & 1;
+    double2 va[4][2];
+    auto vload = [&](const double *Vd, int R) {
+        const double *pv = Vd + (size_t)(min(rkt * 16, R - 16) + m) * N + row0 + 64 * min(rch, nch - 1) + 2 * g;
+#pragma unroll
+        for (int q = 0; q < 4; ++q)
+#pragma unroll
+            for (int h = 0; h < 2; ++h) va[q][h] = *(const double2 *)(pv + 16 * q + 8 * h);
+    };
+    int d = a.L - 1;
+    while (d >= a.d0 && a.R[d] == 0) --d;
+    if (d >= a.d0) vload(a.V[d], a.R[d]);
+    __syncthreads();
+
+    for (; d >= a.d0; --d) {
+        const int R = a.R[d];
+        if (R == 0) continue;
+        const int seg = N >> (d + 1), nsg = span / seg, cps = seg / 64, nkt = R / 16, n2 = 2 * R;
+        const int sg0 = row0 / seg; // global index of the span's first segment
+        // 1. partial T per 64-row chunk
+        if (rch < nch && rkt < nkt) {
+            tl_d4 acc = tl_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+#pragma unroll
+                for (int h = 0; h < 2; ++h)
+#pragma unroll
+                    for (int e = 0; e < 2; ++e) {
+                        const int row = 64 * rch + 16 * q + 8 * h + 2 * g + e;
+                        acc = tl_mfma(e ? va[q][h].y : va[q][h].x, z[m * zs + row], acc);
+                    }
+            // lane holds T[rank rkt*16 + g + 4v][column m] of chunk rch
+#pragma unroll
+            for (int v = 0; v < 4; ++v) T[(rch * RT + rkt * 16 + g + 4 * v) * ts + m] = acc[v];
+        }
+        // operands of the capacitance solve: task wv = (node, 16-row tile it)
+        const int ntl = n2 / 16, cnode = wv / ntl, cit = wv - cnode * ntl;
+        const bool capw = cnode < nsg / 2;
+        double kv[16], sv[4];
+        {
+            const int nodeg = (sg0 >> 1) + min(cnode, nsg / 2 - 1);
+            const double *K = a.Kinv[d] + (size_t)nodeg * n2 * n2 + cit * 16 + m;
+#pragma unroll
+            for (int u = 0; u < 16; ++u) kv[u] = K[(size_t)min(4 * u + g, n2 - 1) * n2];
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ii = cit * 16 + g + 4 * v;
+                const int sgl = 2 * min(cnode, nsg / 2 - 1) + (ii < R ? 0 : 1);
+                sv[v] = a.S[d][(size_t)(sg0 + sgl) * R + (ii < R ? ii : ii - R)];
+            }
+        }
+        __syncthreads();
+        // T of each segment: its chunks summed in order into its first chunk
+        if (cps > 1) {
+            for (int e = threadIdx.x; e < nsg * R * TL_CW; e += 64 * NW) {
+                const int c = e % TL_CW, k = (e / TL_CW) % R, sgl = e / (TL_CW * R);
+                double t = T[((sgl * cps) * RT + k) * ts + c];
+                for (int cc = 1; cc < cps; ++cc) t += T[((sgl * cps + cc) * RT + k) * ts + c];
+                T[((sgl * cps) * RT + k) * ts + c] = t;
+            }
+            __syncthreads();
+        }
+        // 2. c = inv(K) [T_bot; T_top]; W_top = -S_top .* c1, W_bot = -S_bot .* c2
+        if (capw) {
+            const int top = 2 * cnode, bot = top + 1; // local segments
+            tl_d4 acc = tl_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 16; ++u) {
+                const int j = 4 * u + g;
+                if (4 * u < n2) {
+                    const int sgl = j < R ? bot : top, kk = j < R ? j : j - R;
+                    acc = tl_mfma(kv[u], T[((sgl * cps) * RT + kk) * ts + m], acc);
+                }
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                const int ii = cit * 16 + g + 4 * v;
+                const int sgl = ii < R ? top : bot, kk = ii < R ? ii : ii - R;
+                W[(sgl * RT + kk) * ts + m] = -sv[v] * acc[v];
+            }
+        }
+        // operands of the expansion: tiles wv and wv + 16 of the span
+        double uv[2][8];
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int tile = min(wv + NW * p, span / 16 - 1);
+#pragma unroll
+            for (int u = 0; u < 8; ++u)
+                uv[p][u] = a.Ut[d][(size_t)min(4 * u + g, R - 1) * N + row0 + 16 * tile + m];
+        }
+        __syncthreads();
+        // 3. z|_s += Ut_d|_s W_s
+#pragma unroll
+        for (int p = 0; p < 2; ++p) {
+            const int tile = wv + NW * p;
+            if (tile < span / 16) {
+                const int rr = tile * 16, sl = rr / seg;
+                tl_d4 acc = tl_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+                    if (4 * u < R) acc = tl_mfma(uv[p][u], W[(sl * RT + 4 * u + g) * ts + m], acc);
+                // lane holds (Ut W)[row rr + g + 4v][column m]
+#pragma unroll
+                for (int v = 0; v < 4; ++v) z[m * zs + rr + g + 4 * v] += acc[v];
+            }
+        }
+        // the next level's reduction operands
+        {
+            int dn = d - 1;
+            while (dn >= a.d0 && a.R[dn] == 0) --dn;
+            if (dn >= a.d0) vload(a.V[dn], a.R[dn]);
+        }
+        __syncthreads();
+    }
+
+    // 4. T of level d0 - 1 over the span: tasks (16-rank tile, 32-row chunk),
+    // chunk partials combined in a fixed order through the (free) T region
+    if (a.Rn > 0) {
+        const int Rn = a.Rn, nkt = Rn / 16, n32 = span / 32, ntask = nkt * n32;
+        const double *Vn = a.V[a.d0 - 1];
+        double *red = T; // [task][lane][4]
+        for (int task = wv; task < ntask; task += NW) {
+            const int kt = task / n32, ch = task - kt * n32;
+            const double *pv = Vn + (size_t)(kt * 16 + m) * N + row0 + ch * 32 + g;
+            double av[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) av[u] = pv[4 * u];
+            tl_d4 acc = tl_d4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = tl_mfma(av[u], z[m * zs + ch * 32 + 4 * u + g], acc);
+#pragma unroll
+            for (int v = 0; v < 4; ++v) red[(task * 64 + lane) * 4 + v] = acc[v];
+        }
+        __syncthreads();
+        const int sg = row0 / span;
+        for (int kt = wv; kt < nkt; kt += NW) {
+            double t[4] = {0.0, 0.0, 0.0, 0.0};
+            for (int ch = 0; ch < n32; ++ch)
+#pragma unroll
+                for (int v = 0; v < 4; ++v) t[v] += red[((kt * n32 + ch) * 64 + lane) * 4 + v];
+            if (m < nc)
+#pragma unroll
+                for (int v = 0; v < 4; ++v)
+                    a.Tout[(size_t)(c0 + m) * a.tstride + (size_t)sg * Rn + kt * 16 + g + 4 * v] = t[v];
+        }
+    }
+
+    // 5. write the span back
+    for (int e = threadIdx.x; e < TL_CW * span; e += 64 * NW) {
+        const int c = e / span, r = e - c * span;
+        if (c < nc) X[(size_t)(c0 + c) * ldx + row0 + r] = z[c * zs + r];
+    }
+}
+
+// ---------------------------------------------------------------------------
 // One pass per level above the tail (d = d0-1 .. 0), FP64 MFMA throughout.
 // A work-group owns 128 rows x 64 right-hand sides, four waves of 32 rows:
 //   1. rhs = [T_bot; T_top] of its node into LDS
@@ -509,7 +733,30 @@ static int solve_tail_t(dplrh_plan *p, const double *Bm, int ldb, double *Xm, in
     }
     if (lds > 160 * 1024) return -1;
     dim3 grid(N / span, (ncols + TL_CW - 1) / TL_CW);
-    hipLaunchKernelGGL(k_solve_tail, grid, dim3(64 * TL_NW), lds, st, t, Bm, ldb, Xm, ldx, ncols);
+    // the second form: ranks of the tail levels multiples of 16 and at most 32
+    bool second = RT <= 32 && (span == 512 || span == 256 || span == 128);
+    for (int d = d0; d < L; ++d)
+        if (p->R[d] % 16 != 0) second = false;
+    if (second) {
+        static bool attr2 = false;
+        if (!attr2) {
+            PLR_HIP(hipFuncSetAttribute((const void *)k_solve_tail2<16>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            PLR_HIP(hipFuncSetAttribute((const void *)k_solve_tail2<8>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            PLR_HIP(hipFuncSetAttribute((const void *)k_solve_tail2<4>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+            attr2 = true;
+        }
+        if (span == 512)
+            hipLaunchKernelGGL(k_solve_tail2<16>, grid, dim3(64 * 16), lds, st, t, Bm, ldb, Xm, ldx, ncols);
+        else if (span == 256)
+            hipLaunchKernelGGL(k_solve_tail2<8>, grid, dim3(64 * 8), lds, st, t, Bm, ldb, Xm, ldx, ncols);
+        else
+            hipLaunchKernelGGL(k_solve_tail2<4>, grid, dim3(64 * 4), lds, st, t, Bm, ldb, Xm, ldx, ncols);
+    } else {
+        hipLaunchKernelGGL(k_solve_tail, grid, dim3(64 * TL_NW), lds, st, t, Bm, ldb, Xm, ldx, ncols);
+    }
     PLR_HIP(hipGetLastError());
     return d0;
 }
