@@ -240,7 +240,8 @@ def precision_leg(args, torch, D, log):
         B = torch.randn(nrhs, m, generator=g, dtype=torch.float64).to("cuda")
         Z64 = torch.zeros_like(B)
         Z = torch.zeros_like(B)
-        P.solve("N", B, Z64)
+        for _ in range(3):  # eager, captured, replayed
+            P.solve("N", B, Z64)
         ms64 = event_ms(torch, lambda i: P.solve("N", B, Z64), 5)
         for lp in ("f16", "bf16"):
             assert P.factor_mixed(lp) == 0
@@ -385,7 +386,9 @@ def main():
     # --- legs of the step ---------------------------------------------------
     apply = lambda i: plans[i % len(plans)].gepdmm("N", 1.0, X, 0.0, Y)
     solve64 = lambda i: plans[i % len(plans)].solve("N", B, Z)
-    for i in range(3):
+    # every plan's first call runs eagerly and the second is captured into a
+    # HIP graph (plr_run_cached); warm up past both for each plan
+    for i in range(3 * len(plans)):
         apply(i)
         solve64(i)
     ms_apply = event_ms(torch, apply, 10)
@@ -414,7 +417,7 @@ def main():
         iters = 11
     log(f"mixed ({args.lp}): {iters} refined solves reach rel err {err:.2e}")
     mixed = lambda i: plans[i % len(plans)].solve_refined(B, Z, iters)
-    for i in range(2):
+    for i in range(2 * len(plans)):
         mixed(i)
     ms_mixed = event_ms(torch, mixed, 10)
 

@@ -217,8 +217,24 @@ static size_t up_arows(const dplrh_plan *p, int e, size_t *off)
     return r;
 }
 
-// After the factorization: the G blocks (rocBLAS strided-batched GEMMs, one
-// per pair of levels).
+// G[s][i] = sum over j < per of C[(s*per + j)][i], i < mn (fixed order).
+__global__ void k_up_gsum(const double *__restrict__ C, int per, int nblk, int mn,
+                          double *__restrict__ G)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)nblk * mn) return;
+    const size_t s = e / mn, i = e - s * mn;
+    const double *c = C + s * per * mn + i;
+    double t = 0.0;
+    for (int j = 0; j < per; ++j) t += c[(size_t)j * mn];
+    G[e] = t;
+}
+
+// After the factorization: the G blocks.  A segment of level e is long (up
+// to N/4 rows) and G is only R_d x R_e, so each segment is cut into chunks
+// of at most 512 rows: one rocBLAS strided-batched GEMM over all chunks of
+// the level (N/chunk small products), then the chunks of a segment summed
+// in order.
 int plr_up_factor(dplrh_plan *p, hipStream_t st)
 {
     plr_fact &f = p->f;
@@ -241,17 +257,29 @@ int plr_up_factor(dplrh_plan *p, hipStream_t st)
     rocblas_set_stream(h, st);
     const double one = 1.0, zero = 0.0;
     int rc = 0;
+    double *C = f.work; // free once the factorization is done: N*b >= (N/chunk) * R^2 doubles
     for (int e = 1; e < d0 && !rc; ++e) {
         const int seg = N >> (e + 1), nseg = 2 << e, Re = p->R[e];
+        int chunk = seg & -seg;
+        chunk = chunk > 512 ? 512 : chunk;
+        const int per = seg / chunk, nch = nseg * per;
         for (int d = 0; d < e && !rc; ++d) {
             const int Rd = p->R[d];
-            // G[s] = V_d(rows of s)^T Ut_e(rows of s)
+            // C[j] = V_d(rows of chunk j)^T Ut_e(rows of chunk j); G[s] = sum of its chunks
             rocblas_status s = rocblas_dgemm_strided_batched(
-                h, rocblas_operation_transpose, rocblas_operation_none, Rd, Re, seg, &one,
-                p->V + p->uoff[d], N, seg, f.Ut + p->uoff[e], N, seg, &zero, f.G + f.goff[d][e], Rd,
-                (rocblas_stride)Rd * Re, nseg);
+                h, rocblas_operation_transpose, rocblas_operation_none, Rd, Re, chunk, &one,
+                p->V + p->uoff[d], N, chunk, f.Ut + p->uoff[e], N, chunk, &zero, C, Rd,
+                (rocblas_stride)Rd * Re, nch);
             if (s != rocblas_status_success) {
                 plr_set_error("rocblas_dgemm_strided_batched failed");
+                rc = -1;
+                break;
+            }
+            const size_t tot = (size_t)nseg * Rd * Re;
+            hipLaunchKernelGGL(k_up_gsum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, C,
+                               per, nseg, Rd * Re, f.G + f.goff[d][e]);
+            if (hipGetLastError() != hipSuccess) {
+                plr_set_error("k_up_gsum launch failed");
                 rc = -1;
             }
         }
