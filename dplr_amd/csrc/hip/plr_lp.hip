@@ -20,6 +20,7 @@
 // x = 0, r = b; repeat { s = max|r| per column; x += s * solve_lp(r / s);
 // r = b - A x in FP64 }.
 #include <cmath>
+#include <cstdint>
 #include <cstring>
 #include <vector>
 
@@ -1305,15 +1306,14 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
     const int s = r0 / a.seg, istop = !(s & 1), top = s & ~1, bot = top + 1;
     const int nks = R / 32, nkt = R / 16, nktn = a.Rn / 16;
 
-    // old z (columns as the expansion's D: c = 16ct + 4g + v, row ri + 16rt + m)
-    float zo[LV_RT][4][4];
+    // old z as the expansion's D (M = rows, N = columns): rows ri + 16rt +
+    // 4g + v of column 16ct + m, one 16-byte load per tile (ldz % 4 == 0)
+    f4 zo[LV_RT][4];
 #pragma unroll
     for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct)
-#pragma unroll
-            for (int v = 0; v < 4; ++v)
-                zo[rt][ct][v] = Z[(size_t)(c0 + min(ct * 16 + 4 * g + v, nc - 1)) * ldz + ri + 16 * rt + m];
+            zo[rt][ct] = *(const f4 *)(Z + (size_t)(c0 + min(ct * 16 + m, nc - 1)) * ldz + ri + 16 * rt + 4 * g);
     // Ut rows of the expansion
     const T16 *U = (const T16 *)a.Ut;
     v8 bq[LV_RT][KS];
@@ -1390,14 +1390,14 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
         }
     }
     __syncthreads();
-    // 3. z += Ut W (M = columns, N = rows): LV_RT 16-row tiles per wave
+    // 3. z += Ut W (M = rows, N = columns): LV_RT 16-row tiles per wave
     {
-        float isc[4], sc[4][4];
+        const bool xal = X && ((uintptr_t)X & 15) == 0 && ldx % 2 == 0;
+        float isc[4], sc[4];
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) {
-            isc[ct] = 1.f / lp_pow2(smax[ct * 16 + m]);
-#pragma unroll
-            for (int v = 0; v < 4; ++v) sc[ct][v] = lp_pow2(smax[ct * 16 + 4 * g + v]);
+            sc[ct] = lp_pow2(smax[ct * 16 + m]);
+            isc[ct] = 1.f / sc[ct];
         }
         f4 acc[LV_RT][4];
 #pragma unroll
@@ -1409,30 +1409,46 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
             if (ks < nks) {
 #pragma unroll
                 for (int ct = 0; ct < 4; ++ct) {
-                    const v8 aw = cvt8<T16>(lds8(W + (ct * 16 + m) * wsk + ks * 32 + 8 * g), isc[ct]);
+                    const v8 bw = cvt8<T16>(lds8(W + (ct * 16 + m) * wsk + ks * 32 + 8 * g), isc[ct]);
 #pragma unroll
-                    for (int rt = 0; rt < LV_RT; ++rt) acc[rt][ct] = Lp<T16>::mfma(aw, bq[rt][ks], acc[rt][ct]);
+                    for (int rt = 0; rt < LV_RT; ++rt) acc[rt][ct] = Lp<T16>::mfma(bq[rt][ks], bw, acc[rt][ct]);
                 }
             }
         }
-        // lane holds dz[c = ct*16 + 4g + v][i = 16 (LV_RT wv + rt) + m]
+        // lane holds dz[row i = 16 (LV_RT wv + rt) + 4g + v][column c = 16ct + m]
 #pragma unroll
         for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct)
+            for (int ct = 0; ct < 4; ++ct) {
+                const int c = ct * 16 + m, i = (wv * LV_RT + rt) * 16 + 4 * g;
+                const float on = c < nc ? 1.f : 0.f;
+                f4 zn;
 #pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                const int c = ct * 16 + 4 * g + v, i = (wv * LV_RT + rt) * 16 + m;
-                const float zn = zo[rt][ct][v] * (c < nc ? 1.f : 0.f) + acc[rt][ct][v] * sc[ct][v];
+                for (int v = 0; v < 4; ++v) zn[v] = zo[rt][ct][v] * on + acc[rt][ct][v] * sc[ct];
                 if (X) {
                     if (c < nc) {
                         double *xp = X + (size_t)(c0 + c) * ldx + r0 + i;
                         const double sr = scale[c0 + c] > 0.0 ? scale[c0 + c] : 1.0;
-                        *xp = (first ? 0.0 : *xp) + sr * (double)zn;
+                        if (xal) { // four consecutive rows as two 16-byte accesses
+#pragma unroll
+                            for (int h = 0; h < 2; ++h) {
+                                double2 o = first ? double2{0.0, 0.0} : *(const double2 *)(xp + 2 * h);
+                                o.x += sr * (double)zn[2 * h];
+                                o.y += sr * (double)zn[2 * h + 1];
+                                *(double2 *)(xp + 2 * h) = o;
+                            }
+                        } else {
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) xp[v] = (first ? 0.0 : xp[v]) + sr * (double)zn[v];
+                        }
                     }
                 } else {
-                    if (c < nc) Z[(size_t)(c0 + c) * ldz + r0 + i] = zn;
-                    zt[c * zts + i] = (T16)zn;
+                    if (c < nc) *(f4 *)(Z + (size_t)(c0 + c) * ldz + r0 + i) = zn;
+                    typedef T16 t4 __attribute__((ext_vector_type(4)));
+                    t4 zh;
+#pragma unroll
+                    for (int v = 0; v < 4; ++v) zh[v] = (T16)zn[v];
+                    *(t4 *)(zt + c * zts + i) = zh;
                 }
             }
     }
