@@ -142,9 +142,10 @@ def test_solve_graph_replays_match_eager(cuda):
 @pytest.mark.parametrize("m,nlvl,rank,n", [(8192, 7, 32, 64), (16384, 8, 16, 70), (4096, 6, 32, 5),
                                             (8192, 7, 32, 1)])
 def test_solve_upper_levels_fused(cuda, orc, m, nlvl, rank, n):
-    """Leaf 64, N a multiple of 256, ranks 16 or 32: the bottom kernel, then one
-    fused pass per level above it (capacitance rows, expansion, the next
-    level's partial reduction); against the oracle's inverse applied to B."""
+    """Leaf 64, N a multiple of 256, ranks 16 or 32: the bottom kernel, then
+    the reassociated upper levels (one all-level reduction, a small kernel
+    per level with the cross-level blocks, one expansion); against the
+    oracle's inverse applied to B."""
     import torch
     print(f"fused solve m={m} L={nlvl} r={rank} n={n}", flush=True)
     A = PlrTree.random(m, nlvl, rank, seed=32)
@@ -160,3 +161,32 @@ def test_solve_upper_levels_fused(cuda, orc, m, nlvl, rank, n):
     r = D.gepdmm("N", "N", A, X, n, api=orc) - B
     assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-12
     P.close()
+
+
+@pytest.mark.parametrize("m,nlvl,rank,n", [(8192, 7, 32, 40), (16384, 8, 16, 3)])
+def test_solve_upper_levels_reassociated_vs_per_level(cuda, orc, m, nlvl, rank, n):
+    """The reassociated upper levels need a 16-byte aligned X with an even
+    leading dimension; an odd one (m + 1) takes the level-by-level kernels.
+    Both orders of summation must agree with each other and with the
+    oracle's residual to 1e-12."""
+    import torch
+    print(f"upper paths m={m} L={nlvl} r={rank} n={n}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=33)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    rng = np.random.default_rng(10)
+    B = rng.standard_normal((m, n))
+    Bd = torch.tensor(B.T.copy(), device=cuda)
+    Xa = torch.zeros_like(Bd)
+    P.solve("N", Bd, Xa)                                  # reassociated
+    Xpad = torch.zeros((n, m + 1), dtype=torch.float64, device=cuda)
+    Xb = Xpad[:, :m]                                      # ldx = m + 1: per level
+    P.solve("N", Bd, Xb)
+    torch.cuda.synchronize()
+    xa, xb = Xa.cpu().numpy().T, Xb.cpu().numpy().T
+    assert np.linalg.norm(xa - xb) / np.linalg.norm(xb) < 1e-12
+    for X in (xa, xb):
+        r = D.gepdmm("N", "N", A, X, n, api=orc) - B
+        assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-12
+    P.close()
+    A.free()
