@@ -81,6 +81,7 @@ struct dplrh_plan
     plr_fact f;
     plr_solve_ws sw;
     int force_valu; // testing aid: use the general-shape kernels everywhere
+    void *blas;     // rocBLAS handle of the factorization (created on first use)
     // HIP graphs of repeated calls (see plr_run_cached)
     unsigned ws_gen;  // bumped whenever plan-owned device memory is reallocated
     hipStream_t cap;  // private capture stream (default priority)
@@ -179,6 +180,7 @@ int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W,
                     double *Y, int ldy, int ncols, hipStream_t st);
 // Reassociated upper levels of the FP64 solve (plr_up.hip).
 int plr_up_factor(dplrh_plan *p, hipStream_t st);
+void plr_up_release(dplrh_plan *p);
 int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream_t st);
 int plr_tail_d0(const dplrh_plan *p);
 int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,

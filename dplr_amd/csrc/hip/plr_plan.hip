@@ -256,6 +256,7 @@ extern "C" void dplrh_plan_destroy(dplrh_plan *p)
         if (b) (void)hipFree(b);
     plr_free_fact(p);
     plr_lp_release(p);
+    plr_up_release(p);
     delete p;
 }
 

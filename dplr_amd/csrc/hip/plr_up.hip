@@ -235,6 +235,12 @@ __global__ void k_up_gsum(const double *__restrict__ C, int per, int nblk, int m
 // of at most 512 rows: one rocBLAS strided-batched GEMM over all chunks of
 // the level (N/chunk small products), then the chunks of a segment summed
 // in order.
+void plr_up_release(dplrh_plan *p)
+{
+    if (p->blas) rocblas_destroy_handle((rocblas_handle)p->blas);
+    p->blas = nullptr;
+}
+
 int plr_up_factor(dplrh_plan *p, hipStream_t st)
 {
     plr_fact &f = p->f;
@@ -249,11 +255,15 @@ int plr_up_factor(dplrh_plan *p, hipStream_t st)
             tot += ((size_t)2 << e) * p->R[d] * p->R[e];
         }
     if (!f.G) PLR_HIP(hipMalloc(&f.G, tot * sizeof(double)));
-    rocblas_handle h = nullptr;
-    if (rocblas_create_handle(&h) != rocblas_status_success) {
-        plr_set_error("rocblas_create_handle failed");
-        return -1;
+    if (!p->blas) {
+        rocblas_handle hn = nullptr;
+        if (rocblas_create_handle(&hn) != rocblas_status_success) {
+            plr_set_error("rocblas_create_handle failed");
+            return -1;
+        }
+        p->blas = hn;
     }
+    rocblas_handle h = (rocblas_handle)p->blas;
     rocblas_set_stream(h, st);
     const double one = 1.0, zero = 0.0;
     int rc = 0;
@@ -284,7 +294,6 @@ int plr_up_factor(dplrh_plan *p, hipStream_t st)
             }
         }
     }
-    rocblas_destroy_handle(h);
     if (!rc) f.up_d0 = d0;
     return rc;
 }
