@@ -262,11 +262,24 @@ __global__ __launch_bounds__(64 * NW) void k_solve_tail2(TailTab a, const double
 #pragma unroll
             for (int u = 0; u < 16; ++u) dv[i][u] = Dl[(size_t)(4 * u + g) * 64 + 16 * (2 * qh + i) + m];
     }
-    // b of the span (columns c0.., zero padding columns)
-    for (int e = threadIdx.x; e < TL_CW * span; e += 64 * NW) {
-        const int c = e / span, r = e - c * span;
-        const double v = B[(size_t)(c0 + min(c, nc - 1)) * ldb + row0 + r];
-        z[c * zs + r] = c < nc ? v : 0.0;
+    // b of the span (columns c0.., zero padding columns), eight loads per
+    // thread in flight at once
+    for (int e0 = 0; e0 < TL_CW * span; e0 += 8 * 64 * NW) {
+        double v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = min(e0 + u * 64 * NW + (int)threadIdx.x, TL_CW * span - 1);
+            const int c = e / span, r = e - c * span;
+            v[u] = B[(size_t)(c0 + min(c, nc - 1)) * ldb + row0 + r];
+        }
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int e = e0 + u * 64 * NW + (int)threadIdx.x;
+            if (e < TL_CW * span) {
+                const int c = e / span, r = e - c * span;
+                z[c * zs + r] = c < nc ? v[u] : 0.0;
+            }
+        }
     }
     __syncthreads();
     // leaves: z|_leaf = Dinv_leaf b|_leaf
