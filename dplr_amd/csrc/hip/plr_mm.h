@@ -921,7 +921,7 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
 static constexpr int ME3_RB = 256, ME3_NW = 8, ME3_CS = 65;
 
 // LDS doubles of the third form for `rows` staged ranks (W^T rows + sig).
-__host__ __device__ inline size_t me3_lds(int rows) { return (size_t)rows * (ME3_CS + 1); }
+__host__ __device__ inline size_t me3_lds(int rows) { return (size_t)rows * (ME3_CS + 2); }
 
 template <int TR>
 __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
@@ -944,31 +944,38 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
     while (dsh < a.L && a.seglen[dsh] >= ME3_RB) rows += a.R[dsh++];
     double *Ws = lds;                    // [rows][ME3_CS]: W^T[c0 + c][k] at Ws[k * CS + c]
     double *Ss = lds + (size_t)rows * ME3_CS; // sig of the staged ranks
+    // element offset of each staged row in w; the level loop is uniform, so
+    // the table entries come from scalar loads (a per-lane level search
+    // would read the kernel arguments through dependent vector loads)
+    long long *ro = (long long *)(Ss + rows);
+    {
+        int row0 = 0;
+        for (int d = 0; d < dsh; ++d) {
+            const int R = a.R[d], s = rb0 / a.seglen[d];
+            const double *Sd = a.S[d] + (size_t)(TR ? (s ^ 1) : s) * R;
+            for (int k = threadIdx.x; k < R; k += 64 * ME3_NW) {
+                ro[row0 + k] = ((long long)a.w_off[d] + (long long)s * R + k) * wtc + c0;
+                Ss[row0 + k] = a.nosig ? 1.0 : Sd[k];
+            }
+            row0 += R;
+        }
+    }
+    __syncthreads();
     // eight pairs per thread in flight at once (a loop of single loads and
     // stores would wait on memory once per pair)
     for (int e0 = 0; e0 < rows * 32; e0 += 8 * 64 * ME3_NW) {
         double2 v[8];
-        double sgv[8];
-        int ro[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = min(e0 + u * 64 * ME3_NW + (int)threadIdx.x, rows * 32 - 1);
-            const int row = e >> 5, cp = e & 31;
-            int d = 0, k = row;
-            while (k >= a.R[d]) k -= a.R[d++];
-            const int R = a.R[d], s = rb0 / a.seglen[d];
-            ro[u] = row;
-            v[u] = *(const double2 *)(w + ((size_t)a.w_off[d] + (size_t)s * R + k) * wtc + c0 + 2 * cp);
-            sgv[u] = a.nosig ? 1.0 : a.S[d][(size_t)(TR ? (s ^ 1) : s) * R + k];
+            v[u] = *(const double2 *)(w + ro[e >> 5] + 2 * (e & 31));
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * 64 * ME3_NW + (int)threadIdx.x;
             if (e < rows * 32) {
-                const int cp = e & 31;
-                Ws[ro[u] * ME3_CS + 2 * cp] = v[u].x;
-                Ws[ro[u] * ME3_CS + 2 * cp + 1] = v[u].y;
-                if (cp == 0) Ss[ro[u]] = sgv[u];
+                Ws[(e >> 5) * ME3_CS + 2 * (e & 31)] = v[u].x;
+                Ws[(e >> 5) * ME3_CS + 2 * (e & 31) + 1] = v[u].y;
             }
         }
     }
