@@ -195,6 +195,7 @@ typedef double d2v __attribute__((ext_vector_type(2)));
 struct Mv2Tab
 {
     int L, N, nb, nch; // reduction batches of KG ranks, expansion chunks of 32 ranks
+    int trans;
     int R[PLR_MAXL], seg[PLR_MAXL], TW[PLR_MAXL], tps[PLR_MAXL], nkg[PLR_MAXL];
     int task_off[PLR_MAXL + 1], part_off[PLR_MAXL + 1], fin_off[PLR_MAXL + 1];
     int w_off[PLR_MAXL + 1];
@@ -213,8 +214,8 @@ __device__ __forceinline__ double mv_coef(double t, int raw, double alpha, const
 // 64-lane sums go through a per-wave LDS transpose; segments of 64 or 128
 // rows are finished here, longer ones leave one partial per 128-row chunk
 // for k_mv_finish.  Every lane issues its KG 16-byte loads back to back.
-// The raw sum over segment s goes to w at segment s ^ 1 (the rows it is
-// expanded into); the expansion applies alpha and sig.
+// The sum over segment s times the sig of its block goes to w at segment
+// s ^ 1 (the rows it is expanded into); the expansion applies alpha.
 __global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__restrict__ x,
                                                     int ldx, double *__restrict__ part,
                                                     size_t pstride, double *__restrict__ w,
@@ -263,14 +264,15 @@ __global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__re
         sq += __shfl_xor(sq, 16, 64); // 64 rows
         if (seg == 64) {
             const int s = row0 / 64 + (qt >> 1);
-            if (ok && (qt & 1) == 0) w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq;
+            if (ok && (qt & 1) == 0)
+                w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq * a.S[dl][(size_t)(a.trans ? s : s ^ 1) * R + kc];
             return;
         }
         sq += __shfl_xor(sq, 32, 64); // 128 rows
         if (!ok || qt != 0) return;
         const int s = row0 / seg;
         if (seg == 128) {
-            w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq;
+            w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq * a.S[dl][(size_t)(a.trans ? s : s ^ 1) * R + kc];
         } else {
             const int tile = (row0 - s * seg) / 128;
             part[a.part_off[dl] + ((size_t)s * a.tps[dl] + tile) * R + kc] = sq;
@@ -338,7 +340,8 @@ __global__ __launch_bounds__(256) void k_mv_finish(Mv2Tab a, const double *__res
     if (ph != 0 || !ok) return;
     const double t = ((red[0][el] + red[1][el]) + (red[2][el] + red[3][el])) +
                      ((red[4][el] + red[5][el]) + (red[6][el] + red[7][el]));
-    w[(size_t)col * wstride + a.w_off[d] + (size_t)(s ^ 1) * R + k] = t;
+    w[(size_t)col * wstride + a.w_off[d] + (size_t)(s ^ 1) * R + k] =
+        t * a.S[d][(size_t)(a.trans ? s : s ^ 1) * R + k];
 }
 
 // Expansion: work-group = 128 rows x 4 rank quarters; lane l holds rows
@@ -359,6 +362,52 @@ __global__ __launch_bounds__(256) void k_mv_expand2(Mv2Tab a, int b, const doubl
     const int N = a.N;
     const int row0 = blockIdx.x * 128;
     const int i = row0 + 2 * lane;
+    double2 lr = {0.0, 0.0}; // sum over the levels, before alpha
+    // The levels are walked as chunks of 32 ranks (wave g takes ranks
+    // k0 + g + 4u), with the buffers in rotation (a register copy, or a load
+    // under a branch that rejoins, would wait for the prefetch).
+    auto locate = [&](int ci, int &dl, int &kl) {
+        int cb = 0;
+        dl = 0;
+        while (ci >= cb + (a.R[dl] + 31) / 32) cb += (a.R[dl++] + 31) / 32;
+        kl = (ci - cb) * 32;
+    };
+    // (w of segment s already carries the sig of the block whose U, V when
+    // transposed, lives on s; alpha is applied at the end)
+    auto load = [&](int dl, int kl, d2v *qv, double *wr) {
+        const int R = a.R[dl], seg = a.seg[dl];
+        const bool uni = (seg & 127) == 0;
+        const int s = uni ? __builtin_amdgcn_readfirstlane(row0 / seg) : i / seg;
+        const double *__restrict__ Q = a.Q[dl];
+        const double *__restrict__ wd = w + a.w_off[dl] + (size_t)s * R;
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const int k = min(kl + g + 4 * u, R - 1);
+            qv[u] = __builtin_nontemporal_load((const d2v *)(Q + (size_t)k * N + i));
+            wr[u] = wd[k];
+        }
+    };
+    auto comp = [&](int dl, int kl, const d2v *qv, const double *wr) {
+        const int R = a.R[dl];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            const double c = wr[u] * (double)(kl + g + 4 * u < R);
+            lr.x += qv[u].x * c;
+            lr.y += qv[u].y * c;
+        }
+    };
+    // three chunk buffers in rotation; the first two chunks' loads go out
+    // before the leaf product, so they are in flight while the leaf waits on
+    // its own (chunk 1 is clamped to chunk 0 when there is only one)
+    d2v q0[8], q1[8], q2[8];
+    double w0[8], w1[8], w2[8];
+    int d0 = 0, k0 = 0, d1 = 0, k1 = 0, d2, k2;
+    if (a.nch > 0) {
+        locate(0, d0, k0);
+        load(d0, k0, q0, w0);
+        locate(min(1, a.nch - 1), d1, k1);
+        load(d1, k1, q1, w1);
+    }
     double2 acc = {0.0, 0.0};
     if (D) {
         const int leaf = i / b, il = i - leaf * b;
@@ -366,19 +415,21 @@ __global__ __launch_bounds__(256) void k_mv_expand2(Mv2Tab a, int b, const doubl
         const double *__restrict__ xl = x + (size_t)leaf * b;
         double2 sl = {0.0, 0.0};
         if (!trans) {
-            for (int j0 = g; j0 < b; j0 += 32) {
-                double2 dv[8];
-                double xv[8];
+            // sixteen leaf columns per wave in flight at once
+            for (int j0 = g; j0 < b; j0 += 64) {
+                double2 dv[16];
+                double xv[16];
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
+                for (int u = 0; u < 16; ++u) {
                     const int j = min(j0 + 4 * u, b - 1);
                     dv[u] = *(const double2 *)(Dl + (size_t)j * b + il);
-                    xv[u] = (j0 + 4 * u < b) ? xl[j] : 0.0;
+                    xv[u] = xl[j];
                 }
 #pragma unroll
-                for (int u = 0; u < 8; ++u) {
-                    sl.x += dv[u].x * xv[u];
-                    sl.y += dv[u].y * xv[u];
+                for (int u = 0; u < 16; ++u) {
+                    const double xu = j0 + 4 * u < b ? xv[u] : 0.0;
+                    sl.x += dv[u].x * xu;
+                    sl.y += dv[u].y * xu;
                 }
             }
         } else {
@@ -390,64 +441,34 @@ __global__ __launch_bounds__(256) void k_mv_expand2(Mv2Tab a, int b, const doubl
         acc.x = alpha * sl.x;
         acc.y = alpha * sl.y;
     }
-    double2 lr = {0.0, 0.0}; // sum over the levels, before alpha
-    // The levels are walked as chunks of 32 ranks (wave g takes ranks
-    // k0 + g + 4u); the loads of the next chunk are issued before the current
-    // one is used, with two buffers alternating (a register copy, or a load
-    // under a branch that rejoins, would wait for the prefetch).
-    auto locate = [&](int ci, int &dl, int &kl) {
-        int cb = 0;
-        dl = 0;
-        while (ci >= cb + (a.R[dl] + 31) / 32) cb += (a.R[dl++] + 31) / 32;
-        kl = (ci - cb) * 32;
-    };
-    auto load = [&](int dl, int kl, d2v *qv, double *wr, double *sr) {
-        const int R = a.R[dl], seg = a.seg[dl];
-        const bool uni = (seg & 127) == 0;
-        const int s = uni ? __builtin_amdgcn_readfirstlane(row0 / seg) : i / seg;
-        const double *__restrict__ Q = a.Q[dl];
-        const double *__restrict__ wd = w + a.w_off[dl] + (size_t)s * R;
-        // coefficient of rows in segment s: alpha * sig of the block whose
-        // U (V when transposed) lives on s, times the raw sum
-        const double *__restrict__ sd = a.S[dl] + (size_t)(trans ? (s ^ 1) : s) * R;
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int k = min(kl + g + 4 * u, R - 1);
-            qv[u] = __builtin_nontemporal_load((const d2v *)(Q + (size_t)k * N + i));
-            wr[u] = wd[k];
-            sr[u] = sd[k];
-        }
-    };
-    auto comp = [&](int dl, int kl, const d2v *qv, const double *wr, const double *sr) {
-        const int R = a.R[dl];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const double c = wr[u] * sr[u] * (double)(kl + g + 4 * u < R);
-            lr.x += qv[u].x * c;
-            lr.y += qv[u].y * c;
-        }
-    };
+    // (the loads of chunk it + 2 are issued before chunk it is used; a
+    // branch either exits the loop or holds no load)
     if (a.nch > 0) {
-        d2v q0[8], q1[8];
-        double w0[8], w1[8], s0[8], s1[8];
-        int d0, k0, d1, k1;
-        locate(0, d0, k0);
-        load(d0, k0, q0, w0, s0);
-        for (int it = 0;; it += 2) {
-            if (it + 1 >= a.nch) {
-                comp(d0, k0, q0, w0, s0);
-                break;
-            }
-            locate(it + 1, d1, k1);
-            load(d1, k1, q1, w1, s1);
-            comp(d0, k0, q0, w0, s0);
+        for (int it = 0;; it += 3) {
             if (it + 2 >= a.nch) {
-                comp(d1, k1, q1, w1, s1);
+                comp(d0, k0, q0, w0);
+                if (it + 1 < a.nch) comp(d1, k1, q1, w1);
                 break;
             }
-            locate(it + 2, d0, k0);
-            load(d0, k0, q0, w0, s0);
-            comp(d1, k1, q1, w1, s1);
+            locate(it + 2, d2, k2);
+            load(d2, k2, q2, w2);
+            comp(d0, k0, q0, w0);
+            if (it + 3 >= a.nch) {
+                comp(d1, k1, q1, w1);
+                comp(d2, k2, q2, w2);
+                break;
+            }
+            locate(it + 3, d0, k0);
+            load(d0, k0, q0, w0);
+            comp(d1, k1, q1, w1);
+            if (it + 4 >= a.nch) {
+                comp(d2, k2, q2, w2);
+                comp(d0, k0, q0, w0);
+                break;
+            }
+            locate(it + 4, d1, k1);
+            load(d1, k1, q1, w1);
+            comp(d2, k2, q2, w2);
         }
     }
     acc.x += alpha * lr.x;
@@ -473,6 +494,7 @@ static void mv2_tab(const dplrh_plan *p, int trans, Mv2Tab *t)
     memset(t, 0, sizeof(*t));
     t->L = p->L;
     t->N = p->N;
+    t->trans = trans;
     int nb = 0, part = 0, fin = 0;
     for (int d = 0; d < p->L; ++d) {
         const int seg = p->N >> (d + 1), nseg = 2 << d, R = p->R[d];
