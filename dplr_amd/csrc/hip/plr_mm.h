@@ -920,8 +920,9 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
 // two 128-row blocks.
 static constexpr int ME3_RB = 256, ME3_NW = 8, ME3_CS = 65;
 
-// LDS doubles of the third form for `rows` staged ranks (W^T rows + sig).
-__host__ __device__ inline size_t me3_lds(int rows) { return (size_t)rows * (ME3_CS + 2); }
+// LDS doubles of the third form for `rows` staged ranks (W^T rows, sig,
+// row offsets and sig addresses).
+__host__ __device__ inline size_t me3_lds(int rows) { return (size_t)rows * (ME3_CS + 3); }
 
 template <int TR>
 __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
@@ -944,10 +945,12 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
     while (dsh < a.L && a.seglen[dsh] >= ME3_RB) rows += a.R[dsh++];
     double *Ws = lds;                    // [rows][ME3_CS]: W^T[c0 + c][k] at Ws[k * CS + c]
     double *Ss = lds + (size_t)rows * ME3_CS; // sig of the staged ranks
-    // element offset of each staged row in w; the level loop is uniform, so
-    // the table entries come from scalar loads (a per-lane level search
-    // would read the kernel arguments through dependent vector loads)
+    // element offset in w and sig address of each staged row; the level
+    // loop is uniform, so the table entries come from scalar loads (a
+    // per-lane level search would read the kernel arguments through
+    // dependent vector loads), and the loop itself loads nothing
     long long *ro = (long long *)(Ss + rows);
+    const double **sp = (const double **)(ro + rows);
     {
         int row0 = 0;
         for (int d = 0; d < dsh; ++d) {
@@ -955,30 +958,43 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
             const double *Sd = a.S[d] + (size_t)(TR ? (s ^ 1) : s) * R;
             for (int k = threadIdx.x; k < R; k += 64 * ME3_NW) {
                 ro[row0 + k] = ((long long)a.w_off[d] + (long long)s * R + k) * wtc + c0;
-                Ss[row0 + k] = a.nosig ? 1.0 : Sd[k];
+                sp[row0 + k] = Sd + k;
             }
             row0 += R;
         }
     }
     __syncthreads();
+    // the sig of row tid goes out with the first batch (a valid dummy
+    // address when there is nothing to stage)
+    const bool sgo = rows > 0 && !a.nosig;
+    const double *spt = sp[max(min((int)threadIdx.x, rows - 1), 0)];
+    const double sgl = *(sgo ? spt : w);
     // eight pairs per thread in flight at once (a loop of single loads and
-    // stores would wait on memory once per pair)
-    for (int e0 = 0; e0 < rows * 32; e0 += 8 * 64 * ME3_NW) {
+    // stores would wait on memory once per pair); each work-group starts at
+    // another row, so that the many work-groups sharing a top-level segment
+    // do not all ask for the same lines at the same moment
+    const int ne = rows * 32, rot = rows > 0 ? (bx * 37 % rows) * 32 : 0;
+    for (int e0 = 0; e0 < ne; e0 += 8 * 64 * ME3_NW) {
         double2 v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int e = min(e0 + u * 64 * ME3_NW + (int)threadIdx.x, rows * 32 - 1);
+            int e = min(e0 + u * 64 * ME3_NW + (int)threadIdx.x, ne - 1) + rot;
+            e -= e >= ne ? ne : 0;
             v[u] = *(const double2 *)(w + ro[e >> 5] + 2 * (e & 31));
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
-            const int e = e0 + u * 64 * ME3_NW + (int)threadIdx.x;
-            if (e < rows * 32) {
+            int e = e0 + u * 64 * ME3_NW + (int)threadIdx.x;
+            if (e < ne) {
+                e += rot;
+                e -= e >= ne ? ne : 0;
                 Ws[(e >> 5) * ME3_CS + 2 * (e & 31)] = v[u].x;
                 Ws[(e >> 5) * ME3_CS + 2 * (e & 31) + 1] = v[u].y;
             }
         }
     }
+    for (int r = threadIdx.x; r < rows; r += 64 * ME3_NW)
+        Ss[r] = a.nosig ? 1.0 : (r == (int)threadIdx.x ? sgl : *sp[r]);
     __syncthreads();
 
     d4 acc[2][4];
