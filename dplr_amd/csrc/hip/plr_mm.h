@@ -917,8 +917,10 @@ __device__ __forceinline__ void mm_expand2_body(const MmTab &a, int bx, int by,
 // with its own two register buffers: the leaf (A = X^T, B = op(D)^T, from
 // global memory), the shared levels (A from LDS, B = Q from global memory),
 // the other levels (as in the second form).  bmax covers the work-group's
-// two 128-row blocks.
-static constexpr int ME3_RB = 256, ME3_NW = 8, ME3_CS = 65;
+// two 128-row blocks.  A staged row is 66 doubles (16-byte aligned), so a
+// lane reads its two adjacent columns as one 16-byte LDS read, and the 16
+// lanes of a group read 64 consecutive dwords (every bank once).
+static constexpr int ME3_RB = 256, ME3_NW = 8, ME3_CS = 66;
 
 // LDS doubles of the third form for `rows` staged ranks (W^T rows, sig,
 // row offsets and sig addresses).
@@ -988,8 +990,7 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
             if (e < ne) {
                 e += rot;
                 e -= e >= ne ? ne : 0;
-                Ws[(e >> 5) * ME3_CS + 2 * (e & 31)] = v[u].x;
-                Ws[(e >> 5) * ME3_CS + 2 * (e & 31) + 1] = v[u].y;
+                *(double2 *)(Ws + (e >> 5) * ME3_CS + 2 * (e & 31)) = v[u];
             }
         }
     }
@@ -1096,7 +1097,8 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
                 const double *wr = Ws + (size_t)row * ME3_CS + 2 * m;
 #pragma unroll
                 for (int cp = 0; cp < 2; ++cp) {
-                    const double a0 = wr[32 * cp], a1 = wr[32 * cp + 1];
+                    const double2 a01 = *(const double2 *)(wr + 32 * cp);
+                    const double a0 = a01.x, a1 = a01.y;
 #pragma unroll
                     for (int rt = 0; rt < 2; ++rt) {
                         acc[rt][2 * cp] = mfma64(a0, bv[rt], acc[rt][2 * cp]);
