@@ -967,10 +967,10 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
     }
     __syncthreads();
     // the sig of row tid goes out with the first batch (a valid dummy
-    // address when there is nothing to stage)
-    const bool sgo = rows > 0 && !a.nosig;
-    const double *spt = sp[max(min((int)threadIdx.x, rows - 1), 0)];
-    const double sgl = *(sgo ? spt : w);
+    // address when there is nothing to stage; the table is not read then)
+    const double *spt = w;
+    if (rows > 0 && !a.nosig) spt = sp[min((int)threadIdx.x, rows - 1)];
+    const double sgl = *spt;
     // eight pairs per thread in flight at once (a loop of single loads and
     // stores would wait on memory once per pair); each work-group starts at
     // another row, so that the many work-groups sharing a top-level segment

@@ -122,6 +122,8 @@ def test_full_size_matvec(cuda, orc):
     (8192, 7, 64, 40),    # rank 64: 28 rank tiles, two passes over the staged X
     (2048, 6, 16, 8),     # leaf 32: the general path
     (3072, 5, 16, 8),     # leaf 96: the general path
+    (256, 2, 8, 5),       # no level spans a 256-row group: nothing staged
+    (512, 3, 12, 7),      # one staged level of 12 ranks
 ])
 @pytest.mark.parametrize("trans", ["N", "T"])
 def test_multi_rhs_product_fused_path(cuda, orc, m, nlvl, rank, n, trans):
