@@ -306,9 +306,10 @@ __global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__re
     }
 }
 
-// Sum the tile partials of long segments (fixed order): a work-group covers
-// 32 consecutive (level, segment, rank) entries with eight tile phases each
-// (eight loads in flight per thread), combined through LDS.
+// Sum the tile partials of long segments (fixed order) and scale the sum by
+// the sig of its block: a work-group covers 32 consecutive (level, segment,
+// rank) entries with eight tile phases each (eight loads in flight per
+// thread), combined through LDS.
 __global__ __launch_bounds__(256) void k_mv_finish(Mv2Tab a, const double *__restrict__ part,
                                                    size_t pstride, double *__restrict__ w,
                                                    size_t wstride)
