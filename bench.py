@@ -491,6 +491,14 @@ def main():
         "incl_factor": {"mixed_rhs_s": nrhs / ((ms_step + lp_ms) * 1e-3),
                         "plain_fp64_rhs_s": nrhs / ((plain_ms + f64_ms) * 1e-3)},
         "apply_fp64_tflops": flops / (ms_apply * 1e-3) / 1e12,
+        # the step's FP64 products (one apply, two refinement residuals) on
+        # the FP64 MFMA roofline; the matvec above is the HBM-bound kernel
+        "roofline_apply": {"bound": "mfma", "achieved": flops / (ms_apply * 1e-3) / 1e12,
+                           "peak": FP64_PEAK_TFS, "unit": "TFLOP/s",
+                           "frac": flops / (ms_apply * 1e-3) / 1e12 / FP64_PEAK_TFS,
+                           "kernel": "64-RHS FP64 product (k_mm_reduce_fused2 + k_mm_finish<true> "
+                                     "+ k_mm_expand3)",
+                           "flops_per_launch": flops, "us_per_launch": ms_apply * 1e3},
         "roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": mv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "single-RHS FP64 matvec (k_mv_reduce2 + k_mv_finish + k_mv_expand2)",
