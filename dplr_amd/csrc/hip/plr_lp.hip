@@ -370,20 +370,28 @@ static constexpr int LP_SCALE_CHUNKS = 64;
 
 __global__ __launch_bounds__(256) void k_lp_colmax(const double *r, int ldr, int N, double *pmax)
 {
-    __shared__ double red[256];
+    __shared__ double red[4];
     const int c = blockIdx.y, ch = blockIdx.x;
     const int len = (N + LP_SCALE_CHUNKS - 1) / LP_SCALE_CHUNKS;
     const int beg = ch * len, end = min(N, beg + len);
     const double *rc = r + (size_t)c * ldr;
     double mv = 0.0;
-    for (int i = beg + threadIdx.x; i < end; i += 256) mv = fmax(mv, fabs(rc[i]));
-    red[threadIdx.x] = mv;
-    __syncthreads();
-    for (int off = 128; off > 0; off >>= 1) {
-        if (threadIdx.x < off) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + off]);
-        __syncthreads();
+    if ((len & 3) == 0 && (N & 3) == 0 && ((uintptr_t)rc & 15) == 0) {
+        // four rows per thread as two 16-byte loads, both in flight at once
+        for (int i = beg + 4 * (int)threadIdx.x; i < end; i += 1024) {
+            const double2 a0 = *(const double2 *)(rc + i), a1 = *(const double2 *)(rc + i + 2);
+            mv = fmax(mv, fmax(fmax(fabs(a0.x), fabs(a0.y)), fmax(fabs(a1.x), fabs(a1.y))));
+        }
+    } else {
+        for (int i = beg + threadIdx.x; i < end; i += 256) mv = fmax(mv, fabs(rc[i]));
     }
-    if (threadIdx.x == 0) pmax[(size_t)c * LP_SCALE_CHUNKS + ch] = red[0];
+    // the wave's maximum, then the four waves' through LDS
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) mv = fmax(mv, __shfl_xor(mv, o, 64));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mv;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        pmax[(size_t)c * LP_SCALE_CHUNKS + ch] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
 __global__ __launch_bounds__(256) void k_lp_scale_rhs(const double *r, int ldr, int N,
