@@ -872,8 +872,10 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
 //              d = 0, x (+)= max|r| * z in FP64.
 // Two kernels per upper level instead of four, and one pass over z.
 
-static constexpr int LT_THREADS = 512; // eight waves
-static constexpr int LT_MAXT = 4;      // 16-row leaf tiles per wave (span <= 512)
+static constexpr int LT_THREADS = 1024;          // sixteen waves
+static constexpr int LT_NW = LT_THREADS / 64;
+static constexpr int LT_MAXT = 512 / 16 / LT_NW; // 16-row leaf tiles per wave (span <= 512)
+static constexpr int LT_ZU = 16 * 512 / LT_THREADS; // 16-byte r loads per thread and round
 static constexpr int LV_RB = 256, LV_CW = 64, LV_THREADS = 512;
 static constexpr int LV_RT = LV_RB / 128; // 16-row tiles per wave
 
@@ -1007,7 +1009,7 @@ __device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, co
 {
     typedef typename Lp<T16>::v8 v8;
     const int nstep = S / 32, spq = seg / 32, total = nstep * nkt * NCT;
-    for (int e0 = wv * 8; e0 < total; e0 += 64) {
+    for (int e0 = wv * 8; e0 < total; e0 += 8 * LT_NW) {
         v8 av[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
@@ -1084,16 +1086,16 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
     }
     {
         const int hs = S / 2, tot = CW * hs;
-        for (int e0 = 0; e0 < tot; e0 += 16 * LT_THREADS) {
-            double2 v[16];
+        for (int e0 = 0; e0 < tot; e0 += LT_ZU * LT_THREADS) {
+            double2 v[LT_ZU];
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < LT_ZU; ++u) {
                 const int e = min(e0 + u * LT_THREADS + tid, tot - 1);
                 const int c = e / hs, i = 2 * (e - c * hs);
                 v[u] = *(const double2 *)(r + (size_t)(c0 + min(c, nc - 1)) * ldr + row0 + i);
             }
 #pragma unroll
-            for (int u = 0; u < 16; ++u) {
+            for (int u = 0; u < LT_ZU; ++u) {
                 const int e = e0 + u * LT_THREADS + tid;
                 if (e < tot) {
                     const int c = e / hs, i = 2 * (e - c * hs);
@@ -1115,7 +1117,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         int i0[LT_MAXT], lf[LT_MAXT];
 #pragma unroll
         for (int q = 0; q < LT_MAXT; ++q) {
-            i0[q] = min(wv + 8 * q, ntile - 1) * 16;
+            i0[q] = min(wv + LT_NW * q, ntile - 1) * 16;
             lf[q] = i0[q] / b;
 #pragma unroll
             for (int ct = 0; ct < NCT; ++ct) acc[q][ct] = f4{0.f, 0.f, 0.f, 0.f};
@@ -1143,7 +1145,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         __syncthreads();
 #pragma unroll
         for (int q = 0; q < LT_MAXT; ++q)
-            if (wv + 8 * q < ntile)
+            if (wv + LT_NW * q < ntile)
 #pragma unroll
                 for (int ct = 0; ct < NCT; ++ct)
                     *(f4 *)(z + (ct * 16 + m) * zs + i0[q] + 4 * g) = acc[q][ct];
@@ -1170,7 +1172,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         // two tasks per wave and pass
         {
             const int n2 = 2 * R, nit = n2 / 16, ntask = (nsg / 2) * nit * NCT;
-            for (int t0 = wv * 2; t0 < ntask; t0 += 16) {
+            for (int t0 = wv * 2; t0 < ntask; t0 += 2 * LT_NW) {
                 const float *k32[2], *lo[2], *hi[2], *sv[2];
                 const double *k64[2];
                 int i0[2], sl[2], ct[2], kk[2];
@@ -1212,7 +1214,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         // batches of eight MFMA steps per wave with their Ut loads together
         {
             const int nks = R / 32, total = (S / 16) * NCT * nks;
-            for (int e0 = wv * 8; e0 < total; e0 += 64) {
+            for (int e0 = wv * 8; e0 < total; e0 += 8 * LT_NW) {
                 v8 bq[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
@@ -1261,7 +1263,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         }
         if (nh > 1) {
             __syncthreads();
-            for (int tt = wv; tt < nkt * NCT; tt += 8) {
+            for (int tt = wv; tt < nkt * NCT; tt += LT_NW) {
                 const int kt = tt / NCT, ct = tt - kt * NCT, c = ct * 16 + m;
                 f4 x = *(const f4 *)(red + ((tt * nh) * 64 + lane) * 4);
                 for (int h = 1; h < nh; ++h) x += *(const f4 *)(red + (((tt * nh + h) * 64) + lane) * 4);
