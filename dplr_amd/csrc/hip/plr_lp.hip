@@ -193,6 +193,48 @@ __global__ __launch_bounds__(256) void k_lp_finish(LpTab a, const float *__restr
     out[(size_t)c * ostride + (size_t)s * R + kg * LP_KG + kl] = t;
 }
 
+// The same sums for a table of one level, the work-group shape following the
+// level's tile count: P = tps/16 tile phases (a power of two, 1..8) times
+// 256/P elements, sixteen loads in flight per thread, so every level is one
+// round trip and a level of few tiles takes few work-groups.
+__host__ __device__ inline int lpf_phases(int tps)
+{
+    int ph = 1;
+    while (ph < 8 && 32 * ph <= tps) ph *= 2;
+    return ph;
+}
+
+__global__ __launch_bounds__(256) void k_lp_finish1(LpTab a, const float *__restrict__ part,
+                                                    float *__restrict__ out, size_t ostride)
+{
+    __shared__ float red[256];
+    const int tps = a.tps[0], P = lpf_phases(tps), E = 256 / P, nb = LP_BLK / E;
+    const int task = blockIdx.x / nb, blk = blockIdx.x % nb;
+    const int nkg = a.nkg[0], ncg = a.ncg, R = a.R[0];
+    const int kg = task % nkg, cg = (task / nkg) % ncg, s = task / (nkg * ncg);
+    const int el = threadIdx.x % E, ph = threadIdx.x / E;
+    const int e = blk * E + el;
+    const float *pp = part + ((size_t)a.part_off[0] + ((size_t)s * tps * nkg + kg) * ncg + cg) * LP_BLK + e;
+    const size_t step = (size_t)nkg * ncg * LP_BLK;
+    float acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    for (int j0 = ph; j0 < tps; j0 += 16 * P) {
+        float v[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) v[u] = pp[(size_t)min(j0 + P * u, tps - 1) * step];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) acc[u & 7] += j0 + P * u < tps ? v[u] : 0.f;
+    }
+    red[threadIdx.x] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
+    __syncthreads();
+    if (ph != 0) return;
+    float t = red[el];
+    for (int q = 1; q < P; ++q) t += red[q * E + el];
+    const int kl = e % LP_KG, cl = e / LP_KG;
+    const int c = cg * LP_CG + cl;
+    if (c >= a.ncols) return;
+    out[(size_t)c * ostride + (size_t)s * R + kg * LP_KG + kl] = t;
+}
+
 // Z^T (64 columns x 16 rows per wavefront): Z := beta*Z + Dinv X (leaf, when
 // Dl != null) + sum over the table's levels of Q_d (wscale .* W_d).
 template <typename T16>
@@ -1633,8 +1675,8 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
         if (d > 0 && lt.Rn > 0) {
             LpTab t;
             lp_tab(p, q, d - 1, d, n, &t, LV_RB);
-            hipLaunchKernelGGL(k_lp_finish, dim3(t.fin_off[1] * LP_FCH), dim3(256), 0, st, t, q->part,
-                               q->T, q->t_per_col);
+            hipLaunchKernelGGL(k_lp_finish1, dim3(t.fin_off[1] * (LP_BLK / (256 / lpf_phases(t.tps[0])))),
+                               dim3(256), 0, st, t, q->part, q->T, q->t_per_col);
             PLR_HIP(hipGetLastError());
         }
     }
