@@ -70,6 +70,7 @@ def test_refined_residual_report(cuda):
     (8192, 6, 32, 16, "bf16"),   # leaf 128
     (8192, 7, 64, 9, "f16"),     # rank 64: narrower bottom kernel
     (4096, 6, 80, 4, "f16"),     # rank above 64: the per-level kernels
+    (32768, 9, 32, 8, "f16"),    # 64 and 32 tiles per top segment: 4 and 2 finishing phases
 ])
 def test_refined_solve_shapes(cuda, m, nlvl, rank, n, dtype):
     import torch
