@@ -907,11 +907,11 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
 //              z = r / max|r| per column, z|_leaf = Dinv z|_leaf, then every
 //              level d >= d0 (segments inside the span) with z in LDS, and
 //              finally T of level d0 - 1 (the span is one of its segments).
-//   k_lp_level one work-group per (128 rows, 64 right-hand sides) of level
-//              d < d0: the rows of inv(K) for its segment (recomputed per
-//              work-group), z += Ut_d W_d, then the partial sums of
-//              V_{d-1}^T z over its rows (summed by k_lp_finish), or, at
-//              d = 0, x (+)= max|r| * z in FP64.
+//   k_lp_level one work-group per (LV_RB = 256 rows, 64 right-hand sides)
+//              of level d < d0: the rows of inv(K) for its segment
+//              (recomputed per work-group), z += Ut_d W_d, then the partial
+//              sums of V_{d-1}^T z over its rows (summed by k_lp_finish1), or,
+//              at d = 0, x (+)= max|r| * z in FP64.
 // Two kernels per upper level instead of four, and one pass over z.
 
 static constexpr int LT_THREADS = 1024;          // sixteen waves
@@ -1506,8 +1506,8 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
     }
     if (a.Rn == 0) return;
     __syncthreads();
-    // 4. partial V_{d-1}^T z over the work-group's rows, in k_lp_finish's
-    // layout (tiles of LV_RB rows of the segment of level d-1)
+    // 4. partial V_{d-1}^T z over the work-group's rows, in the finishing
+    // passes' layout (tiles of LV_RB rows of the segment of level d-1)
     {
         const int sn = r0 / a.segn, tile = (r0 - sn * a.segn) / LV_RB, st = sn * a.tps + tile;
 #pragma unroll
