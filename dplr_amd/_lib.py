@@ -43,6 +43,8 @@ def _load():
         "dplrx_solve": (c_int, [P, c_char, c_int, P, c_int, P, c_int, P]),
         "dplrx_factor_mixed": (c_int, [P, c_int, POINTER(c_int), P]),
         "dplrx_solve_refined": (c_int, [P, c_int, P, c_int, P, c_int, c_int, PD, P]),
+        "dplrx_solve_refined_tol": (c_int, [P, c_int, P, c_int, P, c_int, c_double, c_int,
+                                            POINTER(c_int), PD, P]),
         "dplrx_grad_outer": (c_int, [P, c_int, c_double, P, c_int, P, c_int, P, P, P, P, P]),
         "dplrx_batch_create": (c_int, [c_int, POINTER(P), POINTER(P)]),
         "dplrx_batch_destroy": (None, [P]),

@@ -94,6 +94,10 @@ int dplrh_solve(dplrh_plan *p, int trans, int n, const double *dB, int ldb, doub
 int dplrh_lp_factor(dplrh_plan *p, int type, int *info, void *stream);
 int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, double *dX, int ldx,
                    int iters, double *res, void *stream);
+/* The refined solve to an error estimate <= tol (at most maxit >= 2 solves);
+ * *iters and *err (optional) receive the solves made and the estimate. */
+int dplrh_lp_solve_tol(dplrh_plan *p, int n, const double *dB, int ldb, double *dX, int ldx,
+                       double tol, int maxit, int *iters, double *err, void *stream);
 
 /* Explicit inverse of a factored plan, downloaded per level d' into
  * hBU[d'], hBV[d'] (N*Rt[d'] doubles, [segment][column][row]) and hBsig[d']

@@ -436,29 +436,72 @@ __global__ __launch_bounds__(256) void k_lp_colmax(const double *r, int ldr, int
         pmax[(size_t)c * LP_SCALE_CHUNKS + ch] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// pm: pm_n partial maxima of |r| per column (column stride pm_n).
 __global__ __launch_bounds__(256) void k_lp_scale_rhs(const double *r, int ldr, int N,
-                                                      const double *pmax, double *scale, float *r32)
+                                                      const double *pm, int pm_n, double *scale,
+                                                      float *r32)
 {
     const int c = blockIdx.y;
     double mv = 0.0;
-    for (int j = 0; j < LP_SCALE_CHUNKS; ++j) mv = fmax(mv, pmax[(size_t)c * LP_SCALE_CHUNKS + j]);
+    for (int j = 0; j < pm_n; ++j) mv = fmax(mv, pm[(size_t)c * pm_n + j]);
     if (blockIdx.x == 0 && threadIdx.x == 0) scale[c] = mv;
     const double inv = 1.0 / (mv > 0.0 ? mv : 1.0);
     const int i = blockIdx.x * 256 + threadIdx.x;
     if (i < N) r32[(size_t)c * N + i] = (float)(r[(size_t)c * ldr + i] * inv);
 }
 
-// x[c] (+)= scale[c] * z[c]  (FP64 accumulation of the correction; the
-// first correction sets x)
-__global__ void k_lp_update(const float *z, int N, const double *scale, double *x, int ldx, int n,
-                            int first)
+// out[c] = max_j pm[c * pm_n + j]
+__global__ void k_lp_pmfold(const double *pm, int pm_n, int n, double *out)
 {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)N * n) return;
-    const int c = (int)(e / N), i = (int)(e % N);
-    const double sc = scale[c] > 0.0 ? scale[c] : 1.0;
-    double *xe = x + (size_t)c * ldx + i;
-    *xe = (first ? 0.0 : *xe) + sc * (double)z[e];
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    double mv = 0.0;
+    for (int j = 0; j < pm_n; ++j) mv = fmax(mv, pm[(size_t)c * pm_n + j]);
+    out[c] = mv;
+}
+
+// Norms of a correction step, for the refinement's error estimate: per
+// column the sums of dx^2 and of x^2 after the update (FP64).  Kernels add
+// their work-group's sums into copy (blockIdx.x % LP_NRM_COPIES) of
+// nrm[copy][column][2], spreading the atomics over several cache lines; the
+// host adds the copies.  NaN and infinity propagate through the sums.
+static constexpr int LP_NRM_COPIES = 16;
+
+__global__ void k_lp_fill0(double *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0.0;
+}
+
+// x[c] (+)= scale[c] * z[c]  (FP64 accumulation of the correction; the
+// first correction sets x); grid (row blocks of 256, columns).
+__global__ __launch_bounds__(256) void k_lp_update(const float *z, int N, const double *scale,
+                                                   double *x, int ldx, int n, int first,
+                                                   double *nrm)
+{
+    const int c = blockIdx.y, i = blockIdx.x * 256 + threadIdx.x;
+    double dd = 0.0, xx = 0.0;
+    if (i < N) {
+        const double sc = scale[c] > 0.0 ? scale[c] : 1.0;
+        double *xe = x + (size_t)c * ldx + i;
+        const double d = sc * (double)z[(size_t)c * N + i];
+        const double xn = (first ? 0.0 : *xe) + d;
+        *xe = xn;
+        dd = d * d;
+        xx = xn * xn;
+    }
+    if (nrm) {
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            dd += __shfl_xor(dd, o, 64);
+            xx += __shfl_xor(xx, o, 64);
+        }
+        if ((threadIdx.x & 63) == 0) {
+            double *q = nrm + ((size_t)(blockIdx.x % LP_NRM_COPIES) * n + c) * 2;
+            atomicAdd(q, dd);
+            atomicAdd(q + 1, xx);
+        }
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -482,15 +525,18 @@ struct plr_lp
     size_t part_cap;
     double *r, *scale, *pmax;
     double *bmax; // maxima of |r| per (column, 128-row block) from the residual product
+    double *nrm;      // norms of a correction step (lp_correct), LP_NRM_COPIES x cols x 2
+    double *nrm_host; // pinned host copy of nrm
 };
 
 static void lp_free(plr_lp *q)
 {
     if (!q) return;
     void *b[] = {q->Vh, q->Uh, q->Dh, q->Sf, q->K64, q->K32, q->T, q->W, q->wscale, q->part,
-                 q->r32, q->z32, q->r, q->scale, q->pmax, q->bmax};
+                 q->r32, q->z32, q->r, q->scale, q->pmax, q->bmax, q->nrm};
     for (void *x : b)
         if (x) (void)hipFree(x);
+    if (q->nrm_host) (void)hipHostFree(q->nrm_host);
     delete q;
 }
 
@@ -568,11 +614,13 @@ static int lp_ws(const dplrh_plan *p, plr_lp *q, int cols)
 {
     if (cols <= q->cols) return 0;
     ++const_cast<dplrh_plan *>(p)->ws_gen;
-    void *old[] = {q->T, q->W, q->wscale, q->r32, q->z32, q->r, q->scale, q->pmax};
+    void *old[] = {q->T, q->W, q->wscale, q->r32, q->z32, q->r, q->scale, q->pmax, q->nrm};
     for (void *x : old)
         if (x) (void)hipFree(x);
+    if (q->nrm_host) (void)hipHostFree(q->nrm_host);
     q->T = q->W = q->wscale = q->r32 = q->z32 = nullptr;
     q->r = q->scale = q->pmax = nullptr;
+    q->nrm = q->nrm_host = nullptr;
     q->cols = 0;
     size_t t = 32, s = 2;
     for (int d = 0; d < p->L; ++d) {
@@ -590,6 +638,9 @@ static int lp_ws(const dplrh_plan *p, plr_lp *q, int cols)
     PLR_HIP(hipMalloc(&q->r, (size_t)p->N * cols * sizeof(double)));
     PLR_HIP(hipMalloc(&q->scale, (size_t)cols * sizeof(double)));
     PLR_HIP(hipMalloc(&q->pmax, (size_t)cols * LP_SCALE_CHUNKS * sizeof(double)));
+    PLR_HIP(hipMalloc(&q->nrm, (size_t)LP_NRM_COPIES * 2 * cols * sizeof(double)));
+    PLR_HIP(hipHostMalloc((void **)&q->nrm_host, (size_t)LP_NRM_COPIES * 2 * cols * sizeof(double),
+                          hipHostMallocDefault));
     q->cols = cols;
     return 0;
 }
@@ -810,83 +861,186 @@ static int lp_solve_t(const dplrh_plan *p, plr_lp *q, int n, hipStream_t st)
 }
 
 static int lp_solve_v2_any(const dplrh_plan *p, plr_lp *q, int n, int first, double *X, int ldx,
-                           const double *r, int ldr, const double *pm, int pm_n, hipStream_t st);
+                           const double *r, int ldr, const double *pm, int pm_n, double *nrm,
+                           hipStream_t st);
 static bool lp_v2_ok(const dplrh_plan *p, const plr_lp *q);
 
-// X := A^{-1} B by mixed-precision solves refined with FP64 residuals:
-// `iters` solves (iters-1 residuals).  When res != NULL, res[c] receives
-// max|b - A x| of column c after the last solve (one more FP64 residual).
-//
-// Second form (shapes lp_v2_shape covers, no residual report): the first
-// solve reads B in place; every later residual r = B - A x comes out of one
+// r = B - A X (r = B when X is still zero) and partial maxima of |r| per
+// column (*pm_n of them, column stride *pm_n).  With `direct` the first
+// residual is B itself, read in place, and later ones come out of one fused
 // multi-RHS product that reads B for its beta term and also writes the
-// maxima of |r| per (column, 128-row block), so nothing is copied and no
-// separate maximum pass runs.
-static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int ldb, double *dX,
-                         int ldx, int iters, double *res, hipStream_t st)
+// maxima of |r| per (column, 128-row block); when that product declines the
+// call (an unaligned X, say), or without `direct`, r is B copied, the
+// product subtracted, and the maxima taken by k_lp_colmax.
+static int lp_residual(dplrh_plan *p, plr_lp *q, int nc, const double *B, int ldb,
+                       const double *X, int ldx, bool zero_x, bool direct, const double **r,
+                       int *ldr, const double **pm, int *pm_n, hipStream_t st)
 {
     const int N = p->N;
-    if (N == 0 || n == 0) return 0;
+    if (direct) {
+        if (zero_x) {
+            hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, B, ldb, N,
+                               q->pmax);
+            PLR_HIP(hipGetLastError());
+            *r = B;
+            *ldr = ldb;
+            *pm = q->pmax;
+            *pm_n = LP_SCALE_CHUNKS;
+            return 0;
+        }
+        const int rc = plr_mm_apply_c(p, 0, nc, -1.0, 1.0, X, ldx, B, ldb, q->r, N, p->w,
+                                      p->w_per_col, q->bmax, st);
+        if (rc < 0) return rc;
+        if (rc == 0) {
+            *r = q->r;
+            *ldr = N;
+            *pm = q->bmax;
+            *pm_n = N / 128;
+            return 0;
+        }
+    }
+    PLR_HIP(hipMemcpy2DAsync(q->r, (size_t)N * sizeof(double), B, (size_t)ldb * sizeof(double),
+                             (size_t)N * sizeof(double), nc, hipMemcpyDeviceToDevice, st));
+    if (!zero_x) {
+        const int rc = plr_gemm_direct(p, 0, nc, -1.0, 1.0, X, ldx, q->r, N, st);
+        if (rc) return rc < 0 ? rc : -1;
+    }
+    hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, q->r, N, N,
+                       q->pmax);
+    PLR_HIP(hipGetLastError());
+    *r = q->r;
+    *ldr = N;
+    *pm = q->pmax;
+    *pm_n = LP_SCALE_CHUNKS;
+    return 0;
+}
+
+// X = (first ? 0 : X) + max|r| * solve_lp(r / max|r|), one low-precision
+// correction; with nrm, the step's norms (largest |dx| and |x| per column)
+// are left in nrm.
+static int lp_correct(dplrh_plan *p, plr_lp *q, int nc, int first, double *X, int ldx,
+                      const double *r, int ldr, const double *pm, int pm_n, double *nrm,
+                      hipStream_t st)
+{
+    const int N = p->N;
+    if (nrm) {
+        const int nn = LP_NRM_COPIES * 2 * nc;
+        hipLaunchKernelGGL(k_lp_fill0, dim3((nn + 255) / 256), dim3(256), 0, st, nrm, nn);
+        PLR_HIP(hipGetLastError());
+    }
+    int rc = lp_solve_v2_any(p, q, nc, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
+    if (rc <= 0) return rc;
+    // first form: the per-level kernels on r / max|r| in FP32
+    hipLaunchKernelGGL(k_lp_scale_rhs, dim3((N + 255) / 256, nc), dim3(256), 0, st, r, ldr, N, pm,
+                       pm_n, q->scale, q->r32);
+    PLR_HIP(hipGetLastError());
+    rc = q->type == 1 ? lp_solve_t<_Float16>(p, q, nc, st) : lp_solve_t<__bf16>(p, q, nc, st);
+    if (rc) return rc;
+    hipLaunchKernelGGL(k_lp_update, dim3((N + 255) / 256, nc), dim3(256), 0, st, q->z32, N,
+                       q->scale, X, ldx, nc, first, nrm);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+// Error estimate of a refinement step from its correction d_k (k >= 1),
+// normwise over the chunk's columns: delta_k = |D_k|_F / |X_k|_F is the
+// relative size of the correction, an estimate of the error of X_{k-1}; the
+// observed contraction rho_k = delta_k / delta_{k-1} (delta_0 = 1: X_0 is
+// the whole first correction) then estimates the relative error of X_k as
+// LP_EST_MARGIN * delta_k * min(1, rho_k).  The margin covers the growth of
+// the contraction from one step to the next as the error concentrates in
+// the directions the low-precision solve contracts least (measured on the
+// seeded random family: 9-13% per step).  NaN propagates.  Reads the sums
+// left by lp_correct (synchronises st).
+static constexpr double LP_EST_MARGIN = 1.25;
+static int lp_estimate(plr_lp *q, int nc, int k, double *dprev, double *est, hipStream_t st)
+{
+    const size_t nn = (size_t)LP_NRM_COPIES * 2 * nc;
+    PLR_HIP(hipMemcpyAsync(q->nrm_host, q->nrm, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    PLR_HIP(hipStreamSynchronize(st));
+    double dd = 0.0, xx = 0.0;
+    for (size_t i = 0; i < nn; i += 2) {
+        dd += q->nrm_host[i];
+        xx += q->nrm_host[i + 1];
+    }
+    const double delta = dd == 0.0 ? 0.0 : sqrt(dd) / sqrt(xx); // xx = 0 < dd: inf
+    double rho = k == 1 ? delta : (*dprev > 0.0 ? delta / *dprev : (delta == 0.0 ? 0.0 : INFINITY));
+    if (rho > 1.0) rho = 1.0;
+    *dprev = delta;
+    *est = LP_EST_MARGIN * delta * rho;
+    return 0;
+}
+
+// X := A^{-1} B by mixed-precision solves refined with FP64 residuals
+// r = B - A X between them (iterative refinement), column chunks of at most
+// 128.  tol < 0: exactly `iters` solves.  tol >= 0: at most `iters` solves,
+// stopping after the first one (from the second on) whose error estimate
+// (lp_estimate, normwise over the chunk) is <= tol; *used and *err
+// (optional) receive the largest number of solves and estimate over the
+// chunks.  res[c] (host, optional) receives
+// max|b - A x| of column c after the last solve (one more FP64 residual).
+static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int ldb, double *dX,
+                         int ldx, int iters, double tol, int *used, double *err, double *res,
+                         hipStream_t st)
+{
+    const int N = p->N;
+    if (used) *used = 0;
+    if (err) *err = INFINITY;
+    if (N == 0 || n == 0) {
+        if (err) *err = 0.0;
+        return 0;
+    }
+    const bool adaptive = tol >= 0.0;
     int rc = 0;
     const int chunk = n < 128 ? n : 128;
     if ((rc = lp_ws(p, q, chunk))) return rc;
-    const bool direct = !res && lp_v2_ok(p, q) && ldb % 2 == 0 && ldx % 2 == 0 &&
-                        ((uintptr_t)dB & 15) == 0 && N % 256 == 0;
+    const bool direct = lp_v2_ok(p, q) && ldb % 2 == 0 && ((uintptr_t)dB & 15) == 0 && N % 256 == 0;
     if (direct) {
         if ((rc = plr_workspace(p, (chunk + 63) & ~63))) return rc;
         if (!q->bmax) PLR_HIP(hipMalloc(&q->bmax, (size_t)128 * (N / 128) * sizeof(double)));
     }
+    int used_max = 0;
+    double est_max = adaptive && iters >= 2 ? 0.0 : INFINITY;
     for (int c0 = 0; c0 < n; c0 += chunk) {
         const int nc = n - c0 < chunk ? n - c0 : chunk;
         const double *B = dB + (size_t)c0 * ldb;
         double *X = dX + (size_t)c0 * ldx;
-        if (direct) {
-            hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, B, ldb, N,
-                               q->pmax);
-            PLR_HIP(hipGetLastError());
-            rc = lp_solve_v2_any(p, q, nc, 1, X, ldx, B, ldb, q->pmax, LP_SCALE_CHUNKS, st);
+        int made = 0;
+        double est = INFINITY, dprev = 1.0;
+        for (int it = 0; it < iters; ++it) {
+            const double *r, *pm;
+            int ldr, pm_n;
+            if ((rc = lp_residual(p, q, nc, B, ldb, X, ldx, it == 0, direct, &r, &ldr, &pm, &pm_n, st)))
+                return rc;
+            const bool track = adaptive && it >= 1;
+            rc = lp_correct(p, q, nc, it == 0, X, ldx, r, ldr, pm, pm_n, track ? q->nrm : nullptr, st);
             if (rc) return rc < 0 ? rc : -1;
-            for (int it = 1; it < iters; ++it) {
-                rc = plr_mm_apply_c(p, 0, nc, -1.0, 1.0, X, ldx, B, ldb, q->r, N, p->w,
-                                    p->w_per_col, q->bmax, st);
-                if (rc) return rc < 0 ? rc : -1;
-                rc = lp_solve_v2_any(p, q, nc, 0, X, ldx, q->r, N, q->bmax, N / 128, st);
-                if (rc) return rc < 0 ? rc : -1;
-            }
-            continue;
+            made = it + 1;
+            if (!track) continue;
+            if ((rc = lp_estimate(q, nc, it, &dprev, &est, st))) return rc;
+            if (est <= tol) break;
         }
-        for (int it = 0; it < iters + (res ? 1 : 0); ++it) {
-            // r = b - A x (x = 0 on the first pass)
-            PLR_HIP(hipMemcpy2DAsync(q->r, (size_t)N * sizeof(double), B, (size_t)ldb * sizeof(double),
-                                     (size_t)N * sizeof(double), nc, hipMemcpyDeviceToDevice, st));
-            if (it > 0 && (rc = plr_gemm_direct(p, 0, nc, -1.0, 1.0, X, ldx, q->r, N, st))) return rc;
-            hipLaunchKernelGGL(k_lp_colmax, dim3(LP_SCALE_CHUNKS, nc), dim3(256), 0, st, q->r, N, N,
-                               q->pmax);
-            if (it < iters) {
-                rc = lp_solve_v2_any(p, q, nc, it == 0, X, ldx, q->r, N, q->pmax, LP_SCALE_CHUNKS, st);
-                if (rc < 0) return rc;
-                if (rc == 0) continue;
-                rc = 0;
-            }
-            hipLaunchKernelGGL(k_lp_scale_rhs, dim3((N + 255) / 256, nc), dim3(256), 0, st, q->r, N,
-                               N, q->pmax, q->scale, q->r32);
-            PLR_HIP(hipGetLastError());
-            if (it == iters) break; // residual of the final iterate only
-            rc = q->type == 1 ? lp_solve_t<_Float16>(p, q, nc, st) : lp_solve_t<__bf16>(p, q, nc, st);
-            if (rc) return rc;
-            const size_t ne = (size_t)N * nc;
-            hipLaunchKernelGGL(k_lp_update, dim3((ne + 255) / 256), dim3(256), 0, st, q->z32, N,
-                               q->scale, X, ldx, nc, it == 0);
-            PLR_HIP(hipGetLastError());
-        }
+        if (made > used_max) used_max = made;
+        if (adaptive && (est > est_max || est != est)) est_max = est;
         if (res) {
+            const double *r, *pm;
+            int ldr, pm_n;
+            if ((rc = lp_residual(p, q, nc, B, ldb, X, ldx, false, direct, &r, &ldr, &pm, &pm_n, st)))
+                return rc;
+            hipLaunchKernelGGL(k_lp_pmfold, dim3((nc + 63) / 64), dim3(64), 0, st, pm, pm_n, nc,
+                               q->scale);
+            PLR_HIP(hipGetLastError());
             PLR_HIP(hipMemcpyAsync(res + c0, q->scale, nc * sizeof(double), hipMemcpyDeviceToHost, st));
             PLR_HIP(hipStreamSynchronize(st));
         }
     }
+    if (used) *used = used_max;
+    if (err) *err = adaptive ? est_max : INFINITY;
     return 0;
 }
 
+// Launched directly: with kernels of 5-100 us the stream stays fed, and
+// measured, the graph's own start cost more than it saved here.
 extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, double *dX,
                               int ldx, int iters, double *res, void *stream)
 {
@@ -895,9 +1049,29 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
         plr_set_error("solve before a successful mixed-precision factorization");
         return -4;
     }
-    // Launched directly: with kernels of 5-100 us the stream stays fed, and
-    // measured, the graph's own start cost more than it saved here.
-    return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, res, (hipStream_t)stream);
+    if (iters < 1) {
+        plr_set_error("refined solve needs at least one iteration (got %d)", iters);
+        return -5;
+    }
+    return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, -1.0, nullptr, nullptr, res,
+                         (hipStream_t)stream);
+}
+
+extern "C" int dplrh_lp_solve_tol(dplrh_plan *p, int n, const double *dB, int ldb, double *dX,
+                                  int ldx, double tol, int maxit, int *iters, double *err,
+                                  void *stream)
+{
+    plr_lp *q = lp_of(p);
+    if (!q) {
+        plr_set_error("solve before a successful mixed-precision factorization");
+        return -4;
+    }
+    if (!(tol >= 0.0) || maxit < 2) {
+        plr_set_error("refined solve: tol must be >= 0 and maxit >= 2 (got %g, %d)", tol, maxit);
+        return -5;
+    }
+    return lp_solve_body(p, q, n, dB, ldb, dX, ldx, maxit, tol, iters, err, nullptr,
+                         (hipStream_t)stream);
 }
 
 // ---------------------------------------------------------------------------
@@ -1339,7 +1513,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
                                                          int ldz, float *__restrict__ part,
                                                          const double *__restrict__ scale,
                                                          double *__restrict__ X, int ldx, int first,
-                                                         int ncols)
+                                                         int ncols, double *__restrict__ nrm)
 {
     typedef typename Lp<T16>::v8 v8;
     constexpr int KS = RM / 32;    // expansion k-steps
@@ -1347,6 +1521,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
     constexpr int NU = LV_RB / 32; // reduction k-steps (rows)
     extern __shared__ float lds[];
     __shared__ unsigned smax[LV_CW];
+    __shared__ double xnrm[2 * LV_CW]; // X pass: sums of dx^2, x^2 per column
     const int R = a.R, n2 = 2 * R, rs = LV_CW + 16, wsk = R + 4, zts = LV_RB + 8;
     float *rhs = lds;                   // [n2][rs]
     float *W = rhs + n2 * rs;           // [CW][wsk]
@@ -1408,6 +1583,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
         }
     }
     if (tid < LV_CW) smax[tid] = 0u;
+    if (tid < 2 * LV_CW) xnrm[tid] = 0.0;
     __syncthreads();
     // 2. W_s = -S_s .* (rows of inv(K) for segment s) rhs, largest |W| per
     // column: tasks (kt, ct), wave wv takes wv + 8p
@@ -1468,6 +1644,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
             }
         }
         // lane holds dz[row i = 16 (LV_RT wv + rt) + 4g + v][column c = 16ct + m]
+        double dmx[4] = {0.0, 0.0, 0.0, 0.0}, xmx[4] = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
         for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
@@ -1485,13 +1662,21 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
 #pragma unroll
                             for (int h = 0; h < 2; ++h) {
                                 double2 o = first ? double2{0.0, 0.0} : *(const double2 *)(xp + 2 * h);
-                                o.x += sr * (double)zn[2 * h];
-                                o.y += sr * (double)zn[2 * h + 1];
+                                const double d0 = sr * (double)zn[2 * h], d1 = sr * (double)zn[2 * h + 1];
+                                o.x += d0;
+                                o.y += d1;
                                 *(double2 *)(xp + 2 * h) = o;
+                                dmx[ct] += d0 * d0 + d1 * d1;
+                                xmx[ct] += o.x * o.x + o.y * o.y;
                             }
                         } else {
 #pragma unroll
-                            for (int v = 0; v < 4; ++v) xp[v] = (first ? 0.0 : xp[v]) + sr * (double)zn[v];
+                            for (int v = 0; v < 4; ++v) {
+                                const double d = sr * (double)zn[v], xn = (first ? 0.0 : xp[v]) + d;
+                                xp[v] = xn;
+                                dmx[ct] += d * d;
+                                xmx[ct] += xn * xn;
+                            }
                         }
                     }
                 } else {
@@ -1503,6 +1688,29 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
                     *(t4 *)(zt + c * zts + i) = zh;
                 }
             }
+        if (X && nrm) { // fold the lane sums: lane groups, then waves, then the copy
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) {
+#pragma unroll
+                for (int o = 16; o < 64; o <<= 1) {
+                    dmx[ct] += __shfl_xor(dmx[ct], o, 64);
+                    xmx[ct] += __shfl_xor(xmx[ct], o, 64);
+                }
+                if (g == 0) {
+                    atomicAdd(&xnrm[ct * 16 + m], dmx[ct]);
+                    atomicAdd(&xnrm[LV_CW + ct * 16 + m], xmx[ct]);
+                }
+            }
+        }
+    }
+    if (X && nrm) {
+        __syncthreads();
+        if (tid < 2 * LV_CW) {
+            const int which = tid / LV_CW, c = tid - which * LV_CW;
+            if (c < nc)
+                atomicAdd(nrm + ((size_t)(blockIdx.x % LP_NRM_COPIES) * ncols + c0 + c) * 2 + which,
+                          xnrm[tid]);
+        }
     }
     if (a.Rn == 0) return;
     __syncthreads();
@@ -1602,7 +1810,7 @@ static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_
 template <typename T16>
 static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, int first,
                        double *X, int ldx, const double *r, int ldr, const double *pm, int pm_n,
-                       hipStream_t st)
+                       double *nrm, hipStream_t st)
 {
     const int N = p->N, L = p->L;
     const T16 *Vh = (const T16 *)q->Vh, *Uh = (const T16 *)q->Uh;
@@ -1632,18 +1840,19 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
                     : lp_launch_tail<T16, 32>(tt, lds, grid, q, n, r, ldr, pm, pm_n, st);
     if (rc) return rc;
     if (v.d0 == 0) {
-        const size_t ne = (size_t)N * n;
-        hipLaunchKernelGGL(k_lp_update, dim3((unsigned)((ne + 255) / 256)), dim3(256), 0, st, q->z32,
-                           N, q->scale, X, ldx, n, first);
+        hipLaunchKernelGGL(k_lp_update, dim3((N + 255) / 256, n), dim3(256), 0, st, q->z32, N,
+                           q->scale, X, ldx, n, first, nrm);
         PLR_HIP(hipGetLastError());
         return 0;
     }
     static bool attr = false;
     if (!attr) {
+        // lpl_lds(64) is 92 KB; the kernel's static LDS (1.25 KB) counts
+        // against the 160 KB too
         PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 32>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
         PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 64>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
+                                    hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
         attr = true;
     }
     const int ncg = (n + LV_CW - 1) / LV_CW;
@@ -1666,11 +1875,11 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
         if (lt.R <= 32 && lt.Rn <= 32)
             hipLaunchKernelGGL((k_lp_level<T16, 32>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
                                lpl_lds(lt.R), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
-                               q->scale, d == 0 ? X : nullptr, ldx, first, n);
+                               q->scale, d == 0 ? X : nullptr, ldx, first, n, nrm);
         else
             hipLaunchKernelGGL((k_lp_level<T16, 64>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
                                lpl_lds(lt.R), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
-                               q->scale, d == 0 ? X : nullptr, ldx, first, n);
+                               q->scale, d == 0 ? X : nullptr, ldx, first, n, nrm);
         PLR_HIP(hipGetLastError());
         if (d > 0 && lt.Rn > 0) {
             LpTab t;
@@ -1684,10 +1893,12 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
 }
 
 static int lp_solve_v2_any(const dplrh_plan *p, plr_lp *q, int n, int first, double *X, int ldx,
-                           const double *r, int ldr, const double *pm, int pm_n, hipStream_t st)
+                           const double *r, int ldr, const double *pm, int pm_n, double *nrm,
+                           hipStream_t st)
 {
     const lp_v2 v = lp_v2_shape(p, q);
     if (!v.ok) return 1; // shape not covered: the caller takes the first form
-    return q->type == 1 ? lp_solve_v2<_Float16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, st)
-                        : lp_solve_v2<__bf16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, st);
+    return q->type == 1
+               ? lp_solve_v2<_Float16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, nrm, st)
+               : lp_solve_v2<__bf16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
 }

@@ -7,7 +7,8 @@
 
 #include "plr_common.h"
 
-static char g_err[512];
+// one message per thread: concurrent calls on distinct plans do not share it
+static thread_local char g_err[512];
 
 void plr_set_error(const char *fmt, ...)
 {

@@ -540,6 +540,7 @@ static int solve_body(dplrh_plan *p, int trans, int n, const double *dB, int ldb
         }
         if (!trans) {
             const int d0 = mm ? plr_solve_tail(p, B, ldb, X, ldx, nc, st) : -1;
+            if (d0 < -1) return d0; // a HIP error (-1: shape not covered)
             if (d0 >= 0)
                 dstop = d0;
             else if (mm)

@@ -632,6 +632,9 @@ __global__ __launch_bounds__(256, 2) void k_fp_level(FpLvlTab a, const double *_
     }
 }
 
+// solve_tail_t: the shallowest level d0 it solved down to, TL_REFUSED when
+// the shape is not covered (nothing launched), or a HIP error code (< -1).
+static constexpr int TL_REFUSED = -1;
 static int solve_tail_t(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx, int ncols,
                         int want_t, hipStream_t st);
 
@@ -666,7 +669,8 @@ int plr_solve_fast(dplrh_plan *p, const double *Bm, int ldb, double *Xm, int ldx
     int rc = plr_mm_part(p, plr_mm_part_blocks(p, 0, d0 - 1, ncols, FL_RB));
     if (rc) return rc;
     rc = solve_tail_t(p, Bm, ldb, Xm, ldx, ncols, 1, st);
-    if (rc < 0) return 1; // the tail does not fit: nothing launched
+    if (rc == TL_REFUSED) return 1; // the tail does not fit: nothing launched
+    if (rc < 0) return rc;          // a HIP error, passed on
     // the reassociated upper levels when the factorization prepared them
     rc = plr_up_solve(p, d0, Xm, ldx, ncols, st);
     if (rc <= 0) return rc;
@@ -704,15 +708,15 @@ static int solve_tail_t(dplrh_plan *p, const double *Bm, int ldb, double *Xm, in
                         int want_t, hipStream_t st)
 {
     const int N = p->N, L = p->L, b = p->b;
-    if (L == 0 || ncols <= 0 || b != 64) return -1;
+    if (L == 0 || ncols <= 0 || b != 64) return TL_REFUSED;
     // d0: the shallowest level whose span (two segments, N >> d0 rows) fits
     int d0 = 0;
     while ((N >> d0) > TL_MAXSPAN) ++d0;
-    if (d0 > L - 1) return -1;
+    if (d0 > L - 1) return TL_REFUSED;
     const int span = N >> d0;
     int RT = 16;
     for (int d = d0 > 0 && want_t ? d0 - 1 : d0; d < L; ++d) {
-        if (p->R[d] > TL_MAXR) return -1;
+        if (p->R[d] > TL_MAXR) return TL_REFUSED;
         RT = ((p->R[d] + 15) / 16) * 16 > RT ? ((p->R[d] + 15) / 16) * 16 : RT;
     }
     TailTab t;
@@ -744,7 +748,7 @@ static int solve_tail_t(dplrh_plan *p, const double *Bm, int ldb, double *Xm, in
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
         attr = true;
     }
-    if (lds > 160 * 1024) return -1;
+    if (lds > 160 * 1024) return TL_REFUSED;
     dim3 grid(N / span, (ncols + TL_CW - 1) / TL_CW);
     // the second form: ranks of the tail levels multiples of 16 and at most 32
     bool second = RT <= 32 && (span == 512 || span == 256 || span == 128);

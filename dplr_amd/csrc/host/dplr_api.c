@@ -221,6 +221,16 @@ int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, doub
     return dplrh_lp_solve(plan, n, dB, ldb, dX, ldx, iters, res, stream);
 }
 
+int dplrx_solve_refined_tol(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
+                            int ldx, double tol, int maxit, int *iters, double *err, void *stream)
+{
+    int N, L, b, R[DPLRH_MAXL];
+    dplrh_plan_dims(plan, &N, &L, &b, R);
+    if (n < 0 || maxit < 2 || !(tol >= 0.0) || ldb < N || ldx < N || (n > 0 && dB == dX))
+        return -1;
+    return dplrh_lp_solve_tol(plan, n, dB, ldb, dX, ldx, tol, maxit, iters, err, stream);
+}
+
 const char *dplrx_last_error(void) { return dplrh_last_error(); }
 
 /* ---- batches ------------------------------------------------------------ */

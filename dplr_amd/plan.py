@@ -131,6 +131,17 @@ class Plan:
                                       iters, rp, _stream(stream)), "dplrx_solve_refined")
         return res
 
+    def solve_refined_tol(self, B, X, tol=1e-10, maxit=8, stream=None):
+        """X := A^{-1} B refined until the estimated relative error of every
+        column is <= tol (at most maxit solves).  Returns (solves made,
+        error estimate); synchronises."""
+        n, m = B.shape
+        it, err = c_int(0), ctypes.c_double(0.0)
+        check(lib.dplrx_solve_refined_tol(self.p, n, _dptr(B), _ld(B), _dptr(X), _ld(X), tol,
+                                          maxit, byref(it), byref(err), _stream(stream)),
+              "dplrx_solve_refined_tol")
+        return it.value, err.value
+
 
 class Batch:
     """Independent plans applied or solved together: one kernel launch per

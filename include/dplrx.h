@@ -65,6 +65,25 @@ int dplrx_factor_mixed(dplrx_plan *plan, int type, int *info, void *stream);
 int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
                         int ldx, int iters, double *res, void *stream);
 
+/* X := A^{-1} B refined until the estimated normwise relative error
+ * |X - A^{-1}B|_F / |A^{-1}B|_F is at most tol, with at most maxit (>= 2)
+ * low-precision solves.  After solve k >= 2 the estimate is
+ * 1.25 * delta_k * min(1, delta_k / delta_{k-1}) with delta_k =
+ * |D_k|_F / |X_k|_F the relative size of the k-th correction D_k
+ * (delta_0 = 1): a correction measures the error of the previous iterate,
+ * and its ratio to the one before the contraction of the refinement (the
+ * factor 1.25 covers the contraction's growth from step to step).  Columns are taken in chunks of
+ * 128, each with its own estimate.  *iters (optional) receives the most
+ * solves a chunk made, *err (optional) the largest estimate (compare it
+ * with tol: 0 is returned also when maxit solves did not reach it).  Synchronises
+ * `stream` once per solve from the second on (the norms are read back).
+ * Fast paths: B 16-byte aligned with an even ldb is read in place and the
+ * residuals come out of the fused FP64 product when X is also 16-byte
+ * aligned with an even ldx; other layouts go through a copy of B. */
+int dplrx_solve_refined_tol(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
+                            int ldx, double tol, int maxit, int *iters, double *err,
+                            void *stream);
+
 /* ---- adjoints ----------------------------------------------------------- */
 
 /* Gradient of <G, A> with respect to A's factors for G = alpha * Ga Gb^T

@@ -1,6 +1,9 @@
 """Batched apply and solve of independent matrices of different order, leaf
-size and rank (dplrx_batch_*): every matrix of the batch against its own
-single-plan call, and against the oracle's dense product for small ones."""
+size and rank (dplrx_batch_*): every matrix of the batch against the oracle
+(its dplrgepdmm, src/dplrpdmm.c:454, for the products and for the residual
+of the solves; its explicit inverse, src/dplrinv.c:93, applied to the
+right-hand sides for the smaller ones), and against its own single-plan
+call."""
 import numpy as np
 import pytest
 
@@ -36,52 +39,70 @@ def batch(cuda):
         t.free()
 
 
-def test_batched_apply_matches_single(batch):
+def orc_apply(orc, tree, x):
+    """The oracle's product A x for an (n, m) device tensor, as a host array."""
+    xh = x.cpu().numpy().T.copy()
+    return D.gepdmm("N", "N", tree, xh, xh.shape[1], api=orc).T
+
+
+def orc_rel(a, ref):
+    a = a.cpu().numpy() if hasattr(a, "cpu") else a
+    return float(np.linalg.norm(a - ref) / np.linalg.norm(ref))
+
+
+def test_batched_apply_matches_oracle(batch, orc):
     import torch
     trees, plans, B, X = batch
     print("batched apply", SPECS, flush=True)
     Y = [torch.zeros_like(x) for x in X]
     B.gepdmm(X, Y)
-    for k, (P, x, y) in enumerate(zip(plans, X, Y)):
+    torch.cuda.synchronize()
+    for k, (T, P, x, y) in enumerate(zip(trees, plans, X, Y)):
+        assert orc_rel(y, orc_apply(orc, T, x)) < 1e-12, SPECS[k]
         y1 = torch.zeros_like(x)
         P.gepdmm("N", 1.0, x, 0.0, y1)
         assert rel(y, y1) < 1e-13, SPECS[k]
-    # the dense product for the two smallest
-    for k in (0, 6):
-        A = torch.tensor(trees[k].dense(), device="cuda")
-        ref = (A @ X[k].T).T
-        assert rel(Y[k], ref) < 1e-12, SPECS[k]
 
 
-def test_batched_solve_matches_single_and_replays(batch):
+def test_batched_solve_matches_oracle_and_replays(batch, orc):
     import torch
     trees, plans, B, X = batch
     print("batched solve", SPECS, flush=True)
     for P in plans:
         assert P.factor() == 0
+    # the oracle's explicit inverse applied to X, for the matrices up to 2048
+    inv_ref = {}
+    for k, (m, _, _) in enumerate(SPECS):
+        if m <= 2048:
+            Binv, info = D.geinv(trees[k], 0.0, api=orc)
+            assert info == 0
+            xh = X[k].cpu().numpy().T.copy()
+            inv_ref[k] = D.gepdmm("N", "N", Binv, xh, NRHS, api=orc).T
+            Binv.free()
     Z = [torch.zeros_like(x) for x in X]
     for rep in range(3):  # eager build, capture, replay
         for z in Z:
             z.zero_()
         B.solve(X, Z)
-        for k, (P, x, z) in enumerate(zip(plans, X, Z)):
+        torch.cuda.synchronize()
+        for k, (T, P, x, z) in enumerate(zip(trees, plans, X, Z)):
+            if k in inv_ref:
+                assert orc_rel(z, inv_ref[k]) < 1e-12, (rep, SPECS[k])
+            back = orc_apply(orc, T, z)
+            assert orc_rel(back, x.cpu().numpy()) < 1e-12, (rep, SPECS[k])
             z1 = torch.zeros_like(x)
             P.solve("N", x, z1)
             assert rel(z, z1) < 1e-12, (rep, SPECS[k])
-            back = torch.zeros_like(x)
-            P.gepdmm("N", 1.0, z, 0.0, back)
-            assert rel(back, x) < 1e-12, (rep, SPECS[k])
     # new right-hand sides in new buffers: the call's pointers are re-uploaded
     X2 = [2.0 * x + 1.0 for x in X]
     Z2 = [torch.zeros_like(x) for x in X]
     B.solve(X2, Z2)
-    for k, (P, x, z) in enumerate(zip(plans, X2, Z2)):
-        back = torch.zeros_like(x)
-        P.gepdmm("N", 1.0, z, 0.0, back)
-        assert rel(back, x) < 1e-12, SPECS[k]
+    torch.cuda.synchronize()
+    for k, (T, x, z) in enumerate(zip(trees, X2, Z2)):
+        assert orc_rel(orc_apply(orc, T, z), x.cpu().numpy()) < 1e-12, SPECS[k]
 
 
-def test_batch_rebuilds_after_plan_workspace_changes(batch):
+def test_batch_rebuilds_after_plan_workspace_changes(batch, orc):
     import torch
     trees, plans, B, X = batch
     for P in plans:
@@ -92,10 +113,9 @@ def test_batch_rebuilds_after_plan_workspace_changes(batch):
     plans[2].solve("N", big, out)
     Z = [torch.zeros_like(x) for x in X]
     B.solve(X, Z)
-    for k, (P, x, z) in enumerate(zip(plans, X, Z)):
-        back = torch.zeros_like(x)
-        P.gepdmm("N", 1.0, z, 0.0, back)
-        assert rel(back, x) < 1e-12, SPECS[k]
+    torch.cuda.synchronize()
+    for k, (T, x, z) in enumerate(zip(trees, X, Z)):
+        assert orc_rel(orc_apply(orc, T, z), x.cpu().numpy()) < 1e-12, SPECS[k]
 
 
 @pytest.mark.parametrize("tile", [128, 512, 1024])

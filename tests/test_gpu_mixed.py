@@ -71,6 +71,7 @@ def test_refined_residual_report(cuda):
     (8192, 7, 64, 9, "f16"),     # rank 64: narrower bottom kernel
     (4096, 6, 80, 4, "f16"),     # rank above 64: the per-level kernels
     (32768, 9, 32, 8, "f16"),    # 64 and 32 tiles per top segment: 4 and 2 finishing phases
+    (65536, 10, 32, 3, "f16"),   # 128 tiles per top segment: 8 finishing phases
 ])
 def test_refined_solve_shapes(cuda, m, nlvl, rank, n, dtype):
     import torch
@@ -120,4 +121,101 @@ def test_refined_solve_strided_blocks(cuda, pad):
     assert rel(X.cpu().numpy(), X64.cpu().numpy()) <= 1e-10
     assert torch.all(Xbig[:, m:] == 7.0)     # padding untouched
     assert torch.equal(big[:, :m], B)        # B is not written
+    P.close()
+
+
+@pytest.mark.parametrize("pad,off", [(6, 1), (4, 1), (5, 1)])
+def test_refined_solve_unaligned_x(cuda, pad, off):
+    """X starting one element into a row (8 bytes past a 16-byte boundary):
+    the fused residual product needs an aligned X and declines, and the
+    refinement takes the copy path for the residuals instead of failing."""
+    import torch
+    m, n = 4096, 9
+    print(f"mixed unaligned X pad={pad} off={off}", flush=True)
+    A = PlrTree.random(m, 6, 32, seed=45)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    assert P.factor_mixed("f16") == 0
+    B = torch.tensor(np.random.default_rng(9).standard_normal((n, m)), device=cuda)
+    X64 = torch.zeros_like(B)
+    P.solve("N", B, X64)
+    for mode in ("iters", "tol"):
+        Xbig = torch.full((n, m + pad), 3.0, dtype=torch.float64, device=cuda)
+        X = Xbig[:, off:off + m]
+        if mode == "iters":
+            P.solve_refined(B, X, 6)
+        else:
+            its, est = P.solve_refined_tol(B, X, tol=1e-10, maxit=8)
+            assert est <= 1e-10 and its <= 4
+        torch.cuda.synchronize()
+        assert rel(X.cpu().numpy(), X64.cpu().numpy()) <= 1e-10, mode
+        assert torch.all(Xbig[:, :off] == 3.0) and torch.all(Xbig[:, off + m:] == 3.0)
+    P.close()
+
+
+@pytest.mark.parametrize("dtype,most", [("f16", 3), ("bf16", 4)])
+def test_refined_tol_reports_its_error(cuda, orc, dtype, most):
+    """The refined solve stops on its own error estimate: the estimate it
+    reports is <= tol, the true error (against the FP64 solve, checked
+    against the oracle in test_gpu_solve.py) is too, and the estimate is
+    not far below the true error."""
+    import torch
+    m, nlvl, rank, n = 8192, 7, 32, 16
+    print(f"refined tol {dtype}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=41)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    assert P.factor_mixed(dtype) == 0
+    B = np.random.default_rng(5).standard_normal((m, n))
+    Bd = torch.tensor(np.ascontiguousarray(B.T), device=cuda)
+    X64 = torch.zeros_like(Bd)
+    P.solve("N", Bd, X64)
+    ref = X64.cpu().numpy()
+    for tol, cap in ((1e-10, most), (1e-6, most - 1)):
+        X = torch.zeros_like(Bd)
+        its, est = P.solve_refined_tol(Bd, X, tol=tol, maxit=10)
+        err = rel(X.cpu().numpy(), ref)
+        print(f"  tol {tol:.0e}: {its} solves, estimate {est:.2e}, error {err:.2e}", flush=True)
+        assert 2 <= its <= cap
+        assert est <= tol and err <= tol
+        assert err / 2 <= est <= 10 * err
+    # an unreachable tolerance: maxit solves, an estimate above it
+    X = torch.zeros_like(Bd)
+    its, est = P.solve_refined_tol(Bd, X, tol=0.0, maxit=3)
+    assert its == 3 and est > 0.0
+    # residual through the oracle's FP64 product
+    X = torch.zeros_like(Bd)
+    P.solve_refined_tol(Bd, X, tol=1e-10, maxit=10)
+    r = D.gepdmm("N", "N", A, X.cpu().numpy().T.copy(), n, api=orc) - B
+    assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-10
+    P.close()
+
+
+@pytest.mark.parametrize("m,nlvl,rank,ld", [(4096, 6, 80, 4096), (4096, 6, 32, 4097),
+                                            (512, 3, 32, 512)])
+def test_refined_tol_other_forms(cuda, m, nlvl, rank, ld):
+    """The estimate from the other code paths: ranks above 64 (per-level
+    kernels and the separate update), an odd leading dimension (B copied),
+    and a matrix that fits the bottom kernel alone; a zero column stays
+    zero with a zero estimate."""
+    import torch
+    n = 5
+    print(f"refined tol forms m={m} r={rank} ld={ld}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=46)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    assert P.factor_mixed("f16") == 0
+    Bbig = torch.tensor(np.random.default_rng(10).standard_normal((n, ld)), device=cuda)
+    Bbig[2] = 0.0
+    B = Bbig[:, :m]
+    X64 = torch.zeros((n, m), dtype=torch.float64, device=cuda)
+    P.solve("N", B.contiguous(), X64)
+    Xbig = torch.zeros((n, ld), dtype=torch.float64, device=cuda)
+    X = Xbig[:, :m]
+    its, est = P.solve_refined_tol(B, X, tol=1e-10, maxit=10)
+    torch.cuda.synchronize()
+    err = rel(X.cpu().numpy(), X64.cpu().numpy())
+    print(f"  {its} solves, estimate {est:.2e}, error {err:.2e}", flush=True)
+    assert est <= 1e-10 and err <= 1e-10
+    assert torch.all(X[2] == 0.0)
     P.close()
