@@ -37,6 +37,7 @@ def _load():
         "dplrx_plan_info": (c_int, [P, POINTER(c_int), POINTER(c_int), POINTER(c_int), POINTER(c_int)]),
         "dplrx_plan_bytes": (c_size_t, [P]),
         "dplrx_gepmv": (c_int, [P, c_char, c_double, P, c_double, P, P]),
+        "dplrx_gepmv_noalloc": (c_int, [P, c_char, c_double, P, c_double, P, P]),
         "dplrx_gepdmm": (c_int, [P, c_char, c_int, c_double, P, c_int, c_double, P, c_int, P]),
         "dplrx_last_error": (ctypes.c_char_p, []),
         "dplrx_factor": (c_int, [P, POINTER(c_int), P]),

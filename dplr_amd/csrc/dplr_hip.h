@@ -59,6 +59,10 @@ size_t dplrh_plan_bytes(const dplrh_plan *p);
 /* y := alpha*op(A)*x + beta*y on device vectors (unit stride). */
 int dplrh_gemv(dplrh_plan *p, int trans, double alpha, double beta,
                const double *dx, double *dy, void *stream);
+/* The same with the kernels' scratch in the library's static device array
+ * (no allocation; -7 when the plan needs more than DPLRX_SCRATCH_BYTES). */
+int dplrh_gemv_noalloc(dplrh_plan *p, int trans, double alpha, double beta,
+                       const double *dx, double *dy, void *stream);
 /* C := alpha*op(A)*B + beta*C on device column-major blocks. */
 /* Outer-product gradient projected onto the plan's factors (plr_grad.hip). */
 int dplrh_grad_outer(dplrh_plan *p, int n, double alpha, const double *A, int lda,

@@ -129,6 +129,9 @@ void plr_set_error(const char *fmt, ...);
 void plr_level_tab(const dplrh_plan *p, int trans, LevelTab *t);
 // Make sure the workspace holds ncols columns.
 int plr_workspace(dplrh_plan *p, int ncols);
+// Doubles of partial sums and coefficients and ints of arrival counters the
+// reduce/expand kernels need per column.
+void plr_ws_per_col(const dplrh_plan *p, size_t *part, size_t *w, size_t *cnt);
 // Level table for level d alone, with explicit operands (P reduced, Q expanded).
 void plr_level_tab1(const dplrh_plan *p, int d, const double *P, const double *Q,
                     const double *S, LevelTab *t);

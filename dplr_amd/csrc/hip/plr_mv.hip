@@ -16,8 +16,10 @@
 // same code serves the multi-RHS product for any shape (each column reads
 // the factors once).
 #include <cstring>
+#include <mutex>
 
 #include "plr_common.h"
+#include "dplrx.h"
 
 static constexpr int KG = 16; // rank group; matches plr_plan.hip
 
@@ -529,29 +531,37 @@ static void mv2_tab(const dplrh_plan *p, int trans, Mv2Tab *t)
 
 // Fast-path matvec for nc columns; the partial buffer p->part holds
 // part_per_col doubles per column (sized by plr_workspace for this layout).
-static int mv2_launch(dplrh_plan *p, int trans, int nc, double alpha, double beta,
-                      const double *x, int ldx, double *y, int ldy, hipStream_t st)
+static int mv2_launch_ws(dplrh_plan *p, int trans, int nc, double alpha, double beta,
+                         const double *x, int ldx, double *y, int ldy, double *part,
+                         size_t pstride, double *w, size_t wstride, hipStream_t st)
 {
     Mv2Tab t;
     mv2_tab(p, trans, &t);
-    if ((size_t)t.part_off[t.L] > p->part_per_col) {
+    if ((size_t)t.part_off[t.L] > pstride) {
         plr_set_error("matvec workspace too small");
         return -9;
     }
     if (t.nb > 0) {
-        hipLaunchKernelGGL(k_mv_reduce2, dim3(p->N / 128, nc), dim3(256), 0, st, t, x, ldx, p->part,
-                           p->part_per_col, p->w, p->w_per_col);
+        hipLaunchKernelGGL(k_mv_reduce2, dim3(p->N / 128, nc), dim3(256), 0, st, t, x, ldx, part,
+                           pstride, w, wstride);
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[t.L] > 0) {
         hipLaunchKernelGGL(k_mv_finish, dim3((t.fin_off[t.L] + 31) / 32, nc), dim3(256), 0, st, t,
-                           p->part, p->part_per_col, p->w, p->w_per_col);
+                           part, pstride, w, wstride);
         PLR_HIP(hipGetLastError());
     }
     hipLaunchKernelGGL(k_mv_expand2, dim3(p->N / 128, nc), dim3(256), 0, st, t, p->b, p->D, x, ldx,
-                       alpha, beta, trans, p->w, p->w_per_col, y, ldy);
+                       alpha, beta, trans, w, wstride, y, ldy);
     PLR_HIP(hipGetLastError());
     return 0;
+}
+
+static int mv2_launch(dplrh_plan *p, int trans, int nc, double alpha, double beta,
+                      const double *x, int ldx, double *y, int ldy, hipStream_t st)
+{
+    return mv2_launch_ws(p, trans, nc, alpha, beta, x, ldx, y, ldy, p->part, p->part_per_col, p->w,
+                         p->w_per_col, st);
 }
 
 int plr_launch_reduce(const LevelTab &t, const double *x, int ldx, double alpha, int trans,
@@ -666,4 +676,87 @@ extern "C" int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double 
     if (p->N == 0 || n == 0) return 0;
     // launched directly, as the matvec (three kernels per 128 columns)
     return plr_gemm_direct(p, trans, n, alpha, beta, dB, ldb, dC, ldc, (hipStream_t)stream);
+}
+
+// ---------------------------------------------------------------------------
+// Matvec that allocates nothing.  The partial sums, coefficients and arrival
+// counters of its kernels live in `scratch`, a static device array of
+// DPLRX_SCRATCH_BYTES per GPU (each GPU the library runs on holds it until
+// the process ends).  Its device address comes from hipGetSymbolAddress,
+// once per device with that device current; host code only ever uses that
+// address.  Every call on one GPU shares the array: calls that overlap in
+// time on one GPU race.  Counters are cleared by a fill kernel on every call
+// (the array keeps whatever the last call left).
+
+static __device__ __align__(256) double scratch[DPLRX_SCRATCH_BYTES / sizeof(double)];
+
+static constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static double *g_scratch[kMaxDevices];
+
+static int scratch_on_device(double **out)
+{
+    int dev = 0;
+    PLR_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) {
+        plr_set_error("device %d is beyond the %d the scratch table holds", dev, kMaxDevices);
+        return -5;
+    }
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch[dev]) {
+        void *ptr = nullptr;
+        PLR_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+        g_scratch[dev] = (double *)ptr;
+    }
+    *out = g_scratch[dev];
+    return 0;
+}
+
+__global__ void k_zero_i32(int *p, size_t n)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+extern "C" int dplrh_gemv_noalloc(dplrh_plan *p, int trans, double alpha, double beta,
+                                  const double *dx, double *dy, void *stream)
+{
+    hipStream_t st = (hipStream_t)stream;
+    if (p->N == 0) return 0;
+    if (alpha == 0.0) return dplrh_scale2(p->N, 1, beta, dy, p->N, st);
+    size_t part, w, cnt;
+    plr_ws_per_col(p, &part, &w, &cnt);
+    // 256-byte aligned regions: partials, coefficients, counters (ints)
+    const size_t part_r = (part + 31) & ~(size_t)31, w_r = (w + 31) & ~(size_t)31;
+    const size_t cnt_d = ((cnt + 1) / 2 + 31) & ~(size_t)31;
+    const size_t need = part_r + w_r + cnt_d;
+    if (need * sizeof(double) > DPLRX_SCRATCH_BYTES) {
+        plr_set_error("matvec needs %zu bytes of scratch, more than the %zu the no-allocation "
+                      "entry point holds", need * sizeof(double), (size_t)DPLRX_SCRATCH_BYTES);
+        return -7;
+    }
+    double *base = nullptr;
+    int rc = scratch_on_device(&base);
+    if (rc) return rc;
+    double *pp = base, *wp = base + part_r;
+    int *cp = (int *)(wp + w_r);
+    if (plr_mm_ok(p) && !p->force_valu)
+        return mv2_launch_ws(p, trans, 1, alpha, beta, dx, p->N, dy, p->N, pp, part, wp, w, st);
+    LevelTab t;
+    plr_level_tab(p, trans, &t);
+    const int ntask = t.task_off[t.L];
+    if (ntask > 0) {
+        if (cnt > 0) {
+            hipLaunchKernelGGL(k_zero_i32, dim3((unsigned)((cnt + 255) / 256)), dim3(256), 0, st,
+                               cp, cnt);
+            PLR_HIP(hipGetLastError());
+        }
+        hipLaunchKernelGGL(k_mv_reduce, dim3((ntask + 3) / 4, 1), dim3(256), 0, st, t, dx, p->N,
+                           alpha, trans, 0, pp, part, cp, cnt, wp, w);
+        PLR_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_mv_expand, dim3((p->N + 63) / 64, 1), dim3(256), 0, st, t, p->D, dx, p->N,
+                       alpha, beta, trans, wp, w, dy, p->N);
+    PLR_HIP(hipGetLastError());
+    return 0;
 }

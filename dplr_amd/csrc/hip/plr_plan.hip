@@ -94,9 +94,8 @@ void plr_level_tab0(const dplrh_plan *p, LevelTab *t)
     t->b = p->b;
 }
 
-int plr_workspace(dplrh_plan *p, int ncols)
+void plr_ws_per_col(const dplrh_plan *p, size_t *part_out, size_t *w_out, size_t *cnt_out)
 {
-    if (ncols <= p->wk_cols) return 0;
     LevelTab t;
     plr_level_tab(p, 0, &t);
     size_t part = 0, part2 = 0;
@@ -106,9 +105,16 @@ int plr_workspace(dplrh_plan *p, int ncols)
         const int seg = p->N >> (d + 1);
         if (seg > 128) part2 += (size_t)(2 << d) * (seg / 128) * t.R[d];
     }
-    if (part2 > part) part = part2;
-    const size_t w = p->soff[p->L];
-    const size_t cnt = (size_t)t.cnt_off[p->L];
+    *part_out = part2 > part ? part2 : part;
+    *w_out = p->soff[p->L];
+    *cnt_out = (size_t)t.cnt_off[p->L];
+}
+
+int plr_workspace(dplrh_plan *p, int ncols)
+{
+    if (ncols <= p->wk_cols) return 0;
+    size_t part, w, cnt;
+    plr_ws_per_col(p, &part, &w, &cnt);
     ++p->ws_gen;
     if (p->part) (void)hipFree(p->part);
     if (p->w) (void)hipFree(p->w);

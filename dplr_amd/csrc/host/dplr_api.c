@@ -184,6 +184,12 @@ int dplrx_gepmv(dplrx_plan *plan, char trans, double alpha, const double *dx, do
     return dplrh_gemv(plan, !is_n(trans), alpha, beta, dx, dy, stream);
 }
 
+int dplrx_gepmv_noalloc(dplrx_plan *plan, char trans, double alpha, const double *dx,
+                        double beta, double *dy, void *stream)
+{
+    return dplrh_gemv_noalloc(plan, !is_n(trans), alpha, beta, dx, dy, stream);
+}
+
 int dplrx_gepdmm(dplrx_plan *plan, char transa, int n, double alpha, const double *dB,
                  int ldb, double beta, double *dC, int ldc, void *stream)
 {

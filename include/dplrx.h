@@ -38,6 +38,22 @@ size_t dplrx_plan_bytes(const dplrx_plan *plan);
 int dplrx_gepmv(dplrx_plan *plan, char trans, double alpha, const double *dx,
                 double beta, double *dy, void *stream);
 
+/* dplrx_gepmv without any allocation.  No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * kernels' partial sums, coefficients and arrival counters live in a static
+ * device array of DPLRX_SCRATCH_BYTES on each GPU, held until the process
+ * ends and shared by every call on that GPU: calls that overlap in time on
+ * one GPU race.  Returns -7 (message in dplrx_last_error) when the plan
+ * needs more scratch than that: with leaves of a multiple of 64 rows it
+ * needs 8 * (N/128 * sum of R_d over the levels with segments longer than
+ * 128 rows + sum of 2^(d+1) R_d) bytes, 1.5 MB for N = 65536 with rank 32
+ * on 10 levels, 3.4 MB for N = 131072 with rank 32 on 11.  The plan must
+ * have been created on the current device. */
+#define DPLRX_SCRATCH_BYTES ((size_t)4 << 20)
+int dplrx_gepmv_noalloc(dplrx_plan *plan, char trans, double alpha, const double *dx,
+                        double beta, double *dy, void *stream);
+
 /* C := alpha*op(A)*B + beta*C with device column-major m x n blocks
  * (ldb, ldc >= m). */
 int dplrx_gepdmm(dplrx_plan *plan, char transa, int n, double alpha, const double *dB,
