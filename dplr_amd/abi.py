@@ -58,6 +58,9 @@ def declare(lib, prefix=""):
         "dplrrnk": (None, [c_int, PM, POINTER(c_int)]),
         "dplrnz": (c_long, [c_int, PM]),
         "dplrd2gep": (None, [c_int, PD, c_int, PM, c_double, c_int, POINTER(c_int)]),
+        "dplrgeppmm": (None, [c_char, c_char, c_int, PD, PM, PM, PD, PM, c_double]),
+        "dplrempty": (None, [PM, PM]),
+        "dplrcpy": (None, [c_int, PM, PM]),
     }
     found = {}
     for name, (res, args) in sig.items():

@@ -119,6 +119,14 @@ int dplrh_geprk(int N, int L, int b, int k, double alpha, double eps, const doub
                 const double *hv, const double *hs, double *hD, double **hUn, double **hVn,
                 double **hsig, void *stream);
 
+/* Level products of dplrgeppmm: for every level d with R_d > 0,
+ * PU[d] := alpha * op_ta(A_{>d}) * (tb ? V_d(B) : U_d(B))   (N x R_d(B)) and
+ * PV[d] := alpha * op_tb(B_{>d})^T * (ta ? U_d(A) : V_d(A)) (N x R_d(A)),
+ * host column-major arrays (ld N); A_{>d} is the block-diagonal forest of
+ * the subtrees below depth d (levels d+1 .. L-1 and the leaves). */
+int dplrh_ppmm_levels(dplrh_plan *pa, dplrh_plan *pb, int ta, int tb, double alpha, double **PU,
+                      double **PV, void *stream);
+
 /* Device memory helpers for the drop-in entry points. */
 void *dplrh_dev_alloc(size_t bytes);
 void dplrh_dev_free(void *p);

@@ -137,6 +137,9 @@ void plr_level_tab1(const dplrh_plan *p, int d, const double *P, const double *Q
                     const double *S, LevelTab *t);
 // A table with no levels (leaf term only).
 void plr_level_tab0(const dplrh_plan *p, LevelTab *t);
+// The levels d0 .. L-1 of A x (trans == 0) or A^T x: with the leaves, the
+// block-diagonal forest of the subtrees whose roots lie at depth d0.
+void plr_level_tab_from(const dplrh_plan *p, int trans, int d0, LevelTab *t);
 
 int plr_launch_reduce(const LevelTab &t, const double *x, int ldx, double alpha, int trans,
                       int raw, int ncols, double *part, size_t pstride, int *cnt, size_t cstride,

@@ -436,12 +436,18 @@ __global__ __launch_bounds__(256) void k_lp_colmax(const double *r, int ldr, int
         pmax[(size_t)c * LP_SCALE_CHUNKS + ch] = fmax(fmax(red[0], red[1]), fmax(red[2], red[3]));
 }
 
+// Copies of the correction-norm sums (see k_lp_update), and the most
+// refinement steps whose sums are kept at once (one slot per step).
+static constexpr int LP_NRM_COPIES = 16, LP_NRM_SLOTS = 16;
+
 // pm: pm_n partial maxima of |r| per column (column stride pm_n).
 __global__ __launch_bounds__(256) void k_lp_scale_rhs(const double *r, int ldr, int N,
                                                       const double *pm, int pm_n, double *scale,
-                                                      float *r32)
+                                                      float *r32, double *nrm)
 {
     const int c = blockIdx.y;
+    if (nrm && blockIdx.x == 0 && threadIdx.x < 2 * LP_NRM_COPIES) // norms of this step: zero
+        nrm[((size_t)(threadIdx.x >> 1) * gridDim.y + c) * 2 + (threadIdx.x & 1)] = 0.0;
     double mv = 0.0;
     for (int j = 0; j < pm_n; ++j) mv = fmax(mv, pm[(size_t)c * pm_n + j]);
     if (blockIdx.x == 0 && threadIdx.x == 0) scale[c] = mv;
@@ -465,13 +471,7 @@ __global__ void k_lp_pmfold(const double *pm, int pm_n, int n, double *out)
 // their work-group's sums into copy (blockIdx.x % LP_NRM_COPIES) of
 // nrm[copy][column][2], spreading the atomics over several cache lines; the
 // host adds the copies.  NaN and infinity propagate through the sums.
-static constexpr int LP_NRM_COPIES = 16;
 
-__global__ void k_lp_fill0(double *p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0.0;
-}
 
 // x[c] (+)= scale[c] * z[c]  (FP64 accumulation of the correction; the
 // first correction sets x); grid (row blocks of 256, columns).
@@ -525,8 +525,11 @@ struct plr_lp
     size_t part_cap;
     double *r, *scale, *pmax;
     double *bmax; // maxima of |r| per (column, 128-row block) from the residual product
-    double *nrm;      // norms of a correction step (lp_correct), LP_NRM_COPIES x cols x 2
+    double *nrm;      // norms of the correction steps (lp_correct): per step a slot of
+                      // LP_NRM_COPIES x cols x 2, LP_NRM_SLOTS slots
     double *nrm_host; // pinned host copy of nrm
+    int pred;         // solves the last refined solve to a tolerance needed,
+    double pred_tol;  // and that tolerance
 };
 
 static void lp_free(plr_lp *q)
@@ -638,9 +641,9 @@ static int lp_ws(const dplrh_plan *p, plr_lp *q, int cols)
     PLR_HIP(hipMalloc(&q->r, (size_t)p->N * cols * sizeof(double)));
     PLR_HIP(hipMalloc(&q->scale, (size_t)cols * sizeof(double)));
     PLR_HIP(hipMalloc(&q->pmax, (size_t)cols * LP_SCALE_CHUNKS * sizeof(double)));
-    PLR_HIP(hipMalloc(&q->nrm, (size_t)LP_NRM_COPIES * 2 * cols * sizeof(double)));
-    PLR_HIP(hipHostMalloc((void **)&q->nrm_host, (size_t)LP_NRM_COPIES * 2 * cols * sizeof(double),
-                          hipHostMallocDefault));
+    const size_t nn = (size_t)LP_NRM_SLOTS * LP_NRM_COPIES * 2 * cols;
+    PLR_HIP(hipMalloc(&q->nrm, nn * sizeof(double)));
+    PLR_HIP(hipHostMalloc((void **)&q->nrm_host, nn * sizeof(double), hipHostMallocDefault));
     q->cols = cols;
     return 0;
 }
@@ -923,16 +926,12 @@ static int lp_correct(dplrh_plan *p, plr_lp *q, int nc, int first, double *X, in
                       hipStream_t st)
 {
     const int N = p->N;
-    if (nrm) {
-        const int nn = LP_NRM_COPIES * 2 * nc;
-        hipLaunchKernelGGL(k_lp_fill0, dim3((nn + 255) / 256), dim3(256), 0, st, nrm, nn);
-        PLR_HIP(hipGetLastError());
-    }
+    // (the first kernel of either form zeroes nrm)
     int rc = lp_solve_v2_any(p, q, nc, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
     if (rc <= 0) return rc;
     // first form: the per-level kernels on r / max|r| in FP32
     hipLaunchKernelGGL(k_lp_scale_rhs, dim3((N + 255) / 256, nc), dim3(256), 0, st, r, ldr, N, pm,
-                       pm_n, q->scale, q->r32);
+                       pm_n, q->scale, q->r32, nrm);
     PLR_HIP(hipGetLastError());
     rc = q->type == 1 ? lp_solve_t<_Float16>(p, q, nc, st) : lp_solve_t<__bf16>(p, q, nc, st);
     if (rc) return rc;
@@ -950,34 +949,46 @@ static int lp_correct(dplrh_plan *p, plr_lp *q, int nc, int first, double *X, in
 // LP_EST_MARGIN * delta_k * min(1, rho_k).  The margin covers the growth of
 // the contraction from one step to the next as the error concentrates in
 // the directions the low-precision solve contracts least (measured on the
-// seeded random family: 9-13% per step).  NaN propagates.  Reads the sums
-// left by lp_correct (synchronises st).
+// seeded random family: 9-13% per step).  NaN propagates.
 static constexpr double LP_EST_MARGIN = 1.25;
-static int lp_estimate(plr_lp *q, int nc, int k, double *dprev, double *est, hipStream_t st)
+
+// Reads the sums of steps k0 .. k1 (slots k0 .. k1, one copy; synchronises
+// st), carries delta through them and returns the estimate of step k1.
+static int lp_estimate(plr_lp *q, int nc, int k0, int k1, double *dprev, double *est,
+                       hipStream_t st)
 {
-    const size_t nn = (size_t)LP_NRM_COPIES * 2 * nc;
-    PLR_HIP(hipMemcpyAsync(q->nrm_host, q->nrm, nn * sizeof(double), hipMemcpyDeviceToHost, st));
+    const size_t per = (size_t)LP_NRM_COPIES * 2 * nc;
+    PLR_HIP(hipMemcpyAsync(q->nrm_host + (size_t)k0 * per, q->nrm + (size_t)k0 * per,
+                           (size_t)(k1 - k0 + 1) * per * sizeof(double), hipMemcpyDeviceToHost, st));
     PLR_HIP(hipStreamSynchronize(st));
-    double dd = 0.0, xx = 0.0;
-    for (size_t i = 0; i < nn; i += 2) {
-        dd += q->nrm_host[i];
-        xx += q->nrm_host[i + 1];
+    double e = INFINITY;
+    for (int k = k0; k <= k1; ++k) {
+        const double *h = q->nrm_host + (size_t)k * per;
+        double dd = 0.0, xx = 0.0;
+        for (size_t i = 0; i < per; i += 2) {
+            dd += h[i];
+            xx += h[i + 1];
+        }
+        const double delta = dd == 0.0 ? 0.0 : sqrt(dd) / sqrt(xx); // xx = 0 < dd: inf
+        double rho = k == 1 ? delta : (*dprev > 0.0 ? delta / *dprev : (delta == 0.0 ? 0.0 : INFINITY));
+        if (rho > 1.0) rho = 1.0;
+        *dprev = delta;
+        e = LP_EST_MARGIN * delta * rho;
     }
-    const double delta = dd == 0.0 ? 0.0 : sqrt(dd) / sqrt(xx); // xx = 0 < dd: inf
-    double rho = k == 1 ? delta : (*dprev > 0.0 ? delta / *dprev : (delta == 0.0 ? 0.0 : INFINITY));
-    if (rho > 1.0) rho = 1.0;
-    *dprev = delta;
-    *est = LP_EST_MARGIN * delta * rho;
+    *est = e;
     return 0;
 }
 
 // X := A^{-1} B by mixed-precision solves refined with FP64 residuals
 // r = B - A X between them (iterative refinement), column chunks of at most
-// 128.  tol < 0: exactly `iters` solves.  tol >= 0: at most `iters` solves,
-// stopping after the first one (from the second on) whose error estimate
-// (lp_estimate, normwise over the chunk) is <= tol; *used and *err
-// (optional) receive the largest number of solves and estimate over the
-// chunks.  res[c] (host, optional) receives
+// 128.  tol < 0: exactly `iters` solves.  tol >= 0: at most `iters`
+// (<= LP_NRM_SLOTS) solves, stopping after the first one (from the second
+// on) whose error estimate (lp_estimate, normwise over the chunk) is <= tol;
+// *used and *err (optional) receive the largest number of solves and
+// estimate over the chunks.  The estimates are read back (a synchronisation
+// each) from the solve count the previous call on the plan with the same
+// tol needed on, the steps before it in one read: a matrix refined again
+// with similar right-hand sides synchronises once per call.  res[c] (host, optional) receives
 // max|b - A x| of column c after the last solve (one more FP64 residual).
 static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int ldb, double *dX,
                          int ldx, int iters, double tol, int *used, double *err, double *res,
@@ -1005,19 +1016,22 @@ static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int 
         const int nc = n - c0 < chunk ? n - c0 : chunk;
         const double *B = dB + (size_t)c0 * ldb;
         double *X = dX + (size_t)c0 * ldx;
-        int made = 0;
+        int made = 0, unread = 1; // first step whose sums are not read yet
         double est = INFINITY, dprev = 1.0;
+        const size_t per = (size_t)LP_NRM_COPIES * 2 * nc;
         for (int it = 0; it < iters; ++it) {
             const double *r, *pm;
             int ldr, pm_n;
             if ((rc = lp_residual(p, q, nc, B, ldb, X, ldx, it == 0, direct, &r, &ldr, &pm, &pm_n, st)))
                 return rc;
             const bool track = adaptive && it >= 1;
-            rc = lp_correct(p, q, nc, it == 0, X, ldx, r, ldr, pm, pm_n, track ? q->nrm : nullptr, st);
+            rc = lp_correct(p, q, nc, it == 0, X, ldx, r, ldr, pm, pm_n,
+                            track ? q->nrm + (size_t)it * per : nullptr, st);
             if (rc) return rc < 0 ? rc : -1;
             made = it + 1;
-            if (!track) continue;
-            if ((rc = lp_estimate(q, nc, it, &dprev, &est, st))) return rc;
+            if (!track || (tol == q->pred_tol && made < q->pred && it + 1 < iters)) continue;
+            if ((rc = lp_estimate(q, nc, unread, it, &dprev, &est, st))) return rc;
+            unread = it + 1;
             if (est <= tol) break;
         }
         if (made > used_max) used_max = made;
@@ -1033,6 +1047,10 @@ static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int 
             PLR_HIP(hipMemcpyAsync(res + c0, q->scale, nc * sizeof(double), hipMemcpyDeviceToHost, st));
             PLR_HIP(hipStreamSynchronize(st));
         }
+    }
+    if (adaptive) {
+        q->pred = used_max;
+        q->pred_tol = tol;
     }
     if (used) *used = used_max;
     if (err) *err = adaptive ? est_max : INFINITY;
@@ -1066,8 +1084,9 @@ extern "C" int dplrh_lp_solve_tol(dplrh_plan *p, int n, const double *dB, int ld
         plr_set_error("solve before a successful mixed-precision factorization");
         return -4;
     }
-    if (!(tol >= 0.0) || maxit < 2) {
-        plr_set_error("refined solve: tol must be >= 0 and maxit >= 2 (got %g, %d)", tol, maxit);
+    if (!(tol >= 0.0) || maxit < 2 || maxit > LP_NRM_SLOTS) {
+        plr_set_error("refined solve: tol must be >= 0 and 2 <= maxit <= %d (got %g, %d)",
+                      LP_NRM_SLOTS, tol, maxit);
         return -5;
     }
     return lp_solve_body(p, q, n, dB, ldb, dX, ldx, maxit, tol, iters, err, nullptr,
@@ -1256,7 +1275,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                                                         int pm_n, double *__restrict__ scale,
                                                         float *__restrict__ Z, int ldz,
                                                         float *__restrict__ Tout, size_t tstride,
-                                                        int ncols)
+                                                        int ncols, double *__restrict__ nrm)
 {
     typedef typename Lp<T16>::v8 v8;
     constexpr int NCT = CW / 16;
@@ -1273,6 +1292,13 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
     const int row0 = blockIdx.x * S, c0 = blockIdx.y * CW, N = a.N, b = a.b;
     const int nc = min(CW, ncols - c0);
 
+    // 0. the correction norms of this solve start from zero (its last
+    // kernel adds into them)
+    if (nrm && blockIdx.x == 0)
+        for (int e = tid; e < LP_NRM_COPIES * 2 * nc; e += LT_THREADS) {
+            const int cp = e / (2 * nc), rem = e - cp * 2 * nc;
+            nrm[((size_t)cp * ncols + c0 + rem / 2) * 2 + (rem & 1)] = 0.0;
+        }
     // 1. column scales (max |r| from the partial maxima, eight threads per
     // column), then z = r / max|r|
     {
@@ -1789,7 +1815,8 @@ static size_t lp_v2_part(const dplrh_plan *p, const plr_lp *q, const lp_v2 &v, i
 
 template <typename T16, int CW>
 static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_lp *q, int n,
-                          const double *r, int ldr, const double *pm, int pm_n, hipStream_t st)
+                          const double *r, int ldr, const double *pm, int pm_n, double *nrm,
+                          hipStream_t st)
 {
     static bool attr = false;
     if (!attr) {
@@ -1799,7 +1826,7 @@ static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_
         attr = true;
     }
     hipLaunchKernelGGL((k_lp_tail<T16, CW>), grid, dim3(LT_THREADS), lds, st, tt, r, ldr, pm, pm_n,
-                       q->scale, q->z32, tt.N, q->T, q->t_per_col, n);
+                       q->scale, q->z32, tt.N, q->T, q->t_per_col, n, nrm);
     PLR_HIP(hipGetLastError());
     return 0;
 }
@@ -1836,8 +1863,8 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
     tt.D = q->Dh;
     const size_t lds = lpt_lds(v.S, v.cw, v.RT, v.tsegs);
     const dim3 grid(N / v.S, (n + v.cw - 1) / v.cw);
-    rc = v.cw == 64 ? lp_launch_tail<T16, 64>(tt, lds, grid, q, n, r, ldr, pm, pm_n, st)
-                    : lp_launch_tail<T16, 32>(tt, lds, grid, q, n, r, ldr, pm, pm_n, st);
+    rc = v.cw == 64 ? lp_launch_tail<T16, 64>(tt, lds, grid, q, n, r, ldr, pm, pm_n, nrm, st)
+                    : lp_launch_tail<T16, 32>(tt, lds, grid, q, n, r, ldr, pm, pm_n, nrm, st);
     if (rc) return rc;
     if (v.d0 == 0) {
         hipLaunchKernelGGL(k_lp_update, dim3((N + 255) / 256, n), dim3(256), 0, st, q->z32, N,

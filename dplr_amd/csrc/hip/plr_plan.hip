@@ -86,6 +86,36 @@ void plr_level_tab1(const dplrh_plan *p, int d, const double *P, const double *Q
     t->S[0] = S;
 }
 
+void plr_level_tab_from(const dplrh_plan *p, int trans, int d0, LevelTab *t)
+{
+    LevelTab all;
+    plr_level_tab(p, trans, &all);
+    memset(t, 0, sizeof(*t));
+    t->L = p->L - d0;
+    t->N = p->N;
+    t->b = p->b;
+    // the workspace offsets stay those of the whole table (the workspace is
+    // sized for it); only the task numbering starts again at zero
+    for (int i = 0; i < t->L; ++i) {
+        const int d = d0 + i;
+        t->R[i] = all.R[d];
+        t->seglen[i] = all.seglen[d];
+        t->T[i] = all.T[d];
+        t->tps[i] = all.tps[d];
+        t->nkg[i] = all.nkg[d];
+        t->task_off[i] = all.task_off[d] - all.task_off[d0];
+        t->part_off[i] = all.part_off[d];
+        t->cnt_off[i] = all.cnt_off[d];
+        t->w_off[i] = all.w_off[d];
+        t->P[i] = all.P[d];
+        t->Q[i] = all.Q[d];
+        t->S[i] = all.S[d];
+    }
+    t->task_off[t->L] = all.task_off[p->L] - all.task_off[d0];
+    t->cnt_off[t->L] = all.cnt_off[p->L];
+    t->w_off[t->L] = all.w_off[p->L];
+}
+
 void plr_level_tab0(const dplrh_plan *p, LevelTab *t)
 {
     memset(t, 0, sizeof(*t));

@@ -174,6 +174,76 @@ void dplrresizelr(DPlrMtx *A, int capacity)
     A->dlr.sig = s;
 }
 
+/* reference src/dplrempty.c:18 */
+void dplrempty(DPlrMtx *A, DPlrMtx *B)
+{
+    if (A->type != PLR_DMTX && A->type != PLR_HMTX) {
+        fprintf(stderr, "%s: A must be a dense or hierarchical matrix\n", __func__);
+        return;
+    }
+    dplrclr(B);
+    if (A->type == PLR_DMTX) {
+        dplrinid(A->size, B);
+        return;
+    }
+    dplrinih(A->size, B);
+    if (B->type != PLR_HMTX) return;
+    dplrinilr(A->size / 2, B->dh.ne, 0);
+    dplrinilr(A->size / 2, B->dh.sw, 0);
+    dplrempty(A->dh.nw, B->dh.nw);
+    dplrempty(A->dh.se, B->dh.se);
+}
+
+static void lr_copy(int m, const DPlrMtx *a, DPlrMtx *b)
+{
+    const int r = a->dlr.rank;
+    if (b->dlr.capacity < r) dplrresizelr(b, r);
+    if (b->dlr.capacity < r) return; /* allocation failed (reported) */
+    if (r > 0) {
+        memcpy(b->dlr.sig, a->dlr.sig, (size_t)r * sizeof(double));
+        memcpy(b->dlr.u, a->dlr.u, (size_t)r * m * sizeof(double));
+        memcpy(b->dlr.v, a->dlr.v, (size_t)r * m * sizeof(double));
+    }
+    b->dlr.rank = r;
+}
+
+/* reference src/dplrcpy.c:19 */
+void dplrcpy(int m, DPlrMtx *A, DPlrMtx *B)
+{
+    if (m < 0) {
+        fprintf(stderr, "%s: invalid matrix size %d\n", __func__, m);
+        return;
+    }
+    if (A->type != PLR_DMTX && A->type != PLR_HMTX) {
+        fprintf(stderr, "%s: invalid A type\n", __func__);
+        return;
+    }
+    if (A->size != m) {
+        fprintf(stderr, "%s: invalid A size\n", __func__);
+        return;
+    }
+    if (A->type == PLR_DMTX) {
+        if (B->type != PLR_DMTX || B->size != m) {
+            dplrclr(B);
+            dplrinid(m, B);
+        }
+        if (m > 0 && B->dd.d) memcpy(B->dd.d, A->dd.d, (size_t)m * m * sizeof(double));
+        return;
+    }
+    if (B->type != PLR_HMTX || B->size != m) {
+        /* new structure; the diagonal blocks are shaped by the recursion */
+        dplrclr(B);
+        dplrinih(m, B);
+        if (B->type != PLR_HMTX) return;
+        dplrinilr(m / 2, B->dh.ne, 0);
+        dplrinilr(m / 2, B->dh.sw, 0);
+    }
+    lr_copy(m / 2, A->dh.ne, B->dh.ne);
+    lr_copy(m / 2, A->dh.sw, B->dh.sw);
+    dplrcpy(m / 2, A->dh.nw, B->dh.nw);
+    dplrcpy(m / 2, A->dh.se, B->dh.se);
+}
+
 /* ---- dense expansion, reference src/dplrp2d.c:18-143 ------------------- */
 
 static void lr_expand(int m, const DPlrMtx *A, double *B, int ldb)

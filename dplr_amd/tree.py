@@ -181,3 +181,44 @@ def geprk(A, u, v, s, alpha=1.0, eps=0.0, api=None):
     a = c_double(alpha)
     f(A.m, k, byref(a), _ptr(u), max(u.shape[0], 1), _ptr(v), max(v.shape[0], 1), _ptr(s), 1,
       A.ref, eps)
+
+
+def geppmm(transa, transb, A, B, C, alpha=1.0, beta=0.0, eps=0.0, api=None):
+    """C := alpha*op(A)*op(B) + beta*C in place via dplrgeppmm (C a PlrTree)."""
+    f = _fn(api, "dplrgeppmm")
+    a, b = c_double(alpha), c_double(beta)
+    f(c_char(transa.encode()), c_char(transb.encode()), A.m, byref(a), A.ref, B.ref, byref(b),
+      C.ref, eps)
+    return C
+
+
+def empty(A, api=None):
+    """A new tree of A's structure (dplrempty)."""
+    B = PlrTree(A.m)
+    _fn(api, "dplrempty")(A.ref, B.ref)
+    return B
+
+
+def copy(A, B=None, api=None):
+    """B := A via dplrcpy (a new tree unless B is given)."""
+    B = B if B is not None else PlrTree(A.m)
+    _fn(api, "dplrcpy")(A.m, A.ref, B.ref)
+    return B
+
+
+def nodes(tree):
+    """Every node in pre-order (node, nw, ne, se, sw as the H node lists them):
+    (type, size, rank or -1, capacity or -1) tuples, for structure checks."""
+    out = []
+
+    def walk(p):
+        c = p.contents
+        if c.type == PLR_LRMTX:
+            out.append((c.type, c.size, c.dlr.rank, c.dlr.capacity))
+        else:
+            out.append((c.type, c.size, -1, -1))
+        if c.type == PLR_HMTX:
+            for w in ("nw", "ne", "se", "sw"):
+                walk(getattr(c.dh, w))
+    walk(ctypes.pointer(tree.root))
+    return out

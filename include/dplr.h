@@ -98,6 +98,16 @@ void dplrinih(int m, DPlrMtx *A);
  * columns and NaN-fills the rest. */
 void dplrresizelr(DPlrMtx *A, int capacity);
 
+/* Replaces dplrempty, reference src/dplrempty.c:18.  B becomes a matrix of
+ * A's structure (sizes and node types) with rank-0 low-rank blocks and
+ * NaN-filled leaves; B's previous contents are freed. */
+void dplrempty(DPlrMtx *A, DPlrMtx *B);
+
+/* Replaces dplrcpy, reference src/dplrcpy.c:19.  B := A for an m x m A;
+ * B (initialised) is reshaped where its structure differs and its
+ * low-rank blocks grow to A's ranks. */
+void dplrcpy(int m, DPlrMtx *A, DPlrMtx *B);
+
 /* ---- compression --------------------------------------------------------- */
 
 /* Replaces dplrd2gep, reference src/dplrd2p.c:187.  Compresses the dense
@@ -148,6 +158,18 @@ void dplrgepdmm(char transa, char transb, int m, int n, double *alpha,
  * Columns whose scale falls below the per-level share of eps get sig = 0. */
 void dplrgeprk(int m, int k, double *alpha, double *u, int ldu, double *v,
                int ldv, double *s, int incs, DPlrMtx *A, double eps);
+
+/* Replaces dplrgeppmm, reference src/dplrppmm.c:998 (workers dplrppmm
+ * :492, dplrptpmm :617, dplrpptmm :742, dplrptptmm :868).
+ *   C := alpha*op(A)*op(B) + beta*C for m x m PLR matrices of one uniform
+ *   structure; op(X) is X for 'N'/'n' and X^T otherwise.
+ * C keeps the reference's structure: each off-diagonal block holds beta
+ * times its columns, then the products with the diagonal subtrees, then one
+ * low-rank correction per ancestor (added by dplrgeprk); columns normalised,
+ * with sig = 0 below eps/4 per level.  The subtree products of one level
+ * run as one product on the GPU.  C must not be A or B. */
+void dplrgeppmm(char transa, char transb, int m, double *alpha, DPlrMtx *A, DPlrMtx *B,
+                double *beta, DPlrMtx *C, double eps);
 
 /* Replaces dplrgeinv, reference src/dplrinv.c:93.
  *   B := inv(A) as an explicit PLR matrix with the reference's structure:

@@ -82,8 +82,8 @@ int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, doub
                         int ldx, int iters, double *res, void *stream);
 
 /* X := A^{-1} B refined until the estimated normwise relative error
- * |X - A^{-1}B|_F / |A^{-1}B|_F is at most tol, with at most maxit (>= 2)
- * low-precision solves.  After solve k >= 2 the estimate is
+ * |X - A^{-1}B|_F / |A^{-1}B|_F is at most tol, with at most maxit
+ * (2 <= maxit <= 16) low-precision solves.  After solve k >= 2 the estimate is
  * 1.25 * delta_k * min(1, delta_k / delta_{k-1}) with delta_k =
  * |D_k|_F / |X_k|_F the relative size of the k-th correction D_k
  * (delta_0 = 1): a correction measures the error of the previous iterate,
@@ -91,8 +91,10 @@ int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, doub
  * factor 1.25 covers the contraction's growth from step to step).  Columns are taken in chunks of
  * 128, each with its own estimate.  *iters (optional) receives the most
  * solves a chunk made, *err (optional) the largest estimate (compare it
- * with tol: 0 is returned also when maxit solves did not reach it).  Synchronises
- * `stream` once per solve from the second on (the norms are read back).
+ * with tol: 0 is returned also when maxit solves did not reach it).  Reads
+ * the estimate back (a synchronisation of `stream`) after each solve from
+ * the second on, or, when the previous call on the plan had the same tol,
+ * first after as many solves as that call made.
  * Fast paths: B 16-byte aligned with an even ldb is read in place and the
  * residuals come out of the fused FP64 product when X is also 16-byte
  * aligned with an even ldx; other layouts go through a copy of B. */

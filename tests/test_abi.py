@@ -126,3 +126,31 @@ def test_gpu_entry_fails_loudly_without_device(capfd):
     out = D.gepmv("N", A, np.ones(256), 1.0, 0.0, y)
     assert np.array_equal(out, y)          # untouched: no silent CPU fallback
     assert "failed" in capfd.readouterr().err
+
+
+def test_empty_and_copy_match_reference(ref):
+    """dplrempty (src/dplrempty.c:18) and dplrcpy (src/dplrcpy.c:19): the same
+    node structure as the reference's, copies equal to the source and
+    independent of it, and a copy into a tree of another shape reshapes it."""
+    from dplr_amd.tree import copy, empty, nodes
+    rng = np.random.default_rng(3)
+    A = PlrTree.from_blocks(512, 3, lambda d, n, w: (3 * d + n + (w == "ne")) % 9, rng)
+    E1, E2 = empty(A), empty(A, api=ref)
+    assert nodes(E1) == nodes(E2)
+    assert all(r in (-1, 0) for (_, _, r, _) in nodes(E1))
+    B1, B2 = copy(A), copy(A, api=ref)
+    assert [n[:3] for n in nodes(B1)] == [n[:3] for n in nodes(B2)] == [n[:3] for n in nodes(A)]
+    dA = A.dense()
+    assert np.array_equal(B1.dense(), dA) and np.array_equal(B2.dense(), dA)
+    # independent storage: changing the copy leaves A alone
+    node = next(p for (_, _, _, p) in B1.blocks() if p.contents.dlr.rank > 0)
+    node.contents.dlr.sig[0] += 1.0
+    assert np.array_equal(A.dense(), dA) and not np.array_equal(B1.dense(), dA)
+    # a copy into a tree of another structure reshapes it
+    C = PlrTree.random(256, 2, 4, seed=9)
+    C.m = 512
+    copy(A, C)
+    assert [n[:3] for n in nodes(C)] == [n[:3] for n in nodes(A)]
+    assert np.array_equal(C.dense(), dA)
+    for t in (A, E1, E2, B1, B2, C):
+        t.free()
