@@ -62,6 +62,36 @@ __device__ __forceinline__ typename Lp<T>::v8 cvt8(f8 x, float s)
     return r;
 }
 
+// Pair storage (type 3): a factor entry is hi + lo, two f16 values, of the
+// entry times a power of two chosen per level so that the largest entry is
+// about LP_PAIR_TOP (lo stays a normal f16 for all but the tiniest
+// entries); activations are split the same way, times LP_PAIR_Z.  A
+// product then takes three MFMAs (the lo x lo term, 2^-22 relative, is
+// dropped) and carries about 22 bits: FP32-like solves.
+static constexpr float LP_PAIR_TOP = 4096.f, LP_PAIR_Z = 256.f;
+
+template <typename T>
+__device__ __forceinline__ void split8(f8 x, float s, typename Lp<T>::v8 &hi, typename Lp<T>::v8 &lo)
+{
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const float v = x[j] * s;
+        const T h = (T)v;
+        hi[j] = h;
+        lo[j] = (T)(v - (float)h);
+    }
+}
+
+// acc + (ah + al)(bh + bl) without the al bl term
+template <typename T>
+__device__ __forceinline__ f4 mfma3(typename Lp<T>::v8 ah, typename Lp<T>::v8 al,
+                                    typename Lp<T>::v8 bh, typename Lp<T>::v8 bl, f4 acc)
+{
+    acc = Lp<T>::mfma(al, bh, acc);
+    acc = Lp<T>::mfma(ah, bl, acc);
+    return Lp<T>::mfma(ah, bh, acc);
+}
+
 static constexpr int LP_KG = 32, LP_CG = 64, LP_TMAX = 512, LP_BLK = LP_KG * LP_CG;
 
 struct LpTab
@@ -399,6 +429,77 @@ __global__ void k_lp_sig(const double *S, int R, int Rp, int nseg, float *Sf)
     Sf[e] = k < R ? (float)S[(size_t)s * R + k] : 0.f;
 }
 
+// ---- pair storage: hi + lo f16 planes of s * x -----------------------------
+
+__device__ __forceinline__ void pair_of(double v, _Float16 *hi, _Float16 *lo, size_t e)
+{
+    const _Float16 h = (_Float16)v;
+    hi[e] = h;
+    lo[e] = (_Float16)(v - (double)h);
+}
+
+// column-major Rp x N planes from an FP64 level array of R columns (ld N)
+__global__ void k_lp_pair_cols(const double *src, int R, int Rp, int N, double s, _Float16 *hi,
+                               _Float16 *lo)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)Rp * N) return;
+    const int k = (int)(e / N);
+    pair_of(k < R ? src[e] * s : 0.0, hi, lo, e);
+}
+
+// row-major N x Rp planes from an FP64 level array of R columns (ld N)
+__global__ void k_lp_pair_rows(const double *src, int R, int Rp, int N, double s, _Float16 *hi,
+                               _Float16 *lo)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)Rp * N) return;
+    const size_t i = e / Rp;
+    const int k = (int)(e % Rp);
+    pair_of(k < R ? src[(size_t)k * N + i] * s : 0.0, hi, lo, e);
+}
+
+// row-major leaf planes from column-major FP64 leaves
+__global__ void k_lp_pair_leaf(const double *src, int b, size_t nleaf, double s, _Float16 *hi,
+                               _Float16 *lo)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t bb = (size_t)b * b;
+    if (e >= nleaf * bb) return;
+    const size_t l = e / bb, r = e % bb;
+    const int i = (int)(r / b), j = (int)(r % b);
+    pair_of(src[l * bb + (size_t)j * b + i] * s, hi, lo, e);
+}
+
+// largest |x| of n entries, as float bits (unsigned order = numeric order)
+__global__ void k_lp_maxabs(const double *x, size_t n, unsigned *out)
+{
+    unsigned mv = 0u;
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
+         e += (size_t)gridDim.x * blockDim.x)
+        mv = max(mv, __float_as_uint(fabsf((float)x[e])));
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) mv = max(mv, (unsigned)__shfl_xor((int)mv, o, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, mv);
+}
+
+// inv(K) of every node with its ranks padded from R to Rp: K's index i < R
+// (bottom segment) stays, i >= R (top) moves to i - R + Rp; padded rows and
+// columns are those of the identity (their factor columns are zero).
+template <typename TK>
+__global__ void k_lp_kpad(const double *src, int R, int Rp, TK *dst)
+{
+    const int node = blockIdx.x, n2 = 2 * R, n2p = 2 * Rp;
+    const double *s = src + (size_t)node * n2 * n2;
+    TK *d = dst + (size_t)node * n2p * n2p;
+    for (int e = threadIdx.x; e < n2p * n2p; e += blockDim.x) {
+        const int ip = e % n2p, jp = e / n2p;
+        const int i = ip < Rp ? (ip < R ? ip : -1) : (ip - Rp < R ? ip - Rp + R : -1);
+        const int j = jp < Rp ? (jp < R ? jp : -1) : (jp - Rp < R ? jp - Rp + R : -1);
+        d[e] = (TK)((i < 0 || j < 0) ? (ip == jp ? 1.0 : 0.0) : s[(size_t)j * n2 + i]);
+    }
+}
+
 __global__ void k_f64_to_f32(const double *src, size_t n, float *dst)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -509,7 +610,8 @@ __global__ __launch_bounds__(256) void k_lp_update(const float *z, int N, const 
 
 struct plr_lp
 {
-    int type; // 1 = f16, 2 = bf16
+    int type; // 1 = f16, 2 = bf16, 3 = f16 pairs (hi + lo planes)
+    float isV[PLR_MAXL], isU[PLR_MAXL], isD; // pairs: inverse factor scales (1 otherwise)
     int Rp[PLR_MAXL];
     size_t off[PLR_MAXL + 1];  // element offset of level d in Vh / Uh / Ut32 (Rp columns of N)
     size_t soff[PLR_MAXL + 1]; // element offset of level d in Sf
@@ -797,7 +899,99 @@ static int lp_factor_t(dplrh_plan *p, plr_lp *q, int *info, hipStream_t st)
     return rc;
 }
 
-// type: 1 = f16, 2 = bf16.  Requires plr_mm_ok(p).
+static bool lp_v2_ok(const dplrh_plan *p, const plr_lp *q);
+
+// Pair storage (type 3) from the FP64 factorization, which runs first when
+// the plan has none: V, Ut and the leaf inverses as hi + lo f16 planes of
+// their FP64 values times a power of two per level (the largest entry at
+// about LP_PAIR_TOP), sig in FP32, the capacitance inverses padded to Rp
+// (FP64 on the top two levels, FP32 below).  The lo planes sit right after
+// the hi planes (offset off[L] for V and Ut, N * b for the leaves).
+static int lp_factor_pair(dplrh_plan *p, plr_lp *q, int *info, hipStream_t st)
+{
+    if (!p->f.done) {
+        const int rc = dplrh_factor(p, info, st);
+        if (rc || *info) return rc;
+    }
+    const int N = p->N, L = p->L, b = p->b;
+    unsigned *dmax = nullptr;
+    PLR_HIP(hipMalloc(&dmax, sizeof(unsigned)));
+    int rc = 0;
+    // the power of two s with s * max|x| in (LP_PAIR_TOP / 2, LP_PAIR_TOP]
+    auto scale_of = [&](const double *x, size_t n, double *s) -> int {
+        *s = 1.0;
+        if (n == 0) return 0;
+        PLR_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned), st));
+        const size_t blocks = (n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048;
+        hipLaunchKernelGGL(k_lp_maxabs, dim3((unsigned)blocks), dim3(256), 0, st, x, n, dmax);
+        PLR_HIP(hipGetLastError());
+        unsigned h = 0;
+        PLR_HIP(hipMemcpyAsync(&h, dmax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        PLR_HIP(hipStreamSynchronize(st));
+        float mx;
+        memcpy(&mx, &h, sizeof(mx));
+        if (mx > 0.f && mx < INFINITY) {
+            int e;
+            (void)frexp((double)LP_PAIR_TOP / (double)mx, &e);
+            *s = ldexp(1.0, e - 1);
+        }
+        return 0;
+    };
+    _Float16 *Vhi = (_Float16 *)q->Vh, *Vlo = Vhi + q->off[L];
+    _Float16 *Uhi = (_Float16 *)q->Uh, *Ulo = Uhi + q->off[L];
+    _Float16 *Dhi = (_Float16 *)q->Dh, *Dlo = Dhi + (size_t)N * b;
+    for (int d = 0; d < L && !rc; ++d) {
+        const int R = p->R[d], Rp = q->Rp[d];
+        const size_t n = (size_t)N * R, np = (size_t)N * Rp;
+        const unsigned gp = (unsigned)((np + 255) / 256);
+        double sV = 1.0, sU = 1.0;
+        if ((rc = scale_of(p->V + p->uoff[d], n, &sV))) break;
+        if ((rc = scale_of(p->f.Ut + p->uoff[d], n, &sU))) break;
+        if (np > 0) {
+            hipLaunchKernelGGL(k_lp_pair_cols, dim3(gp), dim3(256), 0, st, p->V + p->uoff[d], R, Rp,
+                               N, sV, Vhi + q->off[d], Vlo + q->off[d]);
+            hipLaunchKernelGGL(k_lp_pair_rows, dim3(gp), dim3(256), 0, st, p->f.Ut + p->uoff[d], R,
+                               Rp, N, sU, Uhi + q->off[d], Ulo + q->off[d]);
+        }
+        const int ns = (2 << d) * Rp;
+        if (ns > 0)
+            hipLaunchKernelGGL(k_lp_sig, dim3((ns + 255) / 256), dim3(256), 0, st, p->S + p->soff[d],
+                               R, Rp, 2 << d, q->Sf + q->soff[d]);
+        if (Rp > 0) {
+            if (d < 2)
+                hipLaunchKernelGGL(k_lp_kpad<double>, dim3(1 << d), dim3(256), 0, st,
+                                   p->f.Kinv + p->f.koff[d], R, Rp, q->K64 + q->koff[d]);
+            else
+                hipLaunchKernelGGL(k_lp_kpad<float>, dim3(1 << d), dim3(256), 0, st,
+                                   p->f.Kinv + p->f.koff[d], R, Rp, q->K32 + q->koff[d]);
+        }
+        if (hipGetLastError() != hipSuccess) {
+            plr_set_error("pair factors: conversion launch failed");
+            rc = -10;
+        }
+        q->isV[d] = (float)(1.0 / sV);
+        q->isU[d] = (float)(1.0 / sU);
+    }
+    if (!rc) {
+        double sD = 1.0;
+        const size_t nd = (size_t)N * b;
+        rc = scale_of(p->f.Dinv, nd, &sD);
+        if (!rc) {
+            hipLaunchKernelGGL(k_lp_pair_leaf, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st,
+                               p->f.Dinv, b, (size_t)(N / b), sD, Dhi, Dlo);
+            q->isD = (float)(1.0 / sD);
+            if (hipGetLastError() != hipSuccess) {
+                plr_set_error("pair factors: leaf conversion launch failed");
+                rc = -10;
+            }
+        }
+    }
+    if (!rc && hipStreamSynchronize(st) != hipSuccess) rc = -10;
+    (void)hipFree(dmax);
+    return rc;
+}
+
+// type: 1 = f16, 2 = bf16, 3 = f16 pairs.  Requires plr_mm_ok(p).
 extern "C" int dplrh_lp_factor(dplrh_plan *p, int type, int *info, void *stream)
 {
     hipStream_t st = (hipStream_t)stream;
@@ -806,7 +1000,7 @@ extern "C" int dplrh_lp_factor(dplrh_plan *p, int type, int *info, void *stream)
         plr_set_error("mixed precision needs the leaf order to be a multiple of 64");
         return -5;
     }
-    if (type != 1 && type != 2) {
+    if (type != 1 && type != 2 && type != 3) {
         plr_set_error("unknown low-precision type %d", type);
         return -5;
     }
@@ -815,6 +1009,9 @@ extern "C" int dplrh_lp_factor(dplrh_plan *p, int type, int *info, void *stream)
     plr_lp *q = new plr_lp;
     memset(q, 0, sizeof(*q));
     q->type = type;
+    for (int d = 0; d < PLR_MAXL; ++d) q->isV[d] = q->isU[d] = 1.f;
+    q->isD = 1.f;
+    const size_t planes = type == 3 ? 2 : 1;
     size_t off = 0, so = 0, k64 = 0, k32 = 0;
     for (int d = 0; d < p->L; ++d) {
         q->Rp[d] = (p->R[d] + 31) & ~31;
@@ -836,14 +1033,21 @@ extern "C" int dplrh_lp_factor(dplrh_plan *p, int type, int *info, void *stream)
         PLR_HIP(hipMalloc(ptr, bytes ? bytes : 8));
         return 0;
     };
-    if ((rc = alloc(&q->Vh, off * 2)) || (rc = alloc(&q->Uh, off * 2)) ||
-        (rc = alloc(&q->Dh, (size_t)p->N * p->b * 2)) || (rc = alloc((void **)&q->Sf, so * 4)) ||
+    if ((rc = alloc(&q->Vh, off * 2 * planes)) || (rc = alloc(&q->Uh, off * 2 * planes)) ||
+        (rc = alloc(&q->Dh, (size_t)p->N * p->b * 2 * planes)) || (rc = alloc((void **)&q->Sf, so * 4)) ||
         (rc = alloc((void **)&q->K64, k64 * 8)) || (rc = alloc((void **)&q->K32, k32 * 4))) {
         lp_free(q);
         return rc;
     }
     g_lp.push_back({p, q});
-    rc = type == 1 ? lp_factor_t<_Float16>(p, q, info, st) : lp_factor_t<__bf16>(p, q, info, st);
+    rc = type == 1   ? lp_factor_t<_Float16>(p, q, info, st)
+         : type == 2 ? lp_factor_t<__bf16>(p, q, info, st)
+                     : lp_factor_pair(p, q, info, st);
+    if (!rc && !*info && type == 3 && !lp_v2_ok(p, q)) {
+        plr_set_error("f16 pair storage needs leaves of 64..512 rows, ranks <= 64 and a shape the "
+                      "span kernels cover");
+        rc = -5;
+    }
     if (rc || *info) plr_lp_release(p);
     return rc;
 }
@@ -929,6 +1133,10 @@ static int lp_correct(dplrh_plan *p, plr_lp *q, int nc, int first, double *X, in
     // (the first kernel of either form zeroes nrm)
     int rc = lp_solve_v2_any(p, q, nc, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
     if (rc <= 0) return rc;
+    if (q->type == 3) { // the factorization checked the shape; this is not reached
+        plr_set_error("f16 pair storage has no per-level solve");
+        return -5;
+    }
     // first form: the per-level kernels on r / max|r| in FP32
     hipLaunchKernelGGL(k_lp_scale_rhs, dim3((N + 255) / 256, nc), dim3(256), 0, st, r, ldr, N, pm,
                        pm_n, q->scale, q->r32, nrm);
@@ -1123,6 +1331,10 @@ struct LpTailTab
     const float *K32[PLR_MAXL];
     const double *K64[PLR_MAXL];
     const void *D;
+    // pair storage: lo planes at these element offsets from V, Ut, D, and the
+    // inverse factor scales (1 for single storage)
+    size_t vlo, ulo, dlo;
+    float isV[PLR_MAXL], isU[PLR_MAXL], isD;
 };
 
 struct LpLvlTab
@@ -1132,6 +1344,8 @@ struct LpLvlTab
     const float *S;
     const float *K32;
     const double *K64;
+    size_t vlo, ulo;   // pair storage: lo planes of Vn and Ut
+    float isU, isVn;   // inverse factor scales
 };
 
 
@@ -1237,19 +1451,24 @@ static size_t lpt_lds(int S, int CW, int RT, int tsegs)
 // of a batch go out together).  Segments of at most 256 rows end inside a
 // batch and are written by emit(s, kt, ct, acc); longer ones leave one
 // partial per batch, emitted with s = -1 - (batch index in the segment).
-template <typename T16, int NCT, typename Emit>
+// With PR (pair storage) V's lo plane is at V + vlo, z is split into hi and
+// lo times LP_PAIR_Z, and emitted sums are scaled by os (the inverse of V's
+// scale times LP_PAIR_Z).
+template <typename T16, int NCT, bool PR, typename Emit>
 __device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, const float *z,
                                                int zs, int S, int seg, int nkt, int wv, int m,
-                                               int g, Emit emit)
+                                               int g, size_t vlo, float os, Emit emit)
 {
     typedef typename Lp<T16>::v8 v8;
     const int nstep = S / 32, spq = seg / 32, total = nstep * nkt * NCT;
     for (int e0 = wv * 8; e0 < total; e0 += 8 * LT_NW) {
-        v8 av[8];
+        v8 av[8], al[8];
 #pragma unroll
         for (int q = 0; q < 8; ++q) {
             const int e = e0 + q, grp = e / nstep, rs = e - grp * nstep, kt = grp / NCT;
-            av[q] = *(const v8 *)(V + (size_t)(kt * 16 + m) * N + row0 + rs * 32 + 8 * g);
+            const size_t idx = (size_t)(kt * 16 + m) * N + row0 + rs * 32 + 8 * g;
+            av[q] = *(const v8 *)(V + idx);
+            if (PR) al[q] = *(const v8 *)(V + vlo + idx);
         }
         f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -1257,19 +1476,25 @@ __device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, co
             const int e = e0 + q, grp = e / nstep, rs = e - grp * nstep;
             const int kt = grp / NCT, ct = grp - kt * NCT;
             const f8 zv = lds8(z + (ct * 16 + m) * zs + rs * 32 + 8 * g);
-            acc = Lp<T16>::mfma(av[q], cvt8<T16>(zv, 1.f), acc);
+            if (PR) {
+                v8 zh, zl;
+                split8<T16>(zv, LP_PAIR_Z, zh, zl);
+                acc = mfma3<T16>(av[q], al[q], zh, zl, acc);
+            } else {
+                acc = Lp<T16>::mfma(av[q], cvt8<T16>(zv, 1.f), acc);
+            }
             if ((rs + 1) % spq == 0 && spq <= 8) {
-                emit(rs / spq, kt, ct, acc);
+                emit(rs / spq, kt, ct, PR ? acc * os : acc);
                 acc = f4{0.f, 0.f, 0.f, 0.f};
             } else if (q == 7 && spq > 8) {
-                emit(-1 - (rs % spq) / 8, kt, ct, acc);
+                emit(-1 - (rs % spq) / 8, kt, ct, PR ? acc * os : acc);
             }
         }
     }
 }
 
 // pmax: pm_n partial maxima of |r| per column (column stride pm_n).
-template <typename T16, int CW>
+template <typename T16, int CW, bool PR>
 __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const double *__restrict__ r,
                                                         int ldr, const double *__restrict__ pmax,
                                                         int pm_n, double *__restrict__ scale,
@@ -1365,12 +1590,16 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
             for (int ct = 0; ct < NCT; ++ct) acc[q][ct] = f4{0.f, 0.f, 0.f, 0.f};
         }
         for (int j0 = 0; j0 < b; j0 += 64) {
-            v8 a0[LT_MAXT], a1[LT_MAXT];
+            v8 a0[LT_MAXT], a1[LT_MAXT], l0[LT_MAXT], l1[LT_MAXT];
 #pragma unroll
             for (int q = 0; q < LT_MAXT; ++q) {
                 const T16 *D = Dh + ((size_t)(row0 / b + lf[q]) * b + i0[q] - lf[q] * b + m) * b + 8 * g + j0;
                 a0[q] = *(const v8 *)D;
                 a1[q] = *(const v8 *)(D + 32);
+                if (PR) {
+                    l0[q] = *(const v8 *)(D + a.dlo);
+                    l1[q] = *(const v8 *)(D + a.dlo + 32);
+                }
             }
 #pragma unroll
             for (int q = 0; q < LT_MAXT; ++q) {
@@ -1379,10 +1608,25 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                 for (int ct = 0; ct < NCT; ++ct) {
                     const f8 x0 = lds8(zb + ct * 16 * zs);
                     const f8 x1 = lds8(zb + ct * 16 * zs + 32);
-                    acc[q][ct] = Lp<T16>::mfma(a0[q], cvt8<T16>(x0, 1.f), acc[q][ct]);
-                    acc[q][ct] = Lp<T16>::mfma(a1[q], cvt8<T16>(x1, 1.f), acc[q][ct]);
+                    if (PR) {
+                        v8 h0, t0, h1, t1;
+                        split8<T16>(x0, LP_PAIR_Z, h0, t0);
+                        split8<T16>(x1, LP_PAIR_Z, h1, t1);
+                        acc[q][ct] = mfma3<T16>(a0[q], l0[q], h0, t0, acc[q][ct]);
+                        acc[q][ct] = mfma3<T16>(a1[q], l1[q], h1, t1, acc[q][ct]);
+                    } else {
+                        acc[q][ct] = Lp<T16>::mfma(a0[q], cvt8<T16>(x0, 1.f), acc[q][ct]);
+                        acc[q][ct] = Lp<T16>::mfma(a1[q], cvt8<T16>(x1, 1.f), acc[q][ct]);
+                    }
                 }
             }
+        }
+        if (PR) {
+            const float os = a.isD / LP_PAIR_Z;
+#pragma unroll
+            for (int q = 0; q < LT_MAXT; ++q)
+#pragma unroll
+                for (int ct = 0; ct < NCT; ++ct) acc[q][ct] *= os;
         }
         __syncthreads();
 #pragma unroll
@@ -1402,8 +1646,8 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         const T16 *V = (const T16 *)a.V[d], *U = (const T16 *)a.Ut[d];
         // (a) T[s][k][c] = sum over segment s of V[k][i] z[c][i]
         for (int e = tid; e < nsg * CW; e += LT_THREADS) smax[e] = 0u;
-        lp_tail_reduce<T16, NCT>(V, N, row0, z, zs, S, seg, nkt, wv, m, g,
-                                 [&](int s, int kt, int ct, f4 acc) {
+        lp_tail_reduce<T16, NCT, PR>(V, N, row0, z, zs, S, seg, nkt, wv, m, g, a.vlo,
+                                     a.isV[d] / LP_PAIR_Z, [&](int s, int kt, int ct, f4 acc) {
                                      float *t = T + s * tsz + (kt * 16 + 4 * g) * ts + ct * 16 + m;
 #pragma unroll
                                      for (int v = 0; v < 4; ++v) t[v * ts] = acc[v];
@@ -1457,12 +1701,15 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         {
             const int nks = R / 32, total = (S / 16) * NCT * nks;
             for (int e0 = wv * 8; e0 < total; e0 += 8 * LT_NW) {
-                v8 bq[8];
+                v8 bq[8], bl[8];
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
                     const int e = e0 + q, ks = e % nks, it = (e / nks) / NCT;
-                    bq[q] = *(const v8 *)(U + (size_t)(row0 + it * 16 + m) * R + ks * 32 + 8 * g);
+                    const size_t idx = (size_t)(row0 + it * 16 + m) * R + ks * 32 + 8 * g;
+                    bq[q] = *(const v8 *)(U + idx);
+                    if (PR) bl[q] = *(const v8 *)(U + a.ulo + idx);
                 }
+                const float os = PR ? a.isU[d] / LP_PAIR_Z : 1.f;
                 f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                 for (int q = 0; q < 8; ++q) {
@@ -1470,11 +1717,17 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                     const int s = (it * 16) / seg;
                     const unsigned *sm = smax + s * CW + ct * 16;
                     const f8 w8 = lds8(W + s * wsz + (ct * 16 + m) * wsk + ks * 32 + 8 * g);
-                    acc = Lp<T16>::mfma(cvt8<T16>(w8, 1.f / lp_pow2(sm[m])), bq[q], acc);
+                    if (PR) {
+                        v8 wh, wl;
+                        split8<T16>(w8, LP_PAIR_Z / lp_pow2(sm[m]), wh, wl);
+                        acc = mfma3<T16>(wh, wl, bq[q], bl[q], acc);
+                    } else {
+                        acc = Lp<T16>::mfma(cvt8<T16>(w8, 1.f / lp_pow2(sm[m])), bq[q], acc);
+                    }
                     if (ks == nks - 1) {
 #pragma unroll
                         for (int v = 0; v < 4; ++v)
-                            z[(ct * 16 + 4 * g + v) * zs + it * 16 + m] += acc[v] * lp_pow2(sm[4 * g + v]);
+                            z[(ct * 16 + 4 * g + v) * zs + it * 16 + m] += acc[v] * (os * lp_pow2(sm[4 * g + v]));
                         acc = f4{0.f, 0.f, 0.f, 0.f};
                     }
                 }
@@ -1490,8 +1743,9 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         const int nh = S > 256 ? S / 256 : 1; // partials per (kt, ct)
         float *red = T;                        // free now: [kt][ct][h][lane][4]
         if (R > 0) {
-            lp_tail_reduce<T16, NCT>((const T16 *)a.V[dn], N, row0, z, zs, S, S, nkt, wv, m, g,
-                                     [&](int s, int kt, int ct, f4 acc) {
+            lp_tail_reduce<T16, NCT, PR>((const T16 *)a.V[dn], N, row0, z, zs, S, S, nkt, wv, m, g,
+                                         a.vlo, a.isV[dn] / LP_PAIR_Z,
+                                         [&](int s, int kt, int ct, f4 acc) {
                                          if (s >= 0) {
                                              const int c = ct * 16 + m;
                                              if (c < nc)
@@ -1522,18 +1776,19 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
     }
 }
 
-// LDS of k_lp_level: rhs [2R][CW+16], W [CW][R+4] (floats), zt [CW][RB+8] (16-bit)
-static size_t lpl_lds(int R)
+// LDS of k_lp_level: rhs [2R][CW+16], W [CW][R+4] (floats), zt [CW][RB+8]
+// (16-bit; with pair storage a second zt for the lo parts)
+static size_t lpl_lds(int R, bool pair)
 {
     return ((size_t)2 * R * (LV_CW + 16) + (size_t)LV_CW * (R + 4)) * sizeof(float) +
-           (size_t)LV_CW * (LV_RB + 8) * 2;
+           (size_t)LV_CW * (LV_RB + 8) * 2 * (pair ? 2 : 1);
 }
 
 // Every global load of the work-group (old z, Ut, V of level d-1, T, K, sig)
 // is issued at the start, so the work-group waits on memory about once.  Eight
 // waves of LV_RT 16-row tiles each; RM bounds the ranks of levels d and d-1
 // (32 or 64), which sizes the register buffers.
-template <typename T16, int RM>
+template <typename T16, int RM, bool PR>
 __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float *__restrict__ Tin,
                                                          size_t tstride, float *__restrict__ Z,
                                                          int ldz, float *__restrict__ part,
@@ -1552,6 +1807,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
     float *rhs = lds;                   // [n2][rs]
     float *W = rhs + n2 * rs;           // [CW][wsk]
     T16 *zt = (T16 *)(W + LV_CW * wsk); // [CW][zts]
+    T16 *ztl = zt + LV_CW * zts;        // [CW][zts], lo parts (pair storage)
     const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wv = tid >> 6;
     const int r0 = blockIdx.x * LV_RB, c0 = blockIdx.y * LV_CW, N = a.N;
     const int ri = r0 + wv * 16 * LV_RT; // this wave's rows
@@ -1569,23 +1825,29 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
             zo[rt][ct] = *(const f4 *)(Z + (size_t)(c0 + min(ct * 16 + m, nc - 1)) * ldz + ri + 16 * rt + 4 * g);
     // Ut rows of the expansion
     const T16 *U = (const T16 *)a.Ut;
-    v8 bq[LV_RT][KS];
+    v8 bq[LV_RT][KS], bl[LV_RT][KS];
     if (R > 0) // a level of rank 0 has no Ut rows
 #pragma unroll
         for (int rt = 0; rt < LV_RT; ++rt)
 #pragma unroll
-            for (int ks = 0; ks < KS; ++ks)
-                bq[rt][ks] = *(const v8 *)(U + (size_t)(ri + 16 * rt + m) * R + min(ks, nks - 1) * 32 + 8 * g);
+            for (int ks = 0; ks < KS; ++ks) {
+                const size_t idx = (size_t)(ri + 16 * rt + m) * R + min(ks, nks - 1) * 32 + 8 * g;
+                bq[rt][ks] = *(const v8 *)(U + idx);
+                if (PR) bl[rt][ks] = *(const v8 *)(U + a.ulo + idx);
+            }
     // V of level d-1 for this wave's reduction tasks (kt, ct) = wv + 8p
-    v8 av[NP][NU];
+    v8 av[NP][NU], avl[NP][NU];
     if (a.Rn > 0) {
         const T16 *Vn = (const T16 *)a.Vn;
 #pragma unroll
         for (int p = 0; p < NP; ++p) {
             const int task = min(wv + 8 * p, nktn * 4 - 1), kt = task >> 2;
 #pragma unroll
-            for (int u = 0; u < NU; ++u)
-                av[p][u] = *(const v8 *)(Vn + (size_t)(kt * 16 + m) * N + r0 + 32 * u + 8 * g);
+            for (int u = 0; u < NU; ++u) {
+                const size_t idx = (size_t)(kt * 16 + m) * N + r0 + 32 * u + 8 * g;
+                av[p][u] = *(const v8 *)(Vn + idx);
+                if (PR) avl[p][u] = *(const v8 *)(Vn + a.vlo + idx);
+            }
         }
     }
     // 1. rhs = [T_bot; T_top] of the node
@@ -1653,6 +1915,12 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
             sc[ct] = lp_pow2(smax[ct * 16 + m]);
             isc[ct] = 1.f / sc[ct];
         }
+        // pair storage: W split times LP_PAIR_Z; the sums come back times Ut's
+        // inverse scale over LP_PAIR_Z (column scales sco, one per lane's
+        // four columns in the D layout below)
+        float sco[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) sco[ct] = PR ? sc[ct] * (a.isU / LP_PAIR_Z) : sc[ct];
         f4 acc[LV_RT][4];
 #pragma unroll
         for (int rt = 0; rt < LV_RT; ++rt)
@@ -1663,9 +1931,18 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
             if (ks < nks) {
 #pragma unroll
                 for (int ct = 0; ct < 4; ++ct) {
-                    const v8 bw = cvt8<T16>(lds8(W + (ct * 16 + m) * wsk + ks * 32 + 8 * g), isc[ct]);
+                    const f8 w8 = lds8(W + (ct * 16 + m) * wsk + ks * 32 + 8 * g);
+                    if (PR) {
+                        v8 wh, wl;
+                        split8<T16>(w8, isc[ct] * LP_PAIR_Z, wh, wl);
 #pragma unroll
-                    for (int rt = 0; rt < LV_RT; ++rt) acc[rt][ct] = Lp<T16>::mfma(bq[rt][ks], bw, acc[rt][ct]);
+                        for (int rt = 0; rt < LV_RT; ++rt)
+                            acc[rt][ct] = mfma3<T16>(bq[rt][ks], bl[rt][ks], wh, wl, acc[rt][ct]);
+                    } else {
+                        const v8 bw = cvt8<T16>(w8, isc[ct]);
+#pragma unroll
+                        for (int rt = 0; rt < LV_RT; ++rt) acc[rt][ct] = Lp<T16>::mfma(bq[rt][ks], bw, acc[rt][ct]);
+                    }
                 }
             }
         }
@@ -1679,7 +1956,7 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
                 const float on = c < nc ? 1.f : 0.f;
                 f4 zn;
 #pragma unroll
-                for (int v = 0; v < 4; ++v) zn[v] = zo[rt][ct][v] * on + acc[rt][ct][v] * sc[ct];
+                for (int v = 0; v < 4; ++v) zn[v] = zo[rt][ct][v] * on + acc[rt][ct][v] * sco[ct];
                 if (X) {
                     if (c < nc) {
                         double *xp = X + (size_t)(c0 + c) * ldx + r0 + i;
@@ -1708,10 +1985,15 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
                 } else {
                     if (c < nc) *(f4 *)(Z + (size_t)(c0 + c) * ldz + r0 + i) = zn;
                     typedef T16 t4 __attribute__((ext_vector_type(4)));
-                    t4 zh;
+                    t4 zh, zl;
 #pragma unroll
-                    for (int v = 0; v < 4; ++v) zh[v] = (T16)zn[v];
+                    for (int v = 0; v < 4; ++v) {
+                        const float zv = PR ? zn[v] * LP_PAIR_Z : zn[v];
+                        zh[v] = (T16)zv;
+                        if (PR) zl[v] = (T16)(zv - (float)zh[v]);
+                    }
                     *(t4 *)(zt + c * zts + i) = zh;
+                    if (PR) *(t4 *)(ztl + c * zts + i) = zl;
                 }
             }
         if (X && nrm) { // fold the lane sums: lane groups, then waves, then the copy
@@ -1751,8 +2033,17 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
                 const int ct = task & 3, kt = task >> 2;
                 const T16 *pz = zt + (ct * 16 + m) * zts + 8 * g;
                 f4 acc = f4{0.f, 0.f, 0.f, 0.f};
+                if (PR) {
+                    const T16 *pl = ztl + (ct * 16 + m) * zts + 8 * g;
 #pragma unroll
-                for (int u = 0; u < NU; ++u) acc = Lp<T16>::mfma(av[p][u], *(const v8 *)(pz + 32 * u), acc);
+                    for (int u = 0; u < NU; ++u)
+                        acc = mfma3<T16>(av[p][u], avl[p][u], *(const v8 *)(pz + 32 * u),
+                                         *(const v8 *)(pl + 32 * u), acc);
+                    acc *= a.isVn / LP_PAIR_Z;
+                } else {
+#pragma unroll
+                    for (int u = 0; u < NU; ++u) acc = Lp<T16>::mfma(av[p][u], *(const v8 *)(pz + 32 * u), acc);
+                }
                 const int k = kt * 16 + 4 * g, kg = k / LP_KG, kl = k - kg * LP_KG;
                 float *pp = part + ((size_t)(st * a.nkg + kg) * a.ncg + blockIdx.y) * LP_BLK +
                             (ct * 16 + m) * LP_KG + kl;
@@ -1813,7 +2104,7 @@ static size_t lp_v2_part(const dplrh_plan *p, const plr_lp *q, const lp_v2 &v, i
     return need;
 }
 
-template <typename T16, int CW>
+template <typename T16, int CW, bool PR>
 static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_lp *q, int n,
                           const double *r, int ldr, const double *pm, int pm_n, double *nrm,
                           hipStream_t st)
@@ -1821,11 +2112,11 @@ static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_
     static bool attr = false;
     if (!attr) {
         // the kernel's static LDS counts against the 160 KB too
-        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_tail<T16, CW>,
+        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_tail<T16, CW, PR>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 159 * 1024));
         attr = true;
     }
-    hipLaunchKernelGGL((k_lp_tail<T16, CW>), grid, dim3(LT_THREADS), lds, st, tt, r, ldr, pm, pm_n,
+    hipLaunchKernelGGL((k_lp_tail<T16, CW, PR>), grid, dim3(LT_THREADS), lds, st, tt, r, ldr, pm, pm_n,
                        q->scale, q->z32, tt.N, q->T, q->t_per_col, n, nrm);
     PLR_HIP(hipGetLastError());
     return 0;
@@ -1834,7 +2125,7 @@ static int lp_launch_tail(const LpTailTab &tt, size_t lds, dim3 grid, const plr_
 // One low-precision solve of the residual r (ldr; pm_n partial maxima of
 // |r| per column in pm) added into X:
 // X = (first ? 0 : X) + max|r| * solve_lp(r / max|r|).
-template <typename T16>
+template <typename T16, bool PR>
 static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, int first,
                        double *X, int ldx, const double *r, int ldr, const double *pm, int pm_n,
                        double *nrm, hipStream_t st)
@@ -1861,10 +2152,17 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
         tt.K64[d] = d < 2 ? q->K64 + q->koff[d] : nullptr;
     }
     tt.D = q->Dh;
+    tt.vlo = tt.ulo = q->off[L];
+    tt.dlo = (size_t)N * p->b;
+    for (int d = 0; d < L; ++d) {
+        tt.isV[d] = q->isV[d];
+        tt.isU[d] = q->isU[d];
+    }
+    tt.isD = q->isD;
     const size_t lds = lpt_lds(v.S, v.cw, v.RT, v.tsegs);
     const dim3 grid(N / v.S, (n + v.cw - 1) / v.cw);
-    rc = v.cw == 64 ? lp_launch_tail<T16, 64>(tt, lds, grid, q, n, r, ldr, pm, pm_n, nrm, st)
-                    : lp_launch_tail<T16, 32>(tt, lds, grid, q, n, r, ldr, pm, pm_n, nrm, st);
+    rc = v.cw == 64 ? lp_launch_tail<T16, 64, PR>(tt, lds, grid, q, n, r, ldr, pm, pm_n, nrm, st)
+                    : lp_launch_tail<T16, 32, PR>(tt, lds, grid, q, n, r, ldr, pm, pm_n, nrm, st);
     if (rc) return rc;
     if (v.d0 == 0) {
         hipLaunchKernelGGL(k_lp_update, dim3((N + 255) / 256, n), dim3(256), 0, st, q->z32, N,
@@ -1876,9 +2174,9 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
     if (!attr) {
         // lpl_lds(64) is 92 KB; the kernel's static LDS (1.25 KB) counts
         // against the 160 KB too
-        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 32>,
+        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 32, PR>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
-        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 64>,
+        PLR_HIP(hipFuncSetAttribute((const void *)k_lp_level<T16, 64, PR>,
                                     hipFuncAttributeMaxDynamicSharedMemorySize, 128 * 1024));
         attr = true;
     }
@@ -1899,13 +2197,16 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
         lt.S = q->Sf + q->soff[d];
         lt.K32 = d >= 2 ? q->K32 + q->koff[d] : nullptr;
         lt.K64 = d < 2 ? q->K64 + q->koff[d] : nullptr;
+        lt.vlo = lt.ulo = q->off[L];
+        lt.isU = q->isU[d];
+        lt.isVn = d > 0 ? q->isV[d - 1] : 1.f;
         if (lt.R <= 32 && lt.Rn <= 32)
-            hipLaunchKernelGGL((k_lp_level<T16, 32>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
-                               lpl_lds(lt.R), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
+            hipLaunchKernelGGL((k_lp_level<T16, 32, PR>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
+                               lpl_lds(lt.R, PR), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
                                q->scale, d == 0 ? X : nullptr, ldx, first, n, nrm);
         else
-            hipLaunchKernelGGL((k_lp_level<T16, 64>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
-                               lpl_lds(lt.R), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
+            hipLaunchKernelGGL((k_lp_level<T16, 64, PR>), dim3(N / LV_RB, ncg), dim3(LV_THREADS),
+                               lpl_lds(lt.R, PR), st, lt, q->T, q->t_per_col, q->z32, N, q->part,
                                q->scale, d == 0 ? X : nullptr, ldx, first, n, nrm);
         PLR_HIP(hipGetLastError());
         if (d > 0 && lt.Rn > 0) {
@@ -1925,7 +2226,9 @@ static int lp_solve_v2_any(const dplrh_plan *p, plr_lp *q, int n, int first, dou
 {
     const lp_v2 v = lp_v2_shape(p, q);
     if (!v.ok) return 1; // shape not covered: the caller takes the first form
-    return q->type == 1
-               ? lp_solve_v2<_Float16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, nrm, st)
-               : lp_solve_v2<__bf16>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
+    switch (q->type) {
+    case 1: return lp_solve_v2<_Float16, false>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
+    case 2: return lp_solve_v2<__bf16, false>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
+    default: return lp_solve_v2<_Float16, true>(p, q, v, n, first, X, ldx, r, ldr, pm, pm_n, nrm, st);
+    }
 }

@@ -111,10 +111,11 @@ class Plan:
               "dplrx_grad_outer")
         return dU, dV, dS[:so[-1]], dD
 
-    LP_TYPES = {"f16": 1, "bf16": 2}
+    LP_TYPES = {"f16": 1, "bf16": 2, "f16x2": 3}
 
     def factor_mixed(self, dtype="f16", stream=None):
-        """Mixed-precision factors (f16 or bf16); returns info."""
+        """Mixed-precision factors (f16, bf16, or f16x2: hi + lo f16 pairs);
+        returns info."""
         info = c_int(0)
         check(lib.dplrx_factor_mixed(self.p, self.LP_TYPES[dtype], byref(info), _stream(stream)),
               "dplrx_factor_mixed")

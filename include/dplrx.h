@@ -72,7 +72,13 @@ int dplrx_solve(dplrx_plan *plan, char trans, int n, const double *dB, int ldb, 
 /* Mixed-precision factors for refined solves: f16 (type 1) or bf16
  * (type 2) U/V factors and leaf inverses with FP32 accumulation on MFMA,
  * FP64 leaf LU and capacitance LU, FP64 capacitance inverses on the top two
- * levels.  Needs the leaf order to be a multiple of 64. */
+ * levels.  Type 3 stores each factor entry as a pair of f16 values (hi + lo
+ * of the entry times a power of two per level, taken from the FP64
+ * factorization, which runs first when the plan has none): three f16 MFMAs
+ * per product carry about 22 bits, so a refined solve to 1e-10 takes two
+ * solves instead of three (f16) or four (bf16); it needs leaves of 64 to
+ * 512 rows and ranks of at most 64.  Needs the leaf order to be a multiple
+ * of 64. */
 int dplrx_factor_mixed(dplrx_plan *plan, int type, int *info, void *stream);
 
 /* X := A^{-1} B by `iters` mixed-precision solves with FP64 residuals

@@ -219,3 +219,40 @@ def test_refined_tol_other_forms(cuda, m, nlvl, rank, ld):
     assert est <= 1e-10 and err <= 1e-10
     assert torch.all(X[2] == 0.0)
     P.close()
+
+
+@pytest.mark.parametrize("m,nlvl,rank,n", [
+    (8192, 7, 32, 16),     # bottom kernel + upper levels
+    (512, 3, 32, 5),       # the bottom kernel alone
+    (8192, 6, 32, 9),      # leaf 128
+    (4096, 6, 20, 3),      # rank padded to 32
+    (65536, 10, 32, 4),    # the bench shape
+])
+def test_f16_pairs_refine_in_two_solves(cuda, orc, m, nlvl, rank, n):
+    """f16 pair storage (hi + lo planes, three MFMAs per product): one solve
+    is good to about 1e-6, so two reach 1e-10; against the FP64 solve and
+    the oracle's residual."""
+    import torch
+    print(f"f16x2 m={m} L={nlvl} r={rank} n={n}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=47)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    assert P.factor_mixed("f16x2") == 0
+    B = np.random.default_rng(11).standard_normal((m, n))
+    Bd = torch.tensor(np.ascontiguousarray(B.T), device=cuda)
+    X64 = torch.zeros_like(Bd)
+    P.solve("N", Bd, X64)
+    ref = X64.cpu().numpy()
+    X1 = torch.zeros_like(Bd)
+    P.solve_refined(Bd, X1, 1)
+    e1 = rel(X1.cpu().numpy(), ref)
+    X = torch.zeros_like(Bd)
+    its, est = P.solve_refined_tol(Bd, X, tol=1e-10, maxit=6)
+    err = rel(X.cpu().numpy(), ref)
+    print(f"  one solve {e1:.2e}; {its} solves, estimate {est:.2e}, error {err:.2e}", flush=True)
+    assert e1 < 2e-5
+    assert its == 2 and est <= 1e-10 and err <= 1e-10
+    if m <= 8192:
+        r = D.gepdmm("N", "N", A, X.cpu().numpy().T.copy(), n, api=orc) - B
+        assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-10
+    P.close()
