@@ -1460,28 +1460,32 @@ __device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, co
                                                int g, size_t vlo, float os, Emit emit)
 {
     typedef typename Lp<T16>::v8 v8;
+    constexpr int QB = PR ? 4 : 8; // loads in flight: pairs take two halves of the batch
     const int nstep = S / 32, spq = seg / 32, total = nstep * nkt * NCT;
     for (int e0 = wv * 8; e0 < total; e0 += 8 * LT_NW) {
-        v8 av[8], al[8];
+      f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-            const int e = e0 + q, grp = e / nstep, rs = e - grp * nstep, kt = grp / NCT;
+      for (int q0 = 0; q0 < 8; q0 += QB) {
+        v8 av[QB], al[QB];
+#pragma unroll
+        for (int qq = 0; qq < QB; ++qq) {
+            const int e = e0 + q0 + qq, grp = e / nstep, rs = e - grp * nstep, kt = grp / NCT;
             const size_t idx = (size_t)(kt * 16 + m) * N + row0 + rs * 32 + 8 * g;
-            av[q] = *(const v8 *)(V + idx);
-            if (PR) al[q] = *(const v8 *)(V + vlo + idx);
+            av[qq] = *(const v8 *)(V + idx);
+            if (PR) al[qq] = *(const v8 *)(V + vlo + idx);
         }
-        f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
+        for (int qq = 0; qq < QB; ++qq) {
+            const int q = q0 + qq;
             const int e = e0 + q, grp = e / nstep, rs = e - grp * nstep;
             const int kt = grp / NCT, ct = grp - kt * NCT;
             const f8 zv = lds8(z + (ct * 16 + m) * zs + rs * 32 + 8 * g);
             if (PR) {
                 v8 zh, zl;
                 split8<T16>(zv, LP_PAIR_Z, zh, zl);
-                acc = mfma3<T16>(av[q], al[q], zh, zl, acc);
+                acc = mfma3<T16>(av[qq], al[qq], zh, zl, acc);
             } else {
-                acc = Lp<T16>::mfma(av[q], cvt8<T16>(zv, 1.f), acc);
+                acc = Lp<T16>::mfma(av[qq], cvt8<T16>(zv, 1.f), acc);
             }
             if ((rs + 1) % spq == 0 && spq <= 8) {
                 emit(rs / spq, kt, ct, PR ? acc * os : acc);
@@ -1490,6 +1494,7 @@ __device__ __forceinline__ void lp_tail_reduce(const T16 *V, int N, int row0, co
                 emit(-1 - (rs % spq) / 8, kt, ct, PR ? acc * os : acc);
             }
         }
+      }
     }
 }
 
@@ -1700,19 +1705,23 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         // batches of eight MFMA steps per wave with their Ut loads together
         {
             const int nks = R / 32, total = (S / 16) * NCT * nks;
+            constexpr int QB = PR ? 4 : 8; // loads in flight (pairs: two halves of the batch)
+            const float os = PR ? a.isU[d] / LP_PAIR_Z : 1.f;
             for (int e0 = wv * 8; e0 < total; e0 += 8 * LT_NW) {
-                v8 bq[8], bl[8];
+              f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
-                    const int e = e0 + q, ks = e % nks, it = (e / nks) / NCT;
+              for (int q0 = 0; q0 < 8; q0 += QB) {
+                v8 bq[QB], bl[QB];
+#pragma unroll
+                for (int qq = 0; qq < QB; ++qq) {
+                    const int e = e0 + q0 + qq, ks = e % nks, it = (e / nks) / NCT;
                     const size_t idx = (size_t)(row0 + it * 16 + m) * R + ks * 32 + 8 * g;
-                    bq[q] = *(const v8 *)(U + idx);
-                    if (PR) bl[q] = *(const v8 *)(U + a.ulo + idx);
+                    bq[qq] = *(const v8 *)(U + idx);
+                    if (PR) bl[qq] = *(const v8 *)(U + a.ulo + idx);
                 }
-                const float os = PR ? a.isU[d] / LP_PAIR_Z : 1.f;
-                f4 acc = f4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                for (int q = 0; q < 8; ++q) {
+                for (int qq = 0; qq < QB; ++qq) {
+                    const int q = q0 + qq;
                     const int e = e0 + q, ks = e % nks, grp = e / nks, it = grp / NCT, ct = grp - it * NCT;
                     const int s = (it * 16) / seg;
                     const unsigned *sm = smax + s * CW + ct * 16;
@@ -1720,9 +1729,9 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                     if (PR) {
                         v8 wh, wl;
                         split8<T16>(w8, LP_PAIR_Z / lp_pow2(sm[m]), wh, wl);
-                        acc = mfma3<T16>(wh, wl, bq[q], bl[q], acc);
+                        acc = mfma3<T16>(wh, wl, bq[qq], bl[qq], acc);
                     } else {
-                        acc = Lp<T16>::mfma(cvt8<T16>(w8, 1.f / lp_pow2(sm[m])), bq[q], acc);
+                        acc = Lp<T16>::mfma(cvt8<T16>(w8, 1.f / lp_pow2(sm[m])), bq[qq], acc);
                     }
                     if (ks == nks - 1) {
 #pragma unroll
@@ -1731,6 +1740,7 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                         acc = f4{0.f, 0.f, 0.f, 0.f};
                     }
                 }
+              }
             }
         }
         __syncthreads();
