@@ -62,18 +62,33 @@ def parse():
     return ap.parse_args()
 
 
-def measured_traffic():
-    """HBM bytes per matvec from the committed counter summary (FETCH_SIZE and
+def measured_traffic(name="matvec_fp64_N65536_L10_r32.md", key="traffic_bytes_per_launch"):
+    """HBM bytes per launch from a committed counter summary (FETCH_SIZE and
     WRITE_SIZE runs scaled by the copy-kernel calibration), or None."""
-    path = os.path.join(ROOT, "profiles", "matvec_fp64_N65536_L10_r32.md")
+    path = os.path.join(ROOT, "profiles", name)
     try:
         with open(path) as f:
             for line in f:
-                if line.startswith("traffic_bytes_per_launch:"):
+                if line.startswith(key + ":"):
                     return int(line.split(":")[1])
     except OSError:
         pass
     return None
+
+
+def lp_solve_bytes(m, leaf, ranks, nrhs, lp, first):
+    """Algorithmic HBM bytes of one low-precision solve of nrhs columns: V, Ut
+    and the leaf inverses in 16 bit (two planes for f16x2), the capacitance
+    inverses (FP64 on the top two levels, FP32 below), sig in FP32, the
+    right-hand side read once in FP64 and x written (first solve) or read and
+    written (later ones) in FP64.  Ranks padded to multiples of 32."""
+    Rp = [((r + 31) // 32) * 32 for r in ranks]
+    planes = 2 if lp == "f16x2" else 1
+    fac = 2 * planes * (2 * m * sum(Rp) + m * leaf)
+    kinv = sum((1 << d) * (2 * R) ** 2 * (8 if d < 2 else 4) for d, R in enumerate(Rp))
+    sig = 4 * sum((2 << d) * R for d, R in enumerate(Rp))
+    io = 8 * m * nrhs * (2 if first else 3)
+    return fac + kinv + sig + io
 
 
 def event_ms(torch, fn, reps):
@@ -478,6 +493,16 @@ def main():
     for i in range(len(plans)):
         fixed(i)
     ms_fixed = event_ms(torch, fixed, 10)
+    # one low-precision solve (its column maxima of B included) on the HBM
+    # roofline: the refinement's dominant kernels
+    one = lambda i: plans[i % len(plans)].solve_refined(B, Z, 1)
+    for i in range(len(plans)):
+        one(i)
+    ms_one = event_ms(torch, one, 20)
+    lp_bytes = lp_solve_bytes(m, plans[0].leaf, plans[0].ranks, nrhs, args.lp, True)
+    lp_gbs = lp_bytes / (ms_one * 1e-3) / 1e9
+    log(f"one {args.lp} solve: {ms_one * 1e3:.0f} us, {lp_bytes / 1e6:.1f} MB algorithmic -> "
+        f"{lp_gbs:.0f} GB/s ({lp_gbs / HBM_PEAK_GBS:.1%} of HBM)")
 
     # --- the timed step ------------------------------------------------------
     def step(i):
@@ -560,6 +585,15 @@ def main():
                            "kernel": "64-RHS FP64 product (k_mm_reduce_fused2 + k_mm_finish<true> "
                                      "+ k_mm_expand3)",
                            "flops_per_launch": flops, "us_per_launch": ms_apply * 1e3},
+        # one low-precision solve of the refinement (the bulk of the step)
+        "roofline_mixed": {"bound": "hbm", "achieved": lp_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                           "frac": lp_gbs / HBM_PEAK_GBS,
+                           "traffic": measured_traffic(f"mixed_solve_{args.lp}_N65536_L10_r32_64rhs.md",
+                                                       "traffic_bytes_per_solve")
+                           if m == 65536 and nrhs == 64 else None,
+                           "kernel": f"one {args.lp} solve (k_lp_colmax + k_lp_tail + 7 x "
+                                     "(k_lp_level + k_lp_finish1))",
+                           "bytes_per_launch": lp_bytes, "us_per_launch": ms_one * 1e3},
         "roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": mv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
                      "kernel": "single-RHS FP64 matvec (k_mv_reduce2 + k_mv_finish + k_mv_expand2)",

@@ -57,6 +57,8 @@ def _load():
         "dplrx_batch_tune": (c_int, [P, c_int, POINTER(P), POINTER(c_int), POINTER(P),
                                      POINTER(c_int), c_int, POINTER(c_int), PD, P]),
         "dplrx_gen_random": (c_int, [c_int, c_int, c_int, c_double, c_ulonglong, PM]),
+        "dplrx_save": (c_int, [ctypes.c_char_p, c_int, PM]),
+        "dplrx_load": (c_int, [ctypes.c_char_p, PM, POINTER(c_int)]),
         "dplrx_gen_normal": (c_int, [c_size_t, c_ulonglong, c_uint, PD]),
     }
     for name, (res, args) in x.items():

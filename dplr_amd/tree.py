@@ -50,6 +50,21 @@ class PlrTree:
 
     # -- construction ------------------------------------------------------
     @classmethod
+    def load(cls, path):
+        """A tree read from a PLR file (dplrx_load)."""
+        t = cls(0)
+        m = c_int(0)
+        if lib.dplrx_load(str(path).encode(), t.ref, byref(m)) != 0:
+            raise RuntimeError(f"dplrx_load failed for {path}")
+        t.m = m.value
+        return t
+
+    def save(self, path):
+        """Write the tree to a PLR file (dplrx_save)."""
+        if lib.dplrx_save(str(path).encode(), self.m, self.ref) != 0:
+            raise RuntimeError(f"dplrx_save failed for {path}")
+
+    @classmethod
     def random(cls, m, nlvl, rank, shift=4.0, seed=0):
         """Seeded generator of the uniform random PLR family (dplrx_gen_random)."""
         t = cls(m)

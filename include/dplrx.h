@@ -151,6 +151,19 @@ int dplrx_batch_set_tile(dplrx_batch *batch, int tile);
 
 const char *dplrx_last_error(void);
 
+/* ---- PLR files (no reference counterpart: the reference keeps trees in
+ * memory only) ----------------------------------------------------------- */
+
+/* Writes the m x m tree A to `path`: header "DPLRTREE", version 1, the
+ * order, then every node in pre-order (type, size; children nw ne se sw;
+ * leaves column-major; low-rank blocks rank, u, v, sig -- valid columns
+ * only), little-endian.  0 or a negative code. */
+int dplrx_save(const char *path, int m, DPlrMtx *A);
+/* Replaces A (dplrini'd) with the tree stored in `path`; *m (optional) gets
+ * its order.  A file that is truncated, has trailing bytes or sizes that do
+ * not halve down the tree is refused (negative code, A left empty). */
+int dplrx_load(const char *path, DPlrMtx *A, int *m);
+
 /* ---- synthetic inputs (deterministic, counter-based RNG) --------------- */
 
 /* Random uniform PLR tree of order m = leaf * 2^nlvl with every off-diagonal

@@ -4,9 +4,14 @@ SciPy) on seeded inputs, so that parity is pinned without the reference.
 
 Inputs are regenerated by the tests from the same seeds: the PLR matrices
 by dplrx_gen_random (deterministic C generator of this library), vectors by
-numpy's PCG64 (stable across numpy versions).  Run in the build container:
+numpy's PCG64 (stable across numpy versions).  The input trees of the two
+smaller cases, and the Gaussian-kernel tree the reference's dplrd2gep
+compressed, are also stored as PLR files (dplrx_save: c1s.plr, deep.plr,
+kernel.plr), so the fixtures do not depend on the generator or on an SVD
+reproducing them.  Run in the build container:
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py            # golden.npz and the trees
+    python tests/golden/make_golden.py --trees    # the PLR files only
 """
 import ctypes
 import os
@@ -23,6 +28,7 @@ from dplr_amd.tree import PlrTree  # noqa: E402
 
 # (name, m, nlvl, rank, seed)
 CASES = [("c1s", 1024, 4, 8, 1), ("deep", 512, 5, 4, 2), ("wide", 2048, 2, 48, 3)]
+TREE_FILES = ("c1s", "deep")   # the wide case's tree (11 MB) is regenerated only
 
 
 def gaussian(m, ell, nugget, seed=0):
@@ -31,10 +37,37 @@ def gaussian(m, ell, nugget, seed=0):
     return np.asfortranarray(A)
 
 
+def kernel_tree(ref):
+    """N=2048, l=0.1, nugget 1e-2, compressed by the reference's dplrd2gep at
+    tol 1e-10 with 5 levels (leaf 64)."""
+    m = 2048
+    G = gaussian(m, 0.1, 1e-2)
+    T = PlrTree(m)
+    info = ctypes.c_int(0)
+    ref["dplrd2gep"](m, G.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, T.ref, 1e-10, 5,
+                     ctypes.byref(info))
+    assert info.value == 0
+    return T
+
+
+def write_trees(ref):
+    for name, m, nlvl, r, seed in CASES:
+        if name in TREE_FILES:
+            A = PlrTree.random(m, nlvl, r, 4.0, seed)
+            A.save(os.path.join(HERE, f"{name}.plr"))
+            A.free()
+    T = kernel_tree(ref)
+    T.save(os.path.join(HERE, "kernel.plr"))
+    T.free()
+
+
 def main():
     ref = oracle.load_ref()
     if ref is None:
         raise SystemExit("oracle/_ref is not built (make -C oracle)")
+    write_trees(ref)
+    if "--trees" in sys.argv:
+        return
     out = {}
     for name, m, nlvl, r, seed in CASES:
         A = PlrTree.random(m, nlvl, r, 4.0, seed)
@@ -59,12 +92,7 @@ def main():
         A.free()
     # kernel matrix: N=2048, l=0.1, nugget 1e-2, dplrd2gep tol 1e-10 (leaf 64)
     m = 2048
-    G = gaussian(m, 0.1, 1e-2)
-    T = PlrTree(m)
-    info = ctypes.c_int(0)
-    ref["dplrd2gep"](m, G.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, T.ref, 1e-10, 5,
-                     ctypes.byref(info))
-    assert info.value == 0
+    T = kernel_tree(ref)
     sig = [T.lr_arrays(nd)[1] for (_, _, _, nd) in T.blocks()]
     out["kernel/ranks"] = np.array([len(s) for s in sig])
     out["kernel/sig"] = np.concatenate(sig)

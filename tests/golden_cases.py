@@ -8,8 +8,28 @@ import numpy as np
 import dplr_amd as D
 from dplr_amd.tree import PlrTree
 
-GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "golden.npz")
+GOLDEN_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GOLDEN = os.path.join(GOLDEN_DIR, "golden.npz")
 CASES = [("c1s", 1024, 4, 8, 1), ("deep", 512, 5, 4, 2), ("wide", 2048, 2, 48, 3)]
+
+
+def case_tree(name, m, nlvl, r, seed):
+    """The input tree of a case: from its PLR file when the fixtures hold one
+    (checked to equal what the generator makes), else from the generator."""
+    A = PlrTree.random(m, nlvl, r, 4.0, seed)
+    path = os.path.join(GOLDEN_DIR, f"{name}.plr")
+    if not os.path.exists(path):
+        return A
+    F = PlrTree.load(path)
+    assert F.m == m and np.array_equal(F.dense(), A.dense()), name
+    A.free()
+    return F
+
+
+def kernel_tree():
+    """The N=2048 Gaussian-kernel tree compressed by the reference's
+    dplrd2gep (tests/golden/kernel.plr)."""
+    return PlrTree.load(os.path.join(GOLDEN_DIR, "kernel.plr"))
 
 
 def rel(a, b):
@@ -17,8 +37,8 @@ def rel(a, b):
 
 
 def replay(name, m, nlvl, r, seed, g, api, tol):
-    """Every fixture of one generated case through `api` (None = product)."""
-    A = PlrTree.random(m, nlvl, r, 4.0, seed)
+    """Every fixture of one case through `api` (None = product)."""
+    A = case_tree(name, m, nlvl, r, seed)
     rng = np.random.default_rng(seed)
     x = rng.standard_normal(m)
     y0 = rng.standard_normal(m)

@@ -154,3 +154,39 @@ def test_empty_and_copy_match_reference(ref):
     assert np.array_equal(C.dense(), dA)
     for t in (A, E1, E2, B1, B2, C):
         t.free()
+
+
+def test_plr_file_round_trip(tmp_path, capfd):
+    """dplrx_save / dplrx_load: a uniform random tree, a tree of uneven ranks
+    (some blocks of rank 0) and a dense one come back node for node and bit
+    for bit; truncated, padded and corrupted files are refused."""
+    from dplr_amd.tree import nodes
+    rng = np.random.default_rng(4)
+    trees = [PlrTree.random(1024, 4, 12, seed=3),
+             PlrTree.from_blocks(512, 3, lambda d, n, w: (2 * d + n + (w == "sw")) % 6, rng)]
+    dense = PlrTree(64)
+    D.lib.dplrinid(64, dense.ref)
+    vals = rng.standard_normal(64 * 64)
+    ctypes.memmove(dense.root.dd.d, vals.ctypes.data, vals.nbytes)
+    trees.append(dense)
+    for k, t in enumerate(trees):
+        path = tmp_path / f"t{k}.plr"
+        t.save(path)
+        u = PlrTree.load(path)
+        assert u.m == t.m
+        assert [n[:3] for n in nodes(u)] == [n[:3] for n in nodes(t)]
+        assert np.array_equal(u.dense(), t.dense())
+        for (_, _, _, p1), (_, _, _, p2) in zip(t.blocks(), u.blocks()):
+            for a, b in zip(t.lr_arrays(p1), u.lr_arrays(p2)):
+                assert np.array_equal(a, b)
+        raw = path.read_bytes()
+        for bad in (raw[:-8], raw + b"\0" * 8, raw[:4] + b"XXXX" + raw[8:],
+                    raw[:24] + (12345).to_bytes(4, "little") + raw[28:]):
+            bp = tmp_path / "bad.plr"
+            bp.write_bytes(bad)
+            v = PlrTree(0)
+            assert D.lib.dplrx_load(str(bp).encode(), v.ref, None) < 0
+            assert v.root.type == 0        # left empty
+        u.free()
+        t.free()
+    assert "not a readable PLR file" in capfd.readouterr().err

@@ -6,7 +6,7 @@ import pytest
 
 import dplr_amd as D
 from dplr_amd.tree import PlrTree
-from golden_cases import CASES, GOLDEN, rel, replay
+from golden_cases import CASES, GOLDEN, kernel_tree, rel, replay
 
 pytestmark = pytest.mark.gpu
 
@@ -49,5 +49,20 @@ def test_kernel_matrix_golden(cuda, golden):
     assert info == 0
     kappa = 5e4
     assert rel(D.gepmv("N", B, b), golden["kernel/inv_b"]) < 50 * kappa * 1e-16 + 1e-9
+    B.free()
+    T.free()
+
+
+def test_kernel_tree_file_golden(cuda, golden):
+    """The reference-compressed kernel tree (tests/golden/kernel.plr): the GPU
+    matvec and the GPU explicit inverse applied to b against the reference's
+    outputs on exactly that tree."""
+    T = kernel_tree()
+    rng = np.random.default_rng(1)
+    v, b = rng.standard_normal(T.m), rng.standard_normal(T.m)
+    assert rel(D.gepmv("N", T, v), golden["kernel/y"]) < 1e-12
+    B, info = D.geinv(T, 0.0)
+    assert info == 0
+    assert rel(D.gepmv("N", B, b), golden["kernel/inv_b"]) < 1e-9
     B.free()
     T.free()
