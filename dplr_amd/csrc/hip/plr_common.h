@@ -81,7 +81,6 @@ struct dplrh_plan
     plr_fact f;
     plr_solve_ws sw;
     int force_valu; // testing aid: use the general-shape kernels everywhere
-    void *blas;     // rocBLAS handle of the factorization (created on first use)
     // HIP graphs of repeated calls (see plr_run_cached)
     unsigned ws_gen;  // bumped whenever plan-owned device memory is reallocated
     hipStream_t cap;  // private capture stream (default priority)
@@ -182,6 +181,8 @@ int plr_mm_finish_raw(dplrh_plan *p, int d, int ncols, int tmax, double *T, size
 size_t plr_mm_part_blocks(const dplrh_plan *p, int d0, int d1, int ncols, int tmax);
 int plr_mm_reduce_raw(dplrh_plan *p, int d1, const double *X, int ldx, int ncols, double *T,
                       size_t ws, hipStream_t st);
+int plr_mm_cross(dplrh_plan *p, int e, const double *X, int ldx, int ncols, double *C,
+                 int *w_off, size_t *wstride, hipStream_t st);
 int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W, size_t ws,
                     double *Y, int ldy, int ncols, hipStream_t st);
 // Reassociated upper levels of the FP64 solve (plr_up.hip).

@@ -471,16 +471,21 @@ __global__ void k_lp_pair_leaf(const double *src, int b, size_t nleaf, double s,
     pair_of(src[l * bb + (size_t)j * b + i] * s, hi, lo, e);
 }
 
-// largest |x| of n entries, as float bits (unsigned order = numeric order)
-__global__ void k_lp_maxabs(const double *x, size_t n, unsigned *out)
+// largest |x| of n entries, as float bits (unsigned order = numeric
+// order), folded into *out: a wave maximum, the block's four through LDS,
+// one atomic per block
+__global__ __launch_bounds__(256) void k_lp_maxabs(const double *x, size_t n, unsigned *out)
 {
+    __shared__ unsigned red[4];
     unsigned mv = 0u;
     for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n;
          e += (size_t)gridDim.x * blockDim.x)
         mv = max(mv, __float_as_uint(fabsf((float)x[e])));
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) mv = max(mv, (unsigned)__shfl_xor((int)mv, o, 64));
-    if ((threadIdx.x & 63) == 0) atomicMax(out, mv);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mv;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicMax(out, max(max(red[0], red[1]), max(red[2], red[3])));
 }
 
 // inv(K) of every node with its ranks padded from R to Rp: K's index i < R
@@ -914,22 +919,35 @@ static int lp_factor_pair(dplrh_plan *p, plr_lp *q, int *info, hipStream_t st)
         if (rc || *info) return rc;
     }
     const int N = p->N, L = p->L, b = p->b;
+    // largest |x| of V_d, Ut_d (every level) and the leaf inverses, read
+    // back at once; slot 2d: V_d, 2d + 1: Ut_d, 2L: leaves
+    const int nslot = 2 * L + 1;
     unsigned *dmax = nullptr;
-    PLR_HIP(hipMalloc(&dmax, sizeof(unsigned)));
-    int rc = 0;
-    // the power of two s with s * max|x| in (LP_PAIR_TOP / 2, LP_PAIR_TOP]
-    auto scale_of = [&](const double *x, size_t n, double *s) -> int {
-        *s = 1.0;
+    PLR_HIP(hipMalloc(&dmax, nslot * sizeof(unsigned)));
+    PLR_HIP(hipMemsetAsync(dmax, 0, nslot * sizeof(unsigned), st));
+    auto max_of = [&](const double *x, size_t n, int slot) -> int {
         if (n == 0) return 0;
-        PLR_HIP(hipMemsetAsync(dmax, 0, sizeof(unsigned), st));
-        const size_t blocks = (n + 255) / 256 < 2048 ? (n + 255) / 256 : 2048;
-        hipLaunchKernelGGL(k_lp_maxabs, dim3((unsigned)blocks), dim3(256), 0, st, x, n, dmax);
+        const size_t blocks = (n + 255) / 256 < 256 ? (n + 255) / 256 : 256;
+        hipLaunchKernelGGL(k_lp_maxabs, dim3((unsigned)blocks), dim3(256), 0, st, x, n, dmax + slot);
         PLR_HIP(hipGetLastError());
-        unsigned h = 0;
-        PLR_HIP(hipMemcpyAsync(&h, dmax, sizeof(unsigned), hipMemcpyDeviceToHost, st));
+        return 0;
+    };
+    int rc = 0;
+    for (int d = 0; d < L && !rc; ++d) {
+        const size_t n = (size_t)N * p->R[d];
+        if (!(rc = max_of(p->V + p->uoff[d], n, 2 * d))) rc = max_of(p->f.Ut + p->uoff[d], n, 2 * d + 1);
+    }
+    if (!rc) rc = max_of(p->f.Dinv, (size_t)N * b, 2 * L);
+    std::vector<unsigned> hmax(nslot, 0u);
+    if (!rc) {
+        PLR_HIP(hipMemcpyAsync(hmax.data(), dmax, nslot * sizeof(unsigned), hipMemcpyDeviceToHost, st));
         PLR_HIP(hipStreamSynchronize(st));
+    }
+    // the power of two s with s * max|x| in (LP_PAIR_TOP / 2, LP_PAIR_TOP]
+    auto scale_of = [&](int slot, double *s) -> int {
+        *s = 1.0;
         float mx;
-        memcpy(&mx, &h, sizeof(mx));
+        memcpy(&mx, &hmax[slot], sizeof(mx));
         if (mx > 0.f && mx < INFINITY) {
             int e;
             (void)frexp((double)LP_PAIR_TOP / (double)mx, &e);
@@ -945,8 +963,9 @@ static int lp_factor_pair(dplrh_plan *p, plr_lp *q, int *info, hipStream_t st)
         const size_t n = (size_t)N * R, np = (size_t)N * Rp;
         const unsigned gp = (unsigned)((np + 255) / 256);
         double sV = 1.0, sU = 1.0;
-        if ((rc = scale_of(p->V + p->uoff[d], n, &sV))) break;
-        if ((rc = scale_of(p->f.Ut + p->uoff[d], n, &sU))) break;
+        (void)n;
+        scale_of(2 * d, &sV);
+        scale_of(2 * d + 1, &sU);
         if (np > 0) {
             hipLaunchKernelGGL(k_lp_pair_cols, dim3(gp), dim3(256), 0, st, p->V + p->uoff[d], R, Rp,
                                N, sV, Vhi + q->off[d], Vlo + q->off[d]);
@@ -975,7 +994,7 @@ static int lp_factor_pair(dplrh_plan *p, plr_lp *q, int *info, hipStream_t st)
     if (!rc) {
         double sD = 1.0;
         const size_t nd = (size_t)N * b;
-        rc = scale_of(p->f.Dinv, nd, &sD);
+        rc = scale_of(2 * L, &sD);
         if (!rc) {
             hipLaunchKernelGGL(k_lp_pair_leaf, dim3((unsigned)((nd + 255) / 256)), dim3(256), 0, st,
                                p->f.Dinv, b, (size_t)(N / b), sD, Dhi, Dlo);

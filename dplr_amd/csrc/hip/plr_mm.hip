@@ -149,6 +149,71 @@ static int mm_part(dplrh_plan *p, size_t blocks)
 
 int plr_mm_part(dplrh_plan *p, size_t blocks) { return mm_part(p, blocks); }
 
+// C[c * wstride + w_off[d] + s * R_d + k] = sum over segment s of level e of
+// V_d[k][i] X[c][i], for every d < e at once: the per-level reduction with a
+// table whose entries are the coarser levels' V, each cut into level e's
+// segments (rows s * seg_e .. +seg_e), over ncols <= 64 columns of X.
+// w_off (e + 1 entries) receives the offsets, *wstride their total.
+int plr_mm_cross(dplrh_plan *p, int e, const double *X, int ldx, int ncols, double *C,
+                 int *w_off, size_t *wstride, hipStream_t st)
+{
+    if (e <= 0 || ncols <= 0 || ncols > MM_CG) {
+        plr_set_error("plr_mm_cross: level %d, %d columns", e, ncols);
+        return -5;
+    }
+    MmTab t;
+    memset(&t, 0, sizeof(t));
+    const int N = p->N, seg = N >> (e + 1), nseg = 2 << e;
+    const int T = seg < MM_TMAX ? seg : MM_TMAX, tps = seg / T;
+    t.L = e;
+    t.N = N;
+    t.b = p->b;
+    t.ncols = ncols;
+    t.ncg = 1;
+    int task = 0, part = 0, fin = 0, woff = 0;
+    for (int d = 0; d < e; ++d) {
+        const int R = p->R[d], nkg = (R + MM_KG - 1) / MM_KG;
+        t.R[d] = R;
+        t.seglen[d] = seg;
+        t.T[d] = T;
+        t.tps[d] = tps;
+        t.nkg[d] = nkg;
+        t.task_off[d] = task;
+        task += nkg * nseg * tps;
+        t.part_off[d] = part;
+        if (tps > 1) part += nkg * nseg * tps;
+        t.fin_off[d] = fin;
+        if (tps > 1) fin += nkg * nseg;
+        t.w_off[d] = woff;
+        w_off[d] = woff;
+        woff += nseg * R;
+        t.P[d] = p->V + p->uoff[d];
+    }
+    t.task_off[e] = task;
+    t.part_off[e] = part;
+    t.fin_off[e] = fin;
+    t.w_off[e] = woff;
+    w_off[e] = woff;
+    *wstride = (size_t)woff;
+    int rc = mm_part(p, (size_t)part);
+    if (rc) return rc;
+    if (task > 0) {
+        if (T >= 256)
+            hipLaunchKernelGGL(k_mm_reduce<4>, dim3(task), dim3(256), 0, st, t, X, ldx, 1.0, 0, 1,
+                               p->tmp, C, (size_t)woff);
+        else
+            hipLaunchKernelGGL(k_mm_reduce<1>, dim3(task), dim3(64), 0, st, t, X, ldx, 1.0, 0, 1,
+                               p->tmp, C, (size_t)woff);
+        PLR_HIP(hipGetLastError());
+    }
+    if (fin > 0) {
+        hipLaunchKernelGGL(k_mm_finish<false>, dim3(fin * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1,
+                           p->tmp, C, (size_t)woff);
+        PLR_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
 // Reduce-finish-expand over all levels for ncols columns (the multi-RHS
 // product): Y := alpha*op(A)*X + beta*Y.  w must hold soff[L] doubles per
 // column (stride wstride).

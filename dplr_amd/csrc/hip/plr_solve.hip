@@ -217,6 +217,96 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const double *A, int n, size
     if (tid == 0) info[blockIdx.x] = s_info;
 }
 
+// Register Gauss-Jordan for orders n <= NM: one wavefront per matrix, lane
+// i holding row i of the matrix in NM registers (rows and columns past n
+// are those of the identity, which the steps leave alone).  Rows are not
+// moved: each lane carries its logical position, and step k takes as pivot
+// the lane of largest |A[.][k]| among positions >= k (ties: the smaller
+// position), exchanging positions with the lane at k -- the pivot sequence
+// of LU with partial pivoting, so info is dgetrf's.  The pivot row reaches
+// the other lanes by lane reads.  Columns rotate through the registers,
+// eight at a time, so the step's column is always in one of eight fixed
+// registers.  At the end lane L holds row pos[L] of inv(PA), and column k
+// of inv(PA) is column perm[k] of inv(A), perm[k] the lane at position k.
+__device__ __forceinline__ double gj_rdlane(double x, int l)
+{
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), l);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), l);
+    return __hiloint2double(hi, lo);
+}
+
+template <int NM>
+__global__ __launch_bounds__(256) void k_gj_inverse_w(const double *A, int n, size_t stride,
+                                                      double *Inv, size_t istride, int *info,
+                                                      int count)
+{
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int mat = blockIdx.x * 4 + wv;
+    if (mat >= count) return; // the whole wavefront leaves together
+    const double *src = A + (size_t)mat * stride;
+    double a[NM];
+#pragma unroll
+    for (int j = 0; j < NM; ++j)
+        a[j] = (lane < n && j < n) ? src[(size_t)j * n + lane] : (j == lane ? 1.0 : 0.0);
+    int pos = lane, sinfo = 0;
+    const int steps = (n + 7) & ~7;
+    for (int k0 = 0; k0 < steps; k0 += 8) {
+#pragma unroll
+        for (int c = 0; c < 8; ++c) {
+            const int k = k0 + c;
+            // pivot: largest |a[c]| over positions >= k, ties to the smaller
+            // position (NaN ranks below every number: one order for all lanes)
+            double v = pos >= k ? fabs(a[c]) : -1.0;
+            if (v != v) v = -0.5;
+            int key = pos;
+#pragma unroll
+            for (int o = 1; o < 64; o <<= 1) {
+                const double v2 = __shfl_xor(v, o, 64);
+                const int k2 = __shfl_xor(key, o, 64);
+                if (v2 > v || (v2 == v && k2 < key)) {
+                    v = v2;
+                    key = k2;
+                }
+            }
+            const bool piv = pos == key;
+            const int pl = __ffsll((unsigned long long)__ballot(piv)) - 1;
+            if (pos == k)
+                pos = key;
+            else if (piv)
+                pos = k;
+            const double pv = gj_rdlane(a[c], pl);
+            if (pv == 0.0 && sinfo == 0) sinfo = k + 1;
+            const double inv = 1.0 / pv;
+            const double f = a[c];
+#pragma unroll
+            for (int j = 0; j < NM; ++j) {
+                if (j == c) continue;
+                const double pj = gj_rdlane(a[j], pl) * inv;
+                a[j] = piv ? pj : a[j] - f * pj;
+            }
+            a[c] = piv ? inv : -f * inv;
+        }
+        // rotate eight columns: the next block's columns into a[0..7]
+        double t[8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) t[j] = a[j];
+#pragma unroll
+        for (int j = 0; j < NM - 8; ++j) a[j] = a[j + 8];
+#pragma unroll
+        for (int j = 0; j < 8; ++j) a[NM - 8 + j] = t[j];
+    }
+    // register j now holds column (j + steps) % NM; column col of inv(PA)
+    // is column perm[col] of inv(A), perm[col] the lane at position col
+    double *dst = Inv + (size_t)mat * istride;
+#pragma unroll
+    for (int j = 0; j < NM; ++j) {
+        const int col = (j + steps) % NM;
+        const int pc = __ffsll((unsigned long long)__ballot(pos == col)) - 1;
+        if (col < n && lane < n) dst[(size_t)pc * n + pos] = a[j];
+    }
+    if (lane == 0) info[mat] = sinfo;
+}
+
 // Largest order the LDS Gauss-Jordan kernel takes on this device (0 if the
 // dynamic-LDS limit cannot be raised).
 static int gj_max_order()
@@ -247,6 +337,17 @@ static int batched_inverse(double *A, int n, int count, double *Inv, int *piv, i
                            hipStream_t st)
 {
     if (count <= 0 || n <= 0) return 0;
+    if (n <= 64) {
+        const dim3 grid((count + 3) / 4);
+        if (n <= 32)
+            hipLaunchKernelGGL(k_gj_inverse_w<32>, grid, dim3(256), 0, st, A, n, (size_t)n * n, Inv,
+                               (size_t)n * n, info, count);
+        else
+            hipLaunchKernelGGL(k_gj_inverse_w<64>, grid, dim3(256), 0, st, A, n, (size_t)n * n, Inv,
+                               (size_t)n * n, info, count);
+        PLR_HIP(hipGetLastError());
+        return 0;
+    }
     if (n <= gj_max_order()) {
         hipLaunchKernelGGL(k_gj_inverse, dim3(count), dim3(256), (size_t)n * n * sizeof(double), st,
                            A, n, (size_t)n * n, Inv, (size_t)n * n, info);
@@ -408,7 +509,10 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         PLR_HIP(hipMalloc(&f.Ut, (p->uoff[L] ? p->uoff[L] : 1) * sizeof(double)));
         PLR_HIP(hipMalloc(&f.Kinv, (kk ? kk : 1) * sizeof(double)));
         PLR_HIP(hipMalloc(&f.work, (work + piv / 2 + 1) * sizeof(double) + piv * sizeof(int)));
-        PLR_HIP(hipMalloc(&f.dinfo, (size_t)maxnodes * sizeof(int)));
+        // LU status of every leaf, then of every node level by level (level d
+        // at nleaf + 2^d - 1), read once at the end
+        (void)maxnodes;
+        PLR_HIP(hipMalloc(&f.dinfo, ((size_t)nleaf + ((size_t)1 << L)) * sizeof(int)));
     }
     f.done = 0;
     ++p->ws_gen; // factors are rewritten: graphs replaying solves stay valid, but be safe
@@ -430,9 +534,6 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         int rc = batched_inverse(work, b, nleaf, f.Dinv, piv, f.dinfo, st);
         if (rc) return rc;
     }
-    if (read_info(nleaf, f.dinfo, h, st)) return -10;
-    for (int l = 0; l < nleaf; ++l)
-        if (h[l]) fails.push_back({l, 0, h[l]});
 
     // 2. Ut = Dinv U (all levels' columns)
     const int ctot = (int)col_off(p, L);
@@ -470,10 +571,9 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         hipLaunchKernelGGL(k_capf_build, dim3(nodes), dim3(256), 0, st, p->sw.T, p->sw.t_per_col,
                            off, R, Sd, work);
         PLR_HIP(hipGetLastError());
-        if ((rc = batched_inverse(work, n2, nodes, f.Kinv + f.koff[d], piv, f.dinfo, st))) return rc;
-        if (read_info(nodes, f.dinfo, h, st)) return -10;
-        for (int n = 0; n < nodes; ++n)
-            if (h[n]) fails.push_back({((long)(n + 1) << (L - d)) - 1, L - d, h[n]});
+        if ((rc = batched_inverse(work, n2, nodes, f.Kinv + f.koff[d], piv,
+                                  f.dinfo + nleaf + (nodes - 1), st)))
+            return rc;
         if (off > 0) {
             if ((rc = launch_cap(f.Kinv + f.koff[d], d, R, Sd, p->sw.T, p->sw.t_per_col, p->sw.W, off,
                                  0, st)))
@@ -485,6 +585,17 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
                 rc = plr_launch_expand(t1, nullptr, nullptr, N, 1.0, 1.0, 0, p->sw.W,
                                        p->sw.t_per_col, f.Ut, N, off, st);
             if (rc) return rc;
+        }
+    }
+    // every LU status at once: leaves, then the nodes of each level with a rank
+    if (read_info(nleaf + (1 << L) - 1, f.dinfo, h, st)) return -10;
+    for (int l = 0; l < nleaf; ++l)
+        if (h[l]) fails.push_back({l, 0, h[l]});
+    for (int d = 0; d < L; ++d) {
+        if (p->R[d] == 0) continue;
+        for (int n = 0; n < (1 << d); ++n) {
+            const int v = h[nleaf + (1 << d) - 1 + n];
+            if (v) fails.push_back({((long)(n + 1) << (L - d)) - 1, L - d, v});
         }
     }
     // the cross-level blocks of the reassociated upper solve

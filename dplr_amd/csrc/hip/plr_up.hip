@@ -17,7 +17,6 @@
 // expansion pass (z += sum_d Ut_d W_d): z is read twice and written once
 // instead of once per level.  The same quantities as the level-by-level
 // order are computed, summed in another order.
-#include <rocblas/rocblas.h>
 
 #include <cstdint>
 #include <cstdio>
@@ -217,30 +216,25 @@ static size_t up_arows(const dplrh_plan *p, int e, size_t *off)
     return r;
 }
 
-// G[s][i] = sum over j < per of C[(s*per + j)][i], i < mn (fixed order).
-__global__ void k_up_gsum(const double *__restrict__ C, int per, int nblk, int mn,
-                          double *__restrict__ G)
+// G_{d,e}[s][k + j R_d] = C[j * wstride + w_off + s * R_d + k]: the layout
+// of plr_mm_cross's output turned into R_d x R_e column-major blocks.
+__global__ void k_up_greshape(const double *__restrict__ C, size_t wstride, int w_off, int Rd,
+                              int Re, int nseg, double *__restrict__ G)
 {
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)nblk * mn) return;
-    const size_t s = e / mn, i = e - s * mn;
-    const double *c = C + s * per * mn + i;
-    double t = 0.0;
-    for (int j = 0; j < per; ++j) t += c[(size_t)j * mn];
-    G[e] = t;
+    const size_t blk = (size_t)Rd * Re;
+    if (e >= (size_t)nseg * blk) return;
+    const size_t s = e / blk, r = e - s * blk;
+    const int j = (int)(r / Rd), k = (int)(r - (size_t)j * Rd);
+    G[e] = C[(size_t)j * wstride + w_off + s * Rd + k];
 }
 
-// After the factorization: the G blocks.  A segment of level e is long (up
-// to N/4 rows) and G is only R_d x R_e, so each segment is cut into chunks
-// of at most 512 rows: one rocBLAS strided-batched GEMM over all chunks of
-// the level (N/chunk small products), then the chunks of a segment summed
-// in order.
-void plr_up_release(dplrh_plan *p)
-{
-    if (p->blas) rocblas_destroy_handle((rocblas_handle)p->blas);
-    p->blas = nullptr;
-}
+void plr_up_release(dplrh_plan *p) { (void)p; }
 
+// After the factorization: the G blocks.  For each level e one reduction of
+// Ut_e's columns against the V factors of every coarser level, cut into
+// level e's segments (the FP64 MFMA reduction kernels with a table of those
+// levels, tiles of 256 rows summed in a fixed order), then reshaped.
 int plr_up_factor(dplrh_plan *p, hipStream_t st)
 {
     plr_fact &f = p->f;
@@ -255,47 +249,23 @@ int plr_up_factor(dplrh_plan *p, hipStream_t st)
             tot += ((size_t)2 << e) * p->R[d] * p->R[e];
         }
     if (!f.G) PLR_HIP(hipMalloc(&f.G, tot * sizeof(double)));
-    if (!p->blas) {
-        rocblas_handle hn = nullptr;
-        if (rocblas_create_handle(&hn) != rocblas_status_success) {
-            plr_set_error("rocblas_create_handle failed");
-            return -1;
-        }
-        p->blas = hn;
-    }
-    rocblas_handle h = (rocblas_handle)p->blas;
-    rocblas_set_stream(h, st);
-    const double one = 1.0, zero = 0.0;
-    int rc = 0;
-    double *C = f.work; // free once the factorization is done: N*b >= (N/chunk) * R^2 doubles
-    for (int e = 1; e < d0 && !rc; ++e) {
-        const int seg = N >> (e + 1), nseg = 2 << e, Re = p->R[e];
-        int chunk = seg & -seg;
-        chunk = chunk > 512 ? 512 : chunk;
-        const int per = seg / chunk, nch = nseg * per;
-        for (int d = 0; d < e && !rc; ++d) {
+    double *C = f.work; // free once the factorization is done; it holds N * b doubles
+    for (int e = 1; e < d0; ++e) {
+        const int Re = p->R[e], nseg = 2 << e;
+        int w_off[PLR_MAXL + 1];
+        size_t ws = 0;
+        int rc = plr_mm_cross(p, e, f.Ut + p->uoff[e], N, Re, C, w_off, &ws, st);
+        if (rc) return rc;
+        for (int d = 0; d < e; ++d) {
             const int Rd = p->R[d];
-            // C[j] = V_d(rows of chunk j)^T Ut_e(rows of chunk j); G[s] = sum of its chunks
-            rocblas_status s = rocblas_dgemm_strided_batched(
-                h, rocblas_operation_transpose, rocblas_operation_none, Rd, Re, chunk, &one,
-                p->V + p->uoff[d], N, chunk, f.Ut + p->uoff[e], N, chunk, &zero, C, Rd,
-                (rocblas_stride)Rd * Re, nch);
-            if (s != rocblas_status_success) {
-                plr_set_error("rocblas_dgemm_strided_batched failed");
-                rc = -1;
-                break;
-            }
-            const size_t tot = (size_t)nseg * Rd * Re;
-            hipLaunchKernelGGL(k_up_gsum, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st, C,
-                               per, nseg, Rd * Re, f.G + f.goff[d][e]);
-            if (hipGetLastError() != hipSuccess) {
-                plr_set_error("k_up_gsum launch failed");
-                rc = -1;
-            }
+            const size_t n = (size_t)nseg * Rd * Re;
+            hipLaunchKernelGGL(k_up_greshape, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C,
+                               ws, w_off[d], Rd, Re, nseg, f.G + f.goff[d][e]);
+            PLR_HIP(hipGetLastError());
         }
     }
-    if (!rc) f.up_d0 = d0;
-    return rc;
+    f.up_d0 = d0;
+    return 0;
 }
 
 static int up_ws(dplrh_plan *p, int d0, int ncols)
