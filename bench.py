@@ -207,10 +207,12 @@ def batch_specs(count, seed=7):
 
 
 def batch_leg(args, torch, D, log):
-    """Many independent matrices in one go: FP64 apply + FP64 factored solve of
-    16 RHS per matrix for every matrix of the batch, one launch per phase,
-    replayed from a HIP graph.  Per-matrix latency from the same work done
-    one matrix at a time."""
+    """Many independent matrices in one go: apply + factored solve of 16 RHS
+    per matrix for every matrix of the batch, one launch per phase, replayed
+    from a HIP graph, in FP64, in FP32 (FP32 factors on FP32 MFMA), and with
+    BF16 factors refined by FP32 residuals (1 to 3 solves); tiles from the
+    autotuner.  Per-matrix latency from the same work done one matrix at a
+    time."""
     import numpy as np
     nrhs = 16
     specs = batch_specs(args.batch_count)
@@ -275,6 +277,64 @@ def batch_leg(args, torch, D, log):
     log(f"batch: {out['rhs_s']:.0f} RHS/s ({ms:.2f} ms per batch of {total} RHS; apply "
         f"{ms_apply:.2f} ms, solve {ms_solve:.2f} ms); max residual {err:.1e}; one matrix at a "
         f"time p50 {out['per_matrix_ms_p50']:.3f} ms p99 {out['per_matrix_ms_p99']:.3f} ms")
+
+    # FP32 (FP32 factors, FP32 MFMA) and BF16 factors refined with FP32
+    # residuals, on the same matrices and right-hand sides
+    for P in plans:
+        assert P.factor_single("f32") == 0 and P.factor_single("bf16") == 0
+    X32 = [x.float() for x in X]
+    Y32 = [torch.zeros_like(x) for x in X32]
+    Z32 = [torch.zeros_like(x) for x in X32]
+    tile32, tile32_ms = B.tune(X32, Y32, reps=3, prec="f32")
+
+    def both32(_):
+        B.gepdmm(X32, Y32, prec="f32")
+        B.solve(X32, Z32, prec="f32")
+    for i in range(3):
+        both32(i)
+    ms32 = event_ms(torch, both32, 10)
+    ms32_apply = event_ms(torch, lambda i: B.gepdmm(X32, Y32, prec="f32"), 10)
+    ms32_solve = event_ms(torch, lambda i: B.solve(X32, Z32, prec="f32"), 10)
+
+    def rel_vs64(Zs):
+        return max(float((z.double() - z64).norm() / z64.norm()) for z, z64 in zip(Zs, Z))
+    err32 = rel_vs64(Z32)
+    lat32 = []
+    for P, x, y, z in zip(plans, X32, Y32, Z32):
+        def one32(_):
+            P.gepdmm_single("N", 1.0, x, 0.0, y)
+            P.solve_single("N", x, z)
+        for i in range(2):
+            one32(i)
+        lat32.append(event_ms(torch, one32, 3))
+    lat32 = np.array(lat32)
+    f32 = {"dtype": "f32", "rhs_s": total / (ms32 * 1e-3), "ms_per_batch": ms32,
+           "apply_ms": ms32_apply, "solve_ms": ms32_solve, "rel_err_vs_fp64_solve": err32,
+           "tile_rows": tile32, "tuner_ms": tile32_ms,
+           "per_matrix_ms_p50": float(np.percentile(lat32, 50)),
+           "per_matrix_ms_p99": float(np.percentile(lat32, 99))}
+    log(f"batch FP32: {f32['rhs_s']:.0f} RHS/s ({ms32:.2f} ms; apply {ms32_apply:.2f} ms, solve "
+        f"{ms32_solve:.2f} ms, tile {tile32}); error vs FP64 {err32:.1e}; one at a time p50 "
+        f"{f32['per_matrix_ms_p50']:.3f} ms p99 {f32['per_matrix_ms_p99']:.3f} ms")
+    refined = []
+    for iters in (1, 2, 3):
+        ZR = [torch.zeros_like(x) for x in X32]
+
+        def both_r(_):
+            B.gepdmm(X32, Y32, prec="f32")
+            B.solve_refined(X32, ZR, iters)
+        for i in range(3):
+            both_r(i)
+        msr = event_ms(torch, both_r, 10)
+        msr_solve = event_ms(torch, lambda i: B.solve_refined(X32, ZR, iters), 10)
+        errr = rel_vs64(ZR)
+        refined.append({"dtype": "bf16 refined with f32 residuals", "solves": iters,
+                        "rhs_s": total / (msr * 1e-3), "ms_per_batch": msr,
+                        "solve_ms": msr_solve, "rel_err_vs_fp64_solve": errr})
+        log(f"batch BF16 refined ({iters} solves): {total / (msr * 1e-3):.0f} RHS/s ({msr:.2f} ms; "
+            f"solve {msr_solve:.2f} ms); error vs FP64 {errr:.1e}")
+    out["f32"] = f32
+    out["bf16_refined"] = refined
     B.close()
     for P in plans:
         P.close()
@@ -380,6 +440,37 @@ def grad_leg(args, torch, D, log):
            "forward_backward_ms": ms_fb, "grad_b_vs_2y": err}
     log(f"gradient leg: N={m} ranks {ranks}; compress {t_cmp:.1f} s, factor {t_fac * 1e3:.1f} ms; "
         f"forward {ms_f:.3f} ms, forward+backward {ms_fb:.3f} ms; |dq/db - 2y|/|2y| = {err:.1e}")
+
+    # FP32: the same forward and backward on the plan's FP32 factors
+    assert P.factor_single("f32") == 0
+    b32 = b.float()
+    q64 = float(fwd(0))
+    g64 = g.clone()
+    fac64 = [x.clone() for x in P.last_factor_grad]
+
+    def fwd32(_):
+        return (b32 * plr_solve(b32, P)).sum()
+
+    def fwd_bwd32(_):
+        bb = b32.clone().requires_grad_(True)
+        q = (bb * plr_solve(bb, P)).sum()
+        q.backward()
+        return bb.grad
+    for i in range(3):
+        fwd_bwd32(i)
+    ms_f32 = event_ms(torch, fwd32, 10)
+    ms_fb32 = event_ms(torch, fwd_bwd32, 10)
+    g32 = fwd_bwd32(0)
+    fac32 = P.last_factor_grad
+    q32 = float(fwd32(0))
+    fac_err = max(float((a.double() - b_).norm() / b_.norm()) for a, b_ in zip(fac32, fac64))
+    out["f32"] = {"forward_ms": ms_f32, "forward_backward_ms": ms_fb32,
+                  "q_rel_err_vs_fp64": abs(q32 - q64) / abs(q64),
+                  "grad_b_rel_err_vs_fp64": float((g32.double() - g64).norm() / g64.norm()),
+                  "factor_grad_rel_err_vs_fp64": fac_err}
+    log(f"gradient leg FP32: forward {ms_f32:.3f} ms, forward+backward {ms_fb32:.3f} ms; vs FP64: "
+        f"q {out['f32']['q_rel_err_vs_fp64']:.1e}, dq/db {out['f32']['grad_b_rel_err_vs_fp64']:.1e}, "
+        f"factor gradients {fac_err:.1e}")
     P.close()
     return out
 

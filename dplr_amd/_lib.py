@@ -5,7 +5,8 @@ fallback.  Importing this module raises when the library has not been built.
 """
 import ctypes
 import os
-from ctypes import POINTER, c_char, c_double, c_int, c_long, c_size_t, c_uint, c_ulonglong, c_void_p
+from ctypes import (POINTER, c_char, c_double, c_float, c_int, c_long, c_size_t, c_uint,
+                    c_ulonglong, c_void_p)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libdplr_hip.so")
@@ -56,6 +57,17 @@ def _load():
         "dplrx_batch_set_tile": (c_int, [P, c_int]),
         "dplrx_batch_tune": (c_int, [P, c_int, POINTER(P), POINTER(c_int), POINTER(P),
                                      POINTER(c_int), c_int, POINTER(c_int), PD, P]),
+        "dplrx_factor_single": (c_int, [P, c_int, POINTER(c_int), P]),
+        "dplrx_gepdmm_single": (c_int, [P, c_int, c_char, c_int, c_float, P, c_int, c_float, P,
+                                        c_int, P]),
+        "dplrx_solve_single": (c_int, [P, c_int, c_char, c_int, P, c_int, P, c_int, P]),
+        "dplrx_grad_outer_f32": (c_int, [P, c_int, c_float, P, c_int, P, c_int, P, P, P, P, P]),
+        "dplrx_batch_op": (c_int, [P, c_int, c_int, c_char, c_int, POINTER(P), POINTER(c_int),
+                                   POINTER(P), POINTER(c_int), c_double, c_double, P]),
+        "dplrx_batch_solve_refined": (c_int, [P, c_int, POINTER(P), POINTER(c_int), POINTER(P),
+                                              POINTER(c_int), c_int, P]),
+        "dplrx_batch_tune_prec": (c_int, [P, c_int, c_int, POINTER(P), POINTER(c_int), POINTER(P),
+                                          POINTER(c_int), c_int, POINTER(c_int), PD, P]),
         "dplrx_gen_random": (c_int, [c_int, c_int, c_int, c_double, c_ulonglong, PM]),
         "dplrx_save": (c_int, [ctypes.c_char_p, c_int, PM]),
         "dplrx_load": (c_int, [ctypes.c_char_p, PM, POINTER(c_int)]),

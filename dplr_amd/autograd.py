@@ -8,6 +8,9 @@ PlrSolveFn:  y = A^{-1} b for a factored Plan.  backward gives
              plan.last_factor_grad = (dU, dV, dS, dD) (plan layouts).
 PlrApplyFn:  y = A x.  backward gives xbar = A^T ybar and the factor
              gradient of G = ybar x^T.
+float32 inputs run the FP32 path (the plan's FP32 factors from
+factor_single("f32"): dplrx_solve_single, dplrx_gepdmm_single and
+dplrx_grad_outer_f32); float64 inputs the FP64 one.
 """
 import torch
 
@@ -16,7 +19,10 @@ class PlrSolveFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, b, plan):
         y = torch.zeros_like(b)
-        plan.solve("N", b.contiguous(), y)
+        if b.dtype == torch.float32:
+            plan.solve_single("N", b.contiguous(), y)
+        else:
+            plan.solve("N", b.contiguous(), y)
         ctx.plan = plan
         ctx.save_for_backward(y)
         return y
@@ -26,8 +32,12 @@ class PlrSolveFn(torch.autograd.Function):
         (y,) = ctx.saved_tensors
         plan = ctx.plan
         lam = torch.zeros_like(ybar)
-        plan.solve("T", ybar.contiguous(), lam)
-        plan.last_factor_grad = plan.grad_outer(-1.0, lam, y)
+        if ybar.dtype == torch.float32:
+            plan.solve_single("T", ybar.contiguous(), lam)
+            plan.last_factor_grad = plan.grad_outer_f32(-1.0, lam, y)
+        else:
+            plan.solve("T", ybar.contiguous(), lam)
+            plan.last_factor_grad = plan.grad_outer(-1.0, lam, y)
         return lam, None
 
 
@@ -35,7 +45,10 @@ class PlrApplyFn(torch.autograd.Function):
     @staticmethod
     def forward(ctx, x, plan):
         y = torch.zeros_like(x)
-        plan.gepdmm("N", 1.0, x.contiguous(), 0.0, y)
+        if x.dtype == torch.float32:
+            plan.gepdmm_single("N", 1.0, x.contiguous(), 0.0, y)
+        else:
+            plan.gepdmm("N", 1.0, x.contiguous(), 0.0, y)
         ctx.plan = plan
         ctx.save_for_backward(x)
         return y
@@ -45,8 +58,12 @@ class PlrApplyFn(torch.autograd.Function):
         (x,) = ctx.saved_tensors
         plan = ctx.plan
         xbar = torch.zeros_like(ybar)
-        plan.gepdmm("T", 1.0, ybar.contiguous(), 0.0, xbar)
-        plan.last_factor_grad = plan.grad_outer(1.0, ybar.contiguous(), x)
+        if ybar.dtype == torch.float32:
+            plan.gepdmm_single("T", 1.0, ybar.contiguous(), 0.0, xbar)
+            plan.last_factor_grad = plan.grad_outer_f32(1.0, ybar.contiguous(), x)
+        else:
+            plan.gepdmm("T", 1.0, ybar.contiguous(), 0.0, xbar)
+            plan.last_factor_grad = plan.grad_outer(1.0, ybar.contiguous(), x)
         return xbar, None
 
 

@@ -82,6 +82,27 @@ int dplrh_batch_set_tile(dplrh_batch *B, int tmax);
 int dplrh_batch_tune(dplrh_batch *B, int nrhs, const double *const *X, const int *ldx,
                      double *const *Y, const int *ldy, int reps, int *tmax_out, double *ms,
                      void *stream);
+/* Any precision (0 FP64, 1 FP32, 2 BF16 factors with FP32 arithmetic; FP32
+ * vectors for 1 and 2) and direction; op 0 apply, op 1 solve. */
+int dplrh_batch_op(dplrh_batch *B, int prec, int op, int trans, int nrhs, const void *const *X,
+                   const int *ldx, void *const *Y, const int *ldy, double alpha, double beta,
+                   void *stream);
+/* BF16 solves refined with FP32 residuals (iters solves). */
+int dplrh_batch_refined(dplrh_batch *B, int nrhs, const float *const *X, const int *ldx,
+                        float *const *Y, const int *ldy, int iters, void *stream);
+int dplrh_batch_tune_prec(dplrh_batch *B, int prec, int nrhs, const void *const *X,
+                          const int *ldx, void *const *Y, const int *ldy, int reps,
+                          int *tmax_out, double *ms, void *stream);
+/* Single-precision factor copies (plr_sp.hip): type 1 FP32, 2 BF16; with
+ * factor, the FP64 factorization runs first when missing. */
+int dplrh_sp_factor(dplrh_plan *p, int type, int factor, int *info, void *stream);
+int dplrh_sp_gemm(dplrh_plan *p, int prec, int trans, int n, float alpha, float beta,
+                  const float *X, int ldx, float *Y, int ldy, void *stream);
+int dplrh_sp_solve(dplrh_plan *p, int prec, int trans, int n, const float *X, int ldx, float *Y,
+                   int ldy, void *stream);
+int dplrh_grad_outer_f32(dplrh_plan *p, int n, float alpha, const float *A, int lda,
+                         const float *B, int ldb, float *dU, float *dV, float *dS, float *dD,
+                         void *stream);
 int dplrh_gemm(dplrh_plan *p, int trans, int n, double alpha, double beta,
                const double *dB, int ldb, double *dC, int ldc, void *stream);
 

@@ -50,6 +50,17 @@ struct plr_fact
     size_t goff[PLR_MAXL][PLR_MAXL];
 };
 
+// Single-precision copies of the factors (plr_sp.hip): type 1 FP32, type 2
+// BF16 factor matrices (uint16_t bits); sig and inv(K) FP32 in both.  Ut,
+// Dinv and Kinv exist once fact is set.
+struct plr_sp
+{
+    int type, fact;
+    unsigned gen;
+    void *U, *V, *Ut, *D, *Dinv;
+    float *S, *Kinv;
+};
+
 // Workspace of the per-level solve kernels, sized for `cols` columns.
 struct plr_solve_ws
 {
@@ -80,6 +91,8 @@ struct dplrh_plan
     size_t tmp_cap;
     plr_fact f;
     plr_solve_ws sw;
+    plr_sp *sp[2];  // single-precision factors, by type - 1
+    struct dplrh_batch *self_batch; // batch of this plan alone (single-precision ops)
     int force_valu; // testing aid: use the general-shape kernels everywhere
     // HIP graphs of repeated calls (see plr_run_cached)
     unsigned ws_gen;  // bumped whenever plan-owned device memory is reallocated
@@ -164,6 +177,8 @@ int plr_solve_reserve(dplrh_plan *p, int cols);
 
 // Mixed-precision factors of a plan (plr_lp.hip).
 void plr_lp_release(const dplrh_plan *p);
+// Single-precision factors of a plan (plr_sp.hip).
+void plr_sp_release(dplrh_plan *p);
 
 // FP64 MFMA kernels (plr_mm.hip); usable when plr_mm_ok(p).
 int plr_mm_ok(const dplrh_plan *p);

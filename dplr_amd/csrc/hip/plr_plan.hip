@@ -294,6 +294,8 @@ extern "C" void dplrh_plan_destroy(dplrh_plan *p)
     plr_free_fact(p);
     plr_lp_release(p);
     plr_up_release(p);
+    plr_sp_release(p);
+    if (p->self_batch) dplrh_batch_destroy(p->self_batch);
     delete p;
 }
 

@@ -237,6 +237,25 @@ int dplrx_solve_refined_tol(dplrx_plan *plan, int n, const double *dB, int ldb, 
     return dplrh_lp_solve_tol(plan, n, dB, ldb, dX, ldx, tol, maxit, iters, err, stream);
 }
 
+int dplrx_factor_single(dplrx_plan *plan, int type, int *info, void *stream)
+{
+    return dplrh_sp_factor(plan, type, 1, info, stream);
+}
+
+int dplrx_gepdmm_single(dplrx_plan *plan, int type, char trans, int n, float alpha,
+                        const float *dB, int ldb, float beta, float *dC, int ldc, void *stream)
+{
+    if (n < 0 || (type != 1 && type != 2)) return -1;
+    return dplrh_sp_gemm(plan, type, !is_n(trans), n, alpha, beta, dB, ldb, dC, ldc, stream);
+}
+
+int dplrx_solve_single(dplrx_plan *plan, int type, char trans, int n, const float *dB, int ldb,
+                       float *dX, int ldx, void *stream)
+{
+    if (n < 0 || (type != 1 && type != 2) || (n > 0 && dB == dX)) return -1;
+    return dplrh_sp_solve(plan, type, !is_n(trans), n, dB, ldb, dX, ldx, stream);
+}
+
 const char *dplrx_last_error(void) { return dplrh_last_error(); }
 
 /* ---- batches ------------------------------------------------------------ */
@@ -269,6 +288,28 @@ int dplrx_batch_tune(dplrx_batch *batch, int nrhs, const double *const *dX, cons
 
 int dplrx_batch_set_tile(dplrx_batch *batch, int tile) { return dplrh_batch_set_tile(batch, tile); }
 
+int dplrx_batch_op(dplrx_batch *batch, int prec, int op, char trans, int nrhs,
+                   const void *const *dX, const int *ldx, void *const *dY, const int *ldy,
+                   double alpha, double beta, void *stream)
+{
+    return dplrh_batch_op(batch, prec, op, !is_n(trans), nrhs, dX, ldx, dY, ldy, alpha, beta,
+                          stream);
+}
+
+int dplrx_batch_solve_refined(dplrx_batch *batch, int nrhs, const float *const *dB,
+                              const int *ldb, float *const *dX, const int *ldx, int iters,
+                              void *stream)
+{
+    return dplrh_batch_refined(batch, nrhs, dB, ldb, dX, ldx, iters, stream);
+}
+
+int dplrx_batch_tune_prec(dplrx_batch *batch, int prec, int nrhs, const void *const *dX,
+                          const int *ldx, void *const *dY, const int *ldy, int reps, int *tile,
+                          double *ms, void *stream)
+{
+    return dplrh_batch_tune_prec(batch, prec, nrhs, dX, ldx, dY, ldy, reps, tile, ms, stream);
+}
+
 /* ---- adjoints ----------------------------------------------------------- */
 
 int dplrx_grad_outer(dplrx_plan *plan, int n, double alpha, const double *dGa, int lda,
@@ -276,4 +317,11 @@ int dplrx_grad_outer(dplrx_plan *plan, int n, double alpha, const double *dGa, i
                      void *stream)
 {
     return dplrh_grad_outer(plan, n, alpha, dGa, lda, dGb, ldb, dU, dV, dS, dD, stream);
+}
+
+int dplrx_grad_outer_f32(dplrx_plan *plan, int n, float alpha, const float *dGa, int lda,
+                         const float *dGb, int ldb, float *dU, float *dV, float *dS, float *dD,
+                         void *stream)
+{
+    return dplrh_grad_outer_f32(plan, n, alpha, dGa, lda, dGb, ldb, dU, dV, dS, dD, stream);
 }

@@ -111,6 +111,51 @@ class Plan:
               "dplrx_grad_outer")
         return dU, dV, dS[:so[-1]], dD
 
+    def grad_outer_f32(self, alpha, Ga, Gb, stream=None):
+        """grad_outer in FP32 on the plan's FP32 factors (factor_single("f32")):
+        Ga, Gb float32 device tensors; returns float32 (dU, dV, dS, dD)."""
+        import torch
+        col, so = self.layout()
+        dev = Ga.device
+        f = dict(dtype=torch.float32, device=dev)
+        dU = torch.zeros(col[-1], self.m, **f)
+        dV = torch.zeros_like(dU)
+        dS = torch.zeros(max(so[-1], 1), **f)
+        dD = torch.zeros(self.m // self.leaf, self.leaf, self.leaf, **f)
+        n = Ga.shape[0]
+        check(lib.dplrx_grad_outer_f32(self.p, n, alpha, _dptr(Ga), _ld(Ga), _dptr(Gb), _ld(Gb),
+                                       _dptr(dU), _dptr(dV), _dptr(dS), _dptr(dD),
+                                       _stream(stream)), "dplrx_grad_outer_f32")
+        return dU, dV, dS[:so[-1]], dD
+
+    # ---- single precision ------------------------------------------------------
+
+    SP_TYPES = {"f32": 1, "bf16": 2}
+
+    def factor_single(self, dtype="f32", stream=None):
+        """FP32 (or BF16 factor matrices) copies of the factors, after the FP64
+        factorization (run first when missing); returns info."""
+        info = c_int(0)
+        check(lib.dplrx_factor_single(self.p, self.SP_TYPES[dtype], byref(info), _stream(stream)),
+              "dplrx_factor_single")
+        return info.value
+
+    def gepdmm_single(self, trans, alpha, B, beta, C, dtype="f32", stream=None):
+        """C := alpha*op(A)*B + beta*C in single precision; B, C float32 (n, m)."""
+        n, m = B.shape
+        check(lib.dplrx_gepdmm_single(self.p, self.SP_TYPES[dtype], trans.encode(), n, alpha,
+                                      _dptr(B), _ld(B), beta, _dptr(C), _ld(C), _stream(stream)),
+              "dplrx_gepdmm_single")
+        return C
+
+    def solve_single(self, trans, B, X, dtype="f32", stream=None):
+        """X := op(A)^{-1} B in single precision; B, X float32 (n, m)."""
+        n, m = B.shape
+        check(lib.dplrx_solve_single(self.p, self.SP_TYPES[dtype], trans.encode(), n, _dptr(B),
+                                     _ld(B), _dptr(X), _ld(X), _stream(stream)),
+              "dplrx_solve_single")
+        return X
+
     LP_TYPES = {"f16": 1, "bf16": 2, "f16x2": 3}
 
     def factor_mixed(self, dtype="f16", stream=None):
@@ -175,20 +220,40 @@ class Batch:
         ly = (c_int * n)(*[_ld(y) for y in Y])
         return xs, lx, ys, ly
 
-    def gepdmm(self, X, Y, stream=None):
-        """Y[i] := A_i X[i]; X[i], Y[i] are (nrhs, N_i) device tensors."""
+    PREC = {"f64": 0, "f32": 1, "bf16": 2}
+
+    def gepdmm(self, X, Y, prec="f64", trans="N", alpha=1.0, beta=0.0, stream=None):
+        """Y[i] := alpha op(A_i) X[i] + beta Y[i]; X[i], Y[i] are (nrhs, N_i)
+        device tensors (float64 for "f64", float32 for "f32" and "bf16")."""
         nrhs = X[0].shape[0]
         xs, lx, ys, ly = self._io(X, Y)
-        check(lib.dplrx_batch_gepdmm(self.b, nrhs, xs, lx, ys, ly, _stream(stream)),
-              "dplrx_batch_gepdmm")
+        if prec == "f64" and trans == "N" and alpha == 1.0 and beta == 0.0:
+            check(lib.dplrx_batch_gepdmm(self.b, nrhs, xs, lx, ys, ly, _stream(stream)),
+                  "dplrx_batch_gepdmm")
+        else:
+            check(lib.dplrx_batch_op(self.b, self.PREC[prec], 0, trans.encode(), nrhs, xs, lx, ys,
+                                     ly, alpha, beta, _stream(stream)), "dplrx_batch_op")
         return Y
 
-    def solve(self, B, X, stream=None):
-        """X[i] := A_i^{-1} B[i] (every plan factored)."""
+    def solve(self, B, X, prec="f64", trans="N", stream=None):
+        """X[i] := op(A_i)^{-1} B[i] (every plan factored in that precision)."""
         nrhs = B[0].shape[0]
         bs, lb, xs, lx = self._io(B, X)
-        check(lib.dplrx_batch_solve(self.b, nrhs, bs, lb, xs, lx, _stream(stream)),
-              "dplrx_batch_solve")
+        if prec == "f64" and trans == "N":
+            check(lib.dplrx_batch_solve(self.b, nrhs, bs, lb, xs, lx, _stream(stream)),
+                  "dplrx_batch_solve")
+        else:
+            check(lib.dplrx_batch_op(self.b, self.PREC[prec], 1, trans.encode(), nrhs, bs, lb, xs,
+                                     lx, 0.0, 0.0, _stream(stream)), "dplrx_batch_op")
+        return X
+
+    def solve_refined(self, B, X, iters, stream=None):
+        """X[i] := A_i^{-1} B[i] by `iters` BF16 solves refined with FP32
+        residuals (plans prepared with factor_single "f32" and "bf16")."""
+        nrhs = B[0].shape[0]
+        bs, lb, xs, lx = self._io(B, X)
+        check(lib.dplrx_batch_solve_refined(self.b, nrhs, bs, lb, xs, lx, iters, _stream(stream)),
+              "dplrx_batch_solve_refined")
         return X
 
     TILES = (128, 256, 512, 1024)
@@ -197,16 +262,17 @@ class Batch:
         """Rows per reduction task (one of TILES), without timing."""
         check(lib.dplrx_batch_set_tile(self.b, tile), "dplrx_batch_set_tile")
 
-    def tune(self, X, Y, reps=3, stream=None):
-        """Autotuner: times apply + solve of this batch (X read, Y written) for
-        each candidate tile shape (rows per reduction task) and keeps the
-        fastest.  Returns (tile, {tile: ms per repetition})."""
+    def tune(self, X, Y, reps=3, prec="f64", stream=None):
+        """Autotuner: times apply + solve of this batch (X read, Y written) in
+        precision prec for each candidate tile shape (rows per reduction task)
+        and keeps the fastest.  Returns (tile, {tile: ms per repetition})."""
         import numpy as np
         nrhs = X[0].shape[0]
         xs, lx, ys, ly = self._io(X, Y)
         tile = c_int(0)
         ms = np.zeros(len(self.TILES))
-        check(lib.dplrx_batch_tune(self.b, nrhs, xs, lx, ys, ly, reps, byref(tile),
-                                   ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
-                                   _stream(stream)), "dplrx_batch_tune")
+        check(lib.dplrx_batch_tune_prec(self.b, self.PREC[prec], nrhs, xs, lx, ys, ly, reps,
+                                        byref(tile),
+                                        ms.ctypes.data_as(ctypes.POINTER(ctypes.c_double)),
+                                        _stream(stream)), "dplrx_batch_tune_prec")
         return tile.value, dict(zip(self.TILES, ms.tolist()))

@@ -108,6 +108,25 @@ int dplrx_solve_refined_tol(dplrx_plan *plan, int n, const double *dB, int ldb, 
                             int ldx, double tol, int maxit, int *iters, double *err,
                             void *stream);
 
+/* ---- single precision --------------------------------------------------- */
+
+/* Single-precision copies of the factors for FP32 products and solves:
+ * type 1 stores U, V, the leaves and (once factored) the Dinv-applied U and
+ * the leaf inverses in FP32, type 2 the same matrices in BF16; sig and the
+ * capacitance inverses are FP32 in both.  The pivoted leaf LU and every
+ * capacitance LU run in FP64 (dplrx_factor, run first when the plan has no
+ * factorization); the products and solves then run in FP32 on FP32 MFMA
+ * with FP32 vectors.  *info as dplrx_factor. */
+int dplrx_factor_single(dplrx_plan *plan, int type, int *info, void *stream);
+/* C := alpha*op(A)*B + beta*C with FP32 device blocks, after
+ * dplrx_factor_single of `type`.  ldb, ldc >= m and multiples of 4, the
+ * blocks 16-byte aligned. */
+int dplrx_gepdmm_single(dplrx_plan *plan, int type, char trans, int n, float alpha,
+                        const float *dB, int ldb, float beta, float *dC, int ldc, void *stream);
+/* X := op(A)^{-1} B with FP32 device blocks as above (X must not alias B). */
+int dplrx_solve_single(dplrx_plan *plan, int type, char trans, int n, const float *dB, int ldb,
+                       float *dX, int ldx, void *stream);
+
 /* ---- adjoints ----------------------------------------------------------- */
 
 /* Gradient of <G, A> with respect to A's factors for G = alpha * Ga Gb^T
@@ -120,6 +139,12 @@ int dplrx_solve_refined_tol(dplrx_plan *plan, int n, const double *dB, int ldb, 
 int dplrx_grad_outer(dplrx_plan *plan, int n, double alpha, const double *dGa, int lda,
                      const double *dGb, int ldb, double *dU, double *dV, double *dS, double *dD,
                      void *stream);
+
+/* dplrx_grad_outer in FP32 on the plan's FP32 factors (dplrx_factor_single
+ * type 1), FP32 blocks and outputs. */
+int dplrx_grad_outer_f32(dplrx_plan *plan, int n, float alpha, const float *dGa, int lda,
+                         const float *dGb, int ldb, float *dU, float *dV, float *dS, float *dD,
+                         void *stream);
 
 /* ---- batches of independent matrices ----------------------------------- */
 
@@ -146,8 +171,28 @@ int dplrx_batch_solve(dplrx_batch *batch, int nrhs, const double *const *dB, con
 int dplrx_batch_tune(dplrx_batch *batch, int nrhs, const double *const *dX, const int *ldx,
                      double *const *dY, const int *ldy, int reps, int *tile, double *ms,
                      void *stream);
-/* Sets the tile shape (one of the tuner's candidates) without timing. */
+/* Sets the tile shape (one of the tuner's candidates) of every precision
+ * without timing. */
 int dplrx_batch_set_tile(dplrx_batch *batch, int tile);
+/* Any precision and direction.  prec 0: FP64 (double vectors, plans
+ * factored with dplrx_factor for solves); 1: FP32, 2: BF16 factor matrices
+ * with FP32 arithmetic (float vectors, leading dimensions multiples of 4,
+ * blocks 16-byte aligned, plans prepared with dplrx_factor_single of type
+ * prec).  op 0: Y[i] := alpha*op(A_i)*X[i] + beta*Y[i]; op 1:
+ * Y[i] := op(A_i)^{-1} X[i].  trans 'N' or 'T'. */
+int dplrx_batch_op(dplrx_batch *batch, int prec, int op, char trans, int nrhs,
+                   const void *const *dX, const int *ldx, void *const *dY, const int *ldy,
+                   double alpha, double beta, void *stream);
+/* X[i] := A_i^{-1} B[i] by `iters` solves with the BF16 factors refined with
+ * FP32 residuals B - A X (FP32 factors): every plan prepared with
+ * dplrx_factor_single of types 1 and 2.  FP32 blocks as dplrx_batch_op. */
+int dplrx_batch_solve_refined(dplrx_batch *batch, int nrhs, const float *const *dB,
+                              const int *ldb, float *const *dX, const int *ldx, int iters,
+                              void *stream);
+/* dplrx_batch_tune for precision prec (vectors as dplrx_batch_op). */
+int dplrx_batch_tune_prec(dplrx_batch *batch, int prec, int nrhs, const void *const *dX,
+                          const int *ldx, void *const *dY, const int *ldy, int reps, int *tile,
+                          double *ms, void *stream);
 
 const char *dplrx_last_error(void);
 
