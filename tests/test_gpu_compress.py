@@ -91,3 +91,57 @@ def test_kernel_matrix_workload(cuda, orc):
     P.close()
     B.free()
     T.free()
+
+
+def test_truncated_svd_exact_at_full_sample(cuda):
+    """eps = 0 keeps every singular value, so every block is sampled up to
+    its full order and decomposed exactly: the singular values against
+    LAPACK's (numpy) and the triplets reproduce each block."""
+    m, nlvl = 256, 2
+    A = np.asfortranarray(np.random.default_rng(4).standard_normal((m, m)))
+    T, info = compress(A, 0.0, nlvl)
+    assert info == 0
+    for (d, n, w, nd) in T.blocks():
+        u, s, v = T.lr_arrays(nd)
+        seg = m >> (d + 1)
+        q = 2 * n if w == "ne" else 2 * n + 1
+        blk = A[q * seg:(q + 1) * seg, (q ^ 1) * seg:((q ^ 1) + 1) * seg]
+        ref = np.linalg.svd(blk, compute_uv=False)
+        assert len(s) == seg
+        assert np.allclose(s, ref, rtol=1e-12, atol=1e-13 * ref[0])
+        assert np.abs((u * s) @ v.T - blk).max() < 1e-12 * ref[0]
+    T.free()
+
+
+def test_singular_vectors_orthonormal(cuda):
+    m = 2048
+    A = gaussian(m, 0.1, 1e-2)
+    T, info = compress(A, 1e-10, 5)
+    assert info == 0
+    for (_, _, _, nd) in T.blocks():
+        u, s, v = T.lr_arrays(nd)
+        k = len(s)
+        assert np.all(np.diff(s) <= 0)
+        assert np.abs(u.T @ u - np.eye(k)).max() < 1e-12
+        assert np.abs(v.T @ v - np.eye(k)).max() < 1e-12
+    T.free()
+
+
+def test_compression_gradient_workload_size(cuda):
+    """N = 16384 Gaussian kernel (l = 0.05, nugget 1e-2) at tol 1e-10 with
+    leaf 128: the compressed operator against the dense one on random
+    vectors."""
+    import time
+    m = 16384
+    A = gaussian(m, 0.05, 1e-2)
+    print(f"dplrd2gep N={m} starts", flush=True)
+    t0 = time.perf_counter()
+    T, info = compress(A, 1e-10, 7)
+    dt = time.perf_counter() - t0
+    print(f"dplrd2gep N={m}: {dt:.2f} s", flush=True)
+    assert info == 0
+    rng = np.random.default_rng(5)
+    for _ in range(2):
+        v = rng.standard_normal(m)
+        assert rel(D.gepmv("N", T, v), A @ v) < 1e-9
+    T.free()
