@@ -204,6 +204,9 @@ int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W,
 int plr_up_factor(dplrh_plan *p, hipStream_t st);
 void plr_up_release(dplrh_plan *p);
 int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream_t st);
+int plr_up_ready(const dplrh_plan *p, int d0);
+int plr_up_reserve(dplrh_plan *p, int d0, int ncols);
+int plr_up_steps(dplrh_plan *p, int d0, int ncols, hipStream_t st);
 int plr_tail_d0(const dplrh_plan *p);
 int plr_mm_expand1(dplrh_plan *p, int d, const double *Q, const double *W, size_t wstride,
                    const double *D, int trans, const double *X, int ldx, double alpha, double beta,

@@ -1354,6 +1354,11 @@ struct LpTailTab
     // inverse factor scales (1 for single storage)
     size_t vlo, ulo, dlo;
     float isV[PLR_MAXL], isU[PLR_MAXL], isD;
+    // reassociated upper levels: V_d^T z over the span for every level
+    // d < d0, at upart + upoff[d] + (span * Rp[d] + k) * ncols + c
+    float *upart;
+    size_t upoff[PLR_MAXL];
+    int nspan;
 };
 
 struct LpLvlTab
@@ -1765,21 +1770,31 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
         __syncthreads();
     }
 
-    // 4. T of level d0 - 1 over the span (one of its segments); a segment of
-    // more than 256 rows leaves partials combined in a fixed order
-    if (a.d0 > 0) {
-        const int dn = a.d0 - 1, R = a.Rp[dn], nkt = R / 16, sg = row0 / S;
+    // 4. T of level d0 - 1 over the span (one of its segments), or with
+    // upart the span's share of T of level d0 - 1 and of every coarser
+    // level; a span of more than 256 rows leaves partials combined in a
+    // fixed order
+    for (int dn = a.d0 - 1; dn >= (a.upart ? 0 : a.d0 - 1); --dn) {
+        const int R = a.Rp[dn], nkt = R / 16, sg = row0 / S;
         const int nh = S > 256 ? S / 256 : 1; // partials per (kt, ct)
         float *red = T;                        // free now: [kt][ct][h][lane][4]
+        float *up = a.upart ? a.upart + a.upoff[dn] + (size_t)sg * R * ncols : nullptr;
+        // rows k .. k+3 of column c
+        auto put = [&](int c, int k, f4 x) {
+            if (up) {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) up[(size_t)(k + v) * ncols + c0 + c] = x[v];
+            } else {
+                *(f4 *)(Tout + (size_t)(c0 + c) * tstride + (size_t)sg * R + k) = x;
+            }
+        };
         if (R > 0) {
             lp_tail_reduce<T16, NCT, PR>((const T16 *)a.V[dn], N, row0, z, zs, S, S, nkt, wv, m, g,
                                          a.vlo, a.isV[dn] / LP_PAIR_Z,
                                          [&](int s, int kt, int ct, f4 acc) {
                                          if (s >= 0) {
                                              const int c = ct * 16 + m;
-                                             if (c < nc)
-                                                 *(f4 *)(Tout + (size_t)(c0 + c) * tstride +
-                                                         (size_t)sg * R + kt * 16 + 4 * g) = acc;
+                                             if (c < nc) put(c, kt * 16 + 4 * g, acc);
                                          } else {
                                              const int h = -1 - s;
                                              *(f4 *)(red + ((((kt * NCT + ct) * nh + h) * 64) + lane) * 4) = acc;
@@ -1792,9 +1807,9 @@ __global__ __launch_bounds__(LT_THREADS) void k_lp_tail(LpTailTab a, const doubl
                 const int kt = tt / NCT, ct = tt - kt * NCT, c = ct * 16 + m;
                 f4 x = *(const f4 *)(red + ((tt * nh) * 64 + lane) * 4);
                 for (int h = 1; h < nh; ++h) x += *(const f4 *)(red + (((tt * nh + h) * 64) + lane) * 4);
-                if (c < nc)
-                    *(f4 *)(Tout + (size_t)(c0 + c) * tstride + (size_t)sg * R + kt * 16 + 4 * g) = x;
+                if (c < nc) put(c, kt * 16 + 4 * g, x);
             }
+            __syncthreads(); // red is reused by the next level
         }
     }
 
@@ -2082,6 +2097,236 @@ __global__ __launch_bounds__(LV_THREADS) void k_lp_level(LpLvlTab a, const float
     }
 }
 
+// ---------------------------------------------------------------------------
+// Reassociated upper levels (pair storage, from the FP64 factorization):
+//   k_lp_tail   leaves the span's share of V_d^T z for every d < d0,
+//   k_lp_up_t0  sums the shares per segment into FP64 T0 of every upper
+//               level (rows soff[d] + s*R + k, column stride ws),
+//   level steps the FP64 steps of the plain solve (plr_up.hip): W of every
+//               upper level from T0 and the cross-level blocks
+//               G_{d,e}[s] = V_d|_s^T Ut_e|_s,
+//   k_lp_upx    x (+)= max|r| * (z_tail + sum_{d < d0} Ut_d W_d), one pass.
+// z is read twice and x once, instead of z read and written per level.
+
+struct LpUpTab
+{
+    int N, d0, ncols, nspan;
+    int R[PLR_MAXL], Rp[PLR_MAXL];
+    size_t soff[PLR_MAXL + 1]; // offsets of the FP64 layout (true ranks)
+    size_t upoff[PLR_MAXL];
+    size_t eoff[PLR_MAXL + 1]; // first output entry of each level (segments x ranks x columns)
+};
+
+// T0[(soff[d] + s*R + k) * ws + c]: the sum over the spans of segment s of
+// the tail's shares, one thread per entry (columns fastest, so that the
+// shares are read and T0 written along the columns), spans summed in a
+// fixed order.
+__global__ __launch_bounds__(256) void k_lp_up_t0(LpUpTab a, const float *__restrict__ up,
+                                                  double *__restrict__ T0, size_t ws)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= a.eoff[a.d0]) return;
+    int d = 0;
+    while (e >= a.eoff[d + 1]) ++d;
+    const size_t loc = e - a.eoff[d];
+    const int n = a.ncols, c = (int)(loc % n);
+    const size_t sk = loc / n; // s * R + k
+    const int R = a.R[d], Rp = a.Rp[d];
+    const int s = (int)(sk / R), k = (int)(sk - (size_t)s * R);
+    const int per = a.nspan >> (d + 1); // spans per segment
+    const float *p = up + a.upoff[d] + ((size_t)s * per * Rp + k) * n + c;
+    const size_t step = (size_t)Rp * n;
+    float acc[4] = {0.f, 0.f, 0.f, 0.f};
+    int j = 0;
+    for (; j + 4 <= per; j += 4) {
+        float v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = p[(size_t)(j + u) * step];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc[u] += v[u];
+    }
+    for (; j < per; ++j) acc[j & 3] += p[(size_t)j * step];
+    T0[(a.soff[d] + sk) * ws + c] = (double)((acc[0] + acc[1]) + (acc[2] + acc[3]));
+}
+
+struct LpUpxTab
+{
+    int N, d0;
+    int R[PLR_MAXL], Rp[PLR_MAXL], seg[PLR_MAXL];
+    size_t soff[PLR_MAXL];
+    const void *Ut[PLR_MAXL];
+    size_t ulo;
+    float isU[PLR_MAXL];
+};
+
+static constexpr int UX_RB = 256, UX_THREADS = 512, UX_RT = UX_RB / 128, UX_WS = 36;
+
+// LDS of k_lp_upx: W [d0][CW][UX_WS] (floats)
+static size_t lpx_lds(int d0) { return (size_t)d0 * LV_CW * UX_WS * sizeof(float); }
+
+// One work-group per (UX_RB rows, 64 right-hand sides), eight waves of
+// UX_RT 16-row tiles.  Every level's W for the work-group's segment is
+// staged in LDS as FP32 with a power-of-two scale per (level, column), split
+// into 16-bit operands (pairs: hi + lo times LP_PAIR_Z) against Ut's rows;
+// each level's sums are scaled back before they join z.
+template <typename T16, bool PR>
+__global__ __launch_bounds__(UX_THREADS) void k_lp_upx(LpUpxTab a, const float *__restrict__ Z,
+                                                       int ldz, const double *__restrict__ Wup,
+                                                       size_t ws, const double *__restrict__ scale,
+                                                       double *__restrict__ X, int ldx, int first,
+                                                       int ncols, double *__restrict__ nrm)
+{
+    typedef typename Lp<T16>::v8 v8;
+    extern __shared__ float wl[]; // [d][c][k]
+    __shared__ unsigned smax[PLR_MAXL][LV_CW];
+    __shared__ double xnrm[2 * LV_CW];
+    const int tid = threadIdx.x, lane = tid & 63, m = lane & 15, g = lane >> 4, wv = tid >> 6;
+    const int r0 = blockIdx.x * UX_RB, c0 = blockIdx.y * LV_CW;
+    const int ri = r0 + wv * 16 * UX_RT;
+    const int nc = min(LV_CW, ncols - c0), d0 = a.d0;
+    // z_tail as the D operand (rows ri + 16rt + 4g + v of column 16ct + m)
+    f4 zo[UX_RT][4];
+#pragma unroll
+    for (int rt = 0; rt < UX_RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct)
+            zo[rt][ct] = *(const f4 *)(Z + (size_t)(c0 + min(ct * 16 + m, nc - 1)) * ldz + ri + 16 * rt + 4 * g);
+    for (int e = tid; e < PLR_MAXL * LV_CW; e += UX_THREADS) (&smax[0][0])[e] = 0u;
+    if (tid < 2 * LV_CW) xnrm[tid] = 0.0;
+    __syncthreads();
+    // W of every level for this segment, ranks past R zero
+    for (int d = 0; d < d0; ++d) {
+        const int R = a.R[d], s = r0 / a.seg[d];
+        const double *wd = Wup + (a.soff[d] + (size_t)s * R) * ws + c0;
+        for (int e0 = 0; e0 < 32 * LV_CW; e0 += 4 * UX_THREADS) {
+            double v[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * UX_THREADS + tid, k = e / LV_CW, c = e - k * LV_CW;
+                v[u] = wd[(size_t)min(k, R - 1) * ws + min(c, nc - 1)];
+            }
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+                const int e = e0 + u * UX_THREADS + tid, k = e / LV_CW, c = e - k * LV_CW;
+                const float x = (k < R && c < nc) ? (float)v[u] : 0.f;
+                wl[(d * LV_CW + c) * UX_WS + k] = x;
+                atomicMax(&smax[d][c], __float_as_uint(fabsf(x)));
+            }
+        }
+    }
+    __syncthreads();
+    f4 tot[UX_RT][4];
+#pragma unroll
+    for (int rt = 0; rt < UX_RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) tot[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+    // Ut rows of the next level in flight while the current one is used
+    auto uload = [&](int d, v8 (*bq)[2]) {
+        const T16 *U = (const T16 *)a.Ut[d];
+        const int Rp = a.Rp[d];
+#pragma unroll
+        for (int rt = 0; rt < UX_RT; ++rt) {
+            const size_t idx = (size_t)(ri + 16 * rt + m) * Rp + 8 * g;
+            bq[rt][0] = *(const v8 *)(U + idx);
+            if (PR) bq[rt][1] = *(const v8 *)(U + a.ulo + idx);
+        }
+    };
+    v8 bA[UX_RT][2], bB[UX_RT][2];
+    auto level = [&](int d, v8 (*bq)[2]) {
+        f4 acc[UX_RT][4];
+#pragma unroll
+        for (int rt = 0; rt < UX_RT; ++rt)
+#pragma unroll
+            for (int ct = 0; ct < 4; ++ct) acc[rt][ct] = f4{0.f, 0.f, 0.f, 0.f};
+        float sc[4];
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            sc[ct] = lp_pow2(smax[d][ct * 16 + m]);
+            const f8 w8 = lds8(wl + (d * LV_CW + ct * 16 + m) * UX_WS + 8 * g);
+            if (PR) {
+                v8 wh, wlo;
+                split8<T16>(w8, LP_PAIR_Z / sc[ct], wh, wlo);
+#pragma unroll
+                for (int rt = 0; rt < UX_RT; ++rt)
+                    acc[rt][ct] = mfma3<T16>(bq[rt][0], bq[rt][1], wh, wlo, acc[rt][ct]);
+            } else {
+                const v8 bw = cvt8<T16>(w8, 1.f / sc[ct]);
+#pragma unroll
+                for (int rt = 0; rt < UX_RT; ++rt) acc[rt][ct] = Lp<T16>::mfma(bq[rt][0], bw, acc[rt][ct]);
+            }
+        }
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            const float f = PR ? sc[ct] * (a.isU[d] / LP_PAIR_Z) : sc[ct];
+#pragma unroll
+            for (int rt = 0; rt < UX_RT; ++rt) tot[rt][ct] += acc[rt][ct] * f;
+        }
+    };
+    uload(0, bA);
+    for (int d = 0; d < d0; d += 2) {
+        if (d + 1 < d0) uload(d + 1, bB);
+        level(d, bA);
+        if (d + 1 >= d0) break;
+        if (d + 2 < d0) uload(d + 2, bA);
+        level(d + 1, bB);
+    }
+    // x (+)= max|r| * z, with the correction norms
+    const bool xal = ((uintptr_t)X & 15) == 0 && ldx % 2 == 0;
+    double dmx[4] = {0.0, 0.0, 0.0, 0.0}, xmx[4] = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int rt = 0; rt < UX_RT; ++rt)
+#pragma unroll
+        for (int ct = 0; ct < 4; ++ct) {
+            const int c = ct * 16 + m, i = (wv * UX_RT + rt) * 16 + 4 * g;
+            if (c >= nc) continue;
+            f4 zn;
+#pragma unroll
+            for (int v = 0; v < 4; ++v) zn[v] = zo[rt][ct][v] + tot[rt][ct][v];
+            double *xp = X + (size_t)(c0 + c) * ldx + r0 + i;
+            const double sr = scale[c0 + c] > 0.0 ? scale[c0 + c] : 1.0;
+            if (xal) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    double2 o = first ? double2{0.0, 0.0} : *(const double2 *)(xp + 2 * h);
+                    const double e0 = sr * (double)zn[2 * h], e1 = sr * (double)zn[2 * h + 1];
+                    o.x += e0;
+                    o.y += e1;
+                    *(double2 *)(xp + 2 * h) = o;
+                    dmx[ct] += e0 * e0 + e1 * e1;
+                    xmx[ct] += o.x * o.x + o.y * o.y;
+                }
+            } else {
+#pragma unroll
+                for (int v = 0; v < 4; ++v) {
+                    const double dv = sr * (double)zn[v], xn = (first ? 0.0 : xp[v]) + dv;
+                    xp[v] = xn;
+                    dmx[ct] += dv * dv;
+                    xmx[ct] += xn * xn;
+                }
+            }
+        }
+    if (!nrm) return;
+#pragma unroll
+    for (int ct = 0; ct < 4; ++ct) {
+#pragma unroll
+        for (int o = 16; o < 64; o <<= 1) {
+            dmx[ct] += __shfl_xor(dmx[ct], o, 64);
+            xmx[ct] += __shfl_xor(xmx[ct], o, 64);
+        }
+        if (g == 0) {
+            atomicAdd(&xnrm[ct * 16 + m], dmx[ct]);
+            atomicAdd(&xnrm[LV_CW + ct * 16 + m], xmx[ct]);
+        }
+    }
+    __syncthreads();
+    if (tid < 2 * LV_CW) {
+        const int which = tid / LV_CW, c = tid - which * LV_CW;
+        if (c < nc)
+            atomicAdd(nrm + ((size_t)(blockIdx.x % LP_NRM_COPIES) * ncols + c0 + c) * 2 + which,
+                      xnrm[tid]);
+    }
+}
+
 // Host side of the second form.
 struct lp_v2
 {
@@ -2161,8 +2406,20 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
 {
     const int N = p->N, L = p->L;
     const T16 *Vh = (const T16 *)q->Vh, *Uh = (const T16 *)q->Uh;
-    int rc = lp_part(p, q, lp_v2_part(p, q, v, n));
+    dplrh_plan *pm_ = const_cast<dplrh_plan *>(p);
+    // the reassociated upper levels: pair storage (taken from the FP64
+    // factorization, whose level steps and cross-level blocks they use)
+    const bool up = PR && plr_up_ready(p, v.d0);
+    const int nspan = N / v.S;
+    size_t upoff[PLR_MAXL] = {0}, upn = 0;
+    if (up)
+        for (int d = 0; d < v.d0; ++d) {
+            upoff[d] = upn;
+            upn += (size_t)n * nspan * q->Rp[d];
+        }
+    int rc = lp_part(p, q, up ? (upn + LP_BLK - 1) / LP_BLK : lp_v2_part(p, q, v, n));
     if (rc) return rc;
+    if (up && (rc = plr_up_reserve(pm_, v.d0, n))) return rc;
     LpTailTab tt;
     memset(&tt, 0, sizeof(tt));
     tt.N = N;
@@ -2188,6 +2445,11 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
         tt.isU[d] = q->isU[d];
     }
     tt.isD = q->isD;
+    if (up) {
+        tt.upart = q->part;
+        tt.nspan = nspan;
+        memcpy(tt.upoff, upoff, sizeof(upoff));
+    }
     const size_t lds = lpt_lds(v.S, v.cw, v.RT, v.tsegs);
     const dim3 grid(N / v.S, (n + v.cw - 1) / v.cw);
     rc = v.cw == 64 ? lp_launch_tail<T16, 64, PR>(tt, lds, grid, q, n, r, ldr, pm, pm_n, nrm, st)
@@ -2196,6 +2458,52 @@ static int lp_solve_v2(const dplrh_plan *p, plr_lp *q, const lp_v2 &v, int n, in
     if (v.d0 == 0) {
         hipLaunchKernelGGL(k_lp_update, dim3((N + 255) / 256, n), dim3(256), 0, st, q->z32, N,
                            q->scale, X, ldx, n, first, nrm);
+        PLR_HIP(hipGetLastError());
+        return 0;
+    }
+    if (up) {
+        LpUpTab ut;
+        memset(&ut, 0, sizeof(ut));
+        ut.N = N;
+        ut.d0 = v.d0;
+        ut.ncols = n;
+        ut.nspan = nspan;
+        size_t e = 0;
+        for (int d = 0; d < v.d0; ++d) {
+            ut.R[d] = p->R[d];
+            ut.Rp[d] = q->Rp[d];
+            ut.soff[d] = p->soff[d];
+            ut.upoff[d] = upoff[d];
+            ut.eoff[d] = e;
+            e += ((size_t)2 << d) * p->R[d] * n;
+        }
+        ut.eoff[v.d0] = e;
+        hipLaunchKernelGGL(k_lp_up_t0, dim3((unsigned)((e + 255) / 256)), dim3(256), 0, st, ut,
+                           q->part, p->sw.uT0, (size_t)p->sw.ucols);
+        PLR_HIP(hipGetLastError());
+        if ((rc = plr_up_steps(pm_, v.d0, n, st))) return rc < 0 ? rc : -10;
+        LpUpxTab xt;
+        memset(&xt, 0, sizeof(xt));
+        xt.N = N;
+        xt.d0 = v.d0;
+        for (int d = 0; d < v.d0; ++d) {
+            xt.R[d] = p->R[d];
+            xt.Rp[d] = q->Rp[d];
+            xt.seg[d] = N >> (d + 1);
+            xt.soff[d] = p->soff[d];
+            xt.Ut[d] = Uh + q->off[d];
+            xt.isU[d] = q->isU[d];
+        }
+        xt.ulo = q->off[L];
+        static bool xattr = false;
+        if (!xattr) {
+            PLR_HIP(hipFuncSetAttribute((const void *)k_lp_upx<T16, PR>,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 96 * 1024));
+            xattr = true;
+        }
+        hipLaunchKernelGGL((k_lp_upx<T16, PR>), dim3(N / UX_RB, (n + LV_CW - 1) / LV_CW),
+                           dim3(UX_THREADS), lpx_lds(v.d0), st, xt, q->z32, N, p->sw.uW,
+                           (size_t)p->sw.ucols, q->scale, X, ldx, first, n, nrm);
         PLR_HIP(hipGetLastError());
         return 0;
     }

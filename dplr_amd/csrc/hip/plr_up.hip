@@ -293,18 +293,13 @@ static int up_ws(dplrh_plan *p, int d0, int ncols)
     return 0;
 }
 
-// Levels d0-1 .. 0 of the solve over X (ldx), where the tail kernel left
-// z_tail in X and T0 of level d0-1 in p->sw.T.  Returns 1, launching
-// nothing, when the plan or X does not suit this path.
-int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream_t st)
+// The level steps e = d0-1 .. 0 (W of every level into p->sw.uW), with T0
+// of level d0-1 in p->sw.T (column-major, t_per_col) when from_tail, else
+// in p->sw.uT0 like the coarser levels.
+static void up_steps(dplrh_plan *p, int d0, int ncols, int from_tail, hipStream_t st)
 {
-    if (!p->f.up_d0 || p->f.up_d0 != d0 || ldx % 2 != 0 || ((uintptr_t)X & 15) != 0) return 1;
-    int rc = up_ws(p, d0, ncols);
-    if (rc) return rc;
     plr_solve_ws &w = p->sw;
     const size_t ws = w.ucols;
-    // T0 of levels 0 .. d0-2
-    if ((rc = plr_mm_reduce_raw(p, d0 - 1, X, ldx, ncols, w.uT0, ws, st))) return rc;
     for (int e = d0 - 1; e >= 0; --e) {
         UpTab t;
         memset(&t, 0, sizeof(t));
@@ -312,7 +307,7 @@ int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream
         t.L = p->L;
         t.ncols = ncols;
         for (int d = 0; d < p->L; ++d) t.R[d] = p->R[d];
-        if (e == d0 - 1) { // from the tail kernel: T[c * tstride + s*R + k]
+        if (e == d0 - 1 && from_tail) { // from the tail kernel: T[c * tstride + s*R + k]
             t.T = w.T;
             t.trs = 1;
             t.tcs = w.t_per_col;
@@ -320,6 +315,8 @@ int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream
             t.T = w.uT0 + p->soff[e] * ws;
             t.trs = ws;
             t.tcs = 1;
+        }
+        if (e < d0 - 1) {
             t.Ain = w.uA[(e + 1) & 1];
             up_arows(p, e + 1, t.ain_off);
         }
@@ -333,7 +330,40 @@ int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream
         for (int d = 0; d < e; ++d) t.G[d] = p->f.G + p->f.goff[d][e];
         hipLaunchKernelGGL(k_up_step, dim3(1u << e, (ncols + UP_CW - 1) / UP_CW), dim3(64 * UP_NW),
                            0, st, t);
-        PLR_HIP(hipGetLastError());
     }
+}
+
+// Levels d0-1 .. 0 of the solve over X (ldx), where the tail kernel left
+// z_tail in X and T0 of level d0-1 in p->sw.T.  Returns 1, launching
+// nothing, when the plan or X does not suit this path.
+int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream_t st)
+{
+    if (!p->f.up_d0 || p->f.up_d0 != d0 || ldx % 2 != 0 || ((uintptr_t)X & 15) != 0) return 1;
+    int rc = up_ws(p, d0, ncols);
+    if (rc) return rc;
+    plr_solve_ws &w = p->sw;
+    const size_t ws = w.ucols;
+    // T0 of levels 0 .. d0-2
+    if ((rc = plr_mm_reduce_raw(p, d0 - 1, X, ldx, ncols, w.uT0, ws, st))) return rc;
+    up_steps(p, d0, ncols, 1, st);
+    PLR_HIP(hipGetLastError());
     return plr_mm_expand_w(p, d0, p->f.Ut, w.uW, ws, X, ldx, ncols, st);
+}
+
+// Whether the factorization prepared the level steps for levels below d0.
+int plr_up_ready(const dplrh_plan *p, int d0) { return p->f.up_d0 >= d0 && d0 >= 2; }
+
+// Workspace of the level steps for ncols columns (T0 and W: rows soff[e] +
+// s*R + k of p->sw.uT0 and p->sw.uW, column stride p->sw.ucols).
+int plr_up_reserve(dplrh_plan *p, int d0, int ncols) { return up_ws(p, d0, ncols); }
+
+// The level steps alone, from T0 of every level 0 .. d0-1 in p->sw.uT0: W
+// of every level in p->sw.uW.  For a solve that makes its own reductions and
+// expansion (the mixed-precision one).
+int plr_up_steps(dplrh_plan *p, int d0, int ncols, hipStream_t st)
+{
+    if (!plr_up_ready(p, d0)) return 1;
+    up_steps(p, d0, ncols, 0, st);
+    PLR_HIP(hipGetLastError());
+    return 0;
 }
