@@ -693,8 +693,11 @@ def main():
                            "traffic": measured_traffic(f"mixed_solve_{args.lp}_N65536_L10_r32_64rhs.md",
                                                        "traffic_bytes_per_solve")
                            if m == 65536 and nrhs == 64 else None,
-                           "kernel": f"one {args.lp} solve (k_lp_colmax + k_lp_tail + 7 x "
-                                     "(k_lp_level + k_lp_finish1))",
+                           "kernel": f"one {args.lp} solve ("
+                                     + ("k_lp_colmax + k_lp_tail + k_lp_up_t0 + 7 x k_up_step + "
+                                        "k_lp_upx)" if args.lp == "f16x2" else
+                                        "k_lp_colmax + k_lp_tail + 7 x (k_lp_level + "
+                                        "k_lp_finish1))"),
                            "bytes_per_launch": lp_bytes, "us_per_launch": ms_one * 1e3},
         "roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": mv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
