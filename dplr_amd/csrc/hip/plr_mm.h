@@ -57,6 +57,7 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
     const int k0 = kg * MM_KG, c0 = cg * MM_CG;
     const int rbeg = s * seg + tile * T;
     const int rend = min(rbeg + T, (s + 1) * seg);
+    const int nct = min(4, (a.ncols - c0 + 15) >> 4); // live 16-column tiles (uniform)
 
     // per-lane column pointers (zero operand for columns past the end)
     const double *pa[2];
@@ -93,9 +94,11 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
 #pragma unroll
                 for (int h = 0; h < 4; ++h) ua[kt][h] = *(const double2 *)(pa[kt] + r + t0 + 2 * h);
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct)
+            for (int ct = 0; ct < 4; ++ct) {
+                if (ct >= nct) break; // uniform
 #pragma unroll
                 for (int h = 0; h < 4; ++h) ub[ct][h] = *(const double2 *)(pb[ct] + r + t0 + 2 * h);
+            }
 #pragma unroll
             for (int tt = 0; tt < 8; ++tt) {
                 double av[2], bv[4];
@@ -106,13 +109,16 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
                 }
 #pragma unroll
                 for (int ct = 0; ct < 4; ++ct) {
+                    if (ct >= nct) break;
                     const double v = (tt & 1) ? ub[ct][tt >> 1].y : ub[ct][tt >> 1].x;
                     bv[ct] = vb[ct] ? v : 0.0;
                 }
 #pragma unroll
-                for (int kt = 0; kt < 2; ++kt)
+                for (int ct = 0; ct < 4; ++ct) {
+                    if (ct >= nct) break;
 #pragma unroll
-                    for (int ct = 0; ct < 4; ++ct) acc[kt][ct] = mfma64(av[kt], bv[ct], acc[kt][ct]);
+                    for (int kt = 0; kt < 2; ++kt) acc[kt][ct] = mfma64(av[kt], bv[ct], acc[kt][ct]);
+                }
             }
         }
     }
@@ -585,11 +591,15 @@ __device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double al
 __device__ __forceinline__ void mm_stage(double *sh, const double *src, size_t cstride, int nc,
                                          int kn, const double *scale = nullptr)
 {
-    // sh[c*65 + kk] = src[c*cstride + kk] for c < nc, kk < kn, else 0
-    for (int e0 = 0; e0 < 64 * 64; e0 += 8 * 256) {
+    // sh[c*65 + kk] = src[c*cstride + kk] for c < nc, kk < kn, 0 for kk >= kn;
+    // rows of the 16-column tiles past nc are left as they are (they only
+    // feed output columns that are never stored)
+    const int ne = ((nc + 15) & ~15) * 64;
+    for (int e0 = 0; e0 < ne; e0 += 8 * 256) {
         double v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
+            if (e0 + u * 256 >= ne) break; // uniform
             const int e = e0 + u * 256 + (int)threadIdx.x, c = e >> 6, kk = e & 63;
             // unconditional load, padding zeroed by a 0/1 factor (a select
             // would let the compiler branch around the load)
@@ -599,6 +609,7 @@ __device__ __forceinline__ void mm_stage(double *sh, const double *src, size_t c
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
+            if (e0 + u * 256 >= ne) break;
             const int e = e0 + u * 256 + (int)threadIdx.x;
             sh[(e >> 6) * 65 + (e & 63)] = v[u];
         }
@@ -617,6 +628,7 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
     const int rb = bx * 64, r0 = rb + wv * 16;
     const int c0 = by * MM_CG;
     const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
+    const int nct = (nc + 15) >> 4; // live 16-column tiles (uniform)
     // A operand (16 columns c, 4 k): lane holds sh[(16ct + m)*65 + k + g]
     // B operand (4 k x 16 rows of Q^T or D^T): lane holds [k + g][r0 + m]
     // Every global load of a chunk (its B operands, and Y at the start) is
@@ -632,7 +644,7 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
-            yo[ct][v] = Y[(size_t)c * ldy + r0 + m];
+            yo[ct][v] = (ct < nct && beta != 0.0) ? Y[(size_t)c * ldy + r0 + m] : 0.0;
         }
 
     if (D) {
@@ -653,10 +665,12 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
             __syncthreads();
             // rows past the chunk read zeros from sh (and real data from D)
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int ct = 0; ct < 4; ++ct) {
+                if (ct >= nct) break; // uniform
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
+                for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma64(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
+            }
         }
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[ct] *= alpha;
@@ -683,10 +697,12 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
             __syncthreads();
             // ranks past the chunk read zeros from sh (and real data from Q)
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int ct = 0; ct < 4; ++ct) {
+                if (ct >= nct) break; // uniform
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
+                for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma64(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
+            }
         }
     }
 #pragma unroll

@@ -80,6 +80,7 @@ __device__ __forceinline__ void sp_reduce_body(const MmTab &a, int task, const f
     const int k0 = kg * MM_KG, c0 = cg * MM_CG;
     const int rbeg = s * seg + tile * Tt;
     const int rend = min(rbeg + Tt, (s + 1) * seg);
+    const int nct = min(4, (a.ncols - c0 + 15) >> 4); // live 16-column tiles (uniform)
     const T *P = (const T *)a.P[d];
     const T *pa[2];
     const float *pb[4];
@@ -109,14 +110,19 @@ __device__ __forceinline__ void sp_reduce_body(const MmTab &a, int task, const f
 #pragma unroll
             for (int kt = 0; kt < 2; ++kt) SpLd<T>::eight(pa[kt] + r + t0, ua[kt]);
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) SpLd<float>::eight(pb[ct] + r + t0, ub[ct]);
+            for (int ct = 0; ct < 4; ++ct) {
+                if (ct >= nct) break;
+                SpLd<float>::eight(pb[ct] + r + t0, ub[ct]);
+            }
 #pragma unroll
             for (int tt = 0; tt < 8; ++tt)
 #pragma unroll
-                for (int kt = 0; kt < 2; ++kt)
+                for (int ct = 0; ct < 4; ++ct) {
+                    if (ct >= nct) break;
 #pragma unroll
-                    for (int ct = 0; ct < 4; ++ct)
+                    for (int kt = 0; kt < 2; ++kt)
                         acc[kt][ct] = mfma32(ua[kt][tt] * va[kt], ub[ct][tt] * vb[ct], acc[kt][ct]);
+                }
         }
     }
     // lane holds T[k0 + 16kt + 4g + v][c0 + 16ct + m]
@@ -184,14 +190,18 @@ __device__ __forceinline__ void sp_finish_body(const MmTab &a, int bx, float alp
     w[(size_t)c * wstride + a.w_off[d] + (size_t)sp * R + k] = raw ? t : alpha * t;
 }
 
-// sh[c*65 + kk] = scale[kk] * src[c*cstride + kk] for c < nc, kk < kn, else 0
+// sh[c*65 + kk] = scale[kk] * src[c*cstride + kk] for c < nc, kk < kn, 0 for
+// kk >= kn; rows of the 16-column tiles past nc are left as they are (they
+// only feed output columns that are never stored)
 __device__ __forceinline__ void sp_stage(float *sh, const float *src, size_t cstride, int nc,
                                          int kn, const float *scale = nullptr)
 {
-    for (int e0 = 0; e0 < 64 * 64; e0 += 8 * 256) {
+    const int ne = ((nc + 15) & ~15) * 64;
+    for (int e0 = 0; e0 < ne; e0 += 8 * 256) {
         float v[8];
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
+            if (e0 + u * 256 >= ne) break; // uniform
             const int e = e0 + u * 256 + (int)threadIdx.x, c = e >> 6, kk = e & 63;
             const float x = src[(size_t)min(c, nc - 1) * cstride + min(kk, kn - 1)];
             const float f = scale ? scale[min(kk, kn - 1)] : 1.f;
@@ -199,6 +209,7 @@ __device__ __forceinline__ void sp_stage(float *sh, const float *src, size_t cst
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
+            if (e0 + u * 256 >= ne) break;
             const int e = e0 + u * 256 + (int)threadIdx.x;
             sh[(e >> 6) * 65 + (e & 63)] = v[u];
         }
@@ -223,6 +234,7 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
     const int rb = bx * 64, r0 = rb + wv * 16;
     const int c0 = by * MM_CG;
     const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
+    const int nct = (nc + 15) >> 4; // live 16-column tiles (uniform)
     sp_f4 acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ++ct) acc[ct] = sp_f4{0.f, 0.f, 0.f, 0.f};
@@ -233,7 +245,7 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = min(c0 + ct * 16 + sp_drow(g, v), a.ncols - 1);
-            yo[ct][v] = beta == 0.f ? 0.f : Y[(size_t)c * ldy + r0 + m];
+            yo[ct][v] = (ct < nct && beta != 0.f) ? Y[(size_t)c * ldy + r0 + m] : 0.f;
         }
     if (D) {
         const int leaf = rb / b, il = r0 - leaf * b;
@@ -252,10 +264,12 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
             sp_stage(sh, xl + j0, ldx, nc, jn);
             __syncthreads();
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int ct = 0; ct < 4; ++ct) {
+                if (ct >= nct) break; // uniform
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
+                for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma32(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
+            }
         }
 #pragma unroll
         for (int ct = 0; ct < 4; ++ct) acc[ct] *= alpha;
@@ -280,10 +294,12 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
                      a.sx ? Sd + (size_t)(trans ? (s ^ 1) : s) * R + k0 : nullptr);
             __syncthreads();
 #pragma unroll
-            for (int u = 0; u < 8; ++u)
+            for (int ct = 0; ct < 4; ++ct) {
+                if (ct >= nct) break; // uniform
 #pragma unroll
-                for (int ct = 0; ct < 4; ++ct)
+                for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma32(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
+            }
         }
     }
 #pragma unroll
