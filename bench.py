@@ -186,6 +186,23 @@ def cpu_baseline(args, log):
             "blas": blas}
 
 
+def batch_bytes(plans, nrhs, es, ev):
+    """Algorithmic HBM bytes of one batched apply + solve: per matrix U, V,
+    the leaves and sig for the apply, V, Ut, the leaf inverses, sig and the
+    capacitance inverses for the solve (factor entries of es bytes, sig and
+    inv(K) of ev bytes), and the right-hand sides read and results written
+    once per operation (ev bytes)."""
+    tot = 0
+    for P in plans:
+        m, b, R = P.m, P.leaf, P.ranks
+        uv = m * sum(R) * es
+        sig = sum((2 << d) * r for d, r in enumerate(R)) * ev
+        kinv = sum((1 << d) * (2 * r) ** 2 for d, r in enumerate(R)) * ev
+        io = 2 * m * nrhs * ev
+        tot += (2 * uv + m * b * es + sig + io) + (2 * uv + m * b * es + sig + kinv + io)
+    return tot
+
+
 def batch_specs(count, seed=7):
     """Shapes of the batched workload: N log-uniform in [1024, 16384] snapped
     to leaf * 2^L, leaf from {64, 128, 256}, rank uniform in {8..48} (at most
@@ -263,8 +280,12 @@ def batch_leg(args, torch, D, log):
         lat.append(event_ms(torch, one, 3))
     lat = np.array(lat)
     total = len(plans) * nrhs
+    by64 = batch_bytes(plans, nrhs, 8, 8)
     out = {"matrices": len(plans), "rhs_per_matrix": nrhs,
            "rhs_s": total / (ms * 1e-3), "ms_per_batch": ms,
+           "roofline": {"bound": "hbm", "bytes_per_batch": by64,
+                        "achieved": by64 / (ms * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by64 / (ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
            "apply_ms": ms_apply, "solve_ms": ms_solve,
            "max_rel_residual": err, "dtype": "f64",
            "tile_rows": tile, "tuner_ms": tile_ms,
@@ -275,7 +296,8 @@ def batch_leg(args, torch, D, log):
                     f"rank in [{min(s[2] for s in specs)}, {max(s[2] for s in specs)}], "
                     "leaf in {64, 128, 256}"}
     log(f"batch: {out['rhs_s']:.0f} RHS/s ({ms:.2f} ms per batch of {total} RHS; apply "
-        f"{ms_apply:.2f} ms, solve {ms_solve:.2f} ms); max residual {err:.1e}; one matrix at a "
+        f"{ms_apply:.2f} ms, solve {ms_solve:.2f} ms; {by64 / 1e9:.2f} GB algorithmic, "
+        f"{out['roofline']['frac']:.1%} of HBM); max residual {err:.1e}; one matrix at a "
         f"time p50 {out['per_matrix_ms_p50']:.3f} ms p99 {out['per_matrix_ms_p99']:.3f} ms")
 
     # FP32 (FP32 factors, FP32 MFMA) and BF16 factors refined with FP32
@@ -308,14 +330,19 @@ def batch_leg(args, torch, D, log):
             one32(i)
         lat32.append(event_ms(torch, one32, 3))
     lat32 = np.array(lat32)
+    by32 = batch_bytes(plans, nrhs, 4, 4)
     f32 = {"dtype": "f32", "rhs_s": total / (ms32 * 1e-3), "ms_per_batch": ms32,
+           "roofline": {"bound": "hbm", "bytes_per_batch": by32,
+                        "achieved": by32 / (ms32 * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                        "unit": "GB/s", "frac": by32 / (ms32 * 1e-3) / 1e9 / HBM_PEAK_GBS},
            "apply_ms": ms32_apply, "solve_ms": ms32_solve, "rel_err_vs_fp64_solve": err32,
            "tile_rows": tile32, "tuner_ms": tile32_ms,
            "per_matrix_ms_p50": float(np.percentile(lat32, 50)),
            "per_matrix_ms_p99": float(np.percentile(lat32, 99))}
     log(f"batch FP32: {f32['rhs_s']:.0f} RHS/s ({ms32:.2f} ms; apply {ms32_apply:.2f} ms, solve "
-        f"{ms32_solve:.2f} ms, tile {tile32}); error vs FP64 {err32:.1e}; one at a time p50 "
-        f"{f32['per_matrix_ms_p50']:.3f} ms p99 {f32['per_matrix_ms_p99']:.3f} ms")
+        f"{ms32_solve:.2f} ms, tile {tile32}; {f32['roofline']['frac']:.1%} of HBM); error vs "
+        f"FP64 {err32:.1e}; one at a time p50 {f32['per_matrix_ms_p50']:.3f} ms p99 "
+        f"{f32['per_matrix_ms_p99']:.3f} ms")
     refined = []
     for iters in (1, 2, 3):
         ZR = [torch.zeros_like(x) for x in X32]

@@ -83,7 +83,16 @@ int dplrx_factor_mixed(dplrx_plan *plan, int type, int *info, void *stream);
 
 /* X := A^{-1} B by `iters` mixed-precision solves with FP64 residuals
  * r = B - A X between them (iterative refinement).  res (host, n entries,
- * optional) receives max|B - A X| per column after the last solve. */
+ * optional) receives max|B - A X| per column after the last solve.
+ * Speed: every refinement step costs one FP64 product, and on MI355X the
+ * FP64 dplrx_solve costs about one such product, so to 1e-10 dplrx_solve is
+ * the faster call at every shape measured (N = 65536 and 131072, ranks 32
+ * and 64, 64 columns: f16 pairs 0.51x, f16 0.33-0.41x, bf16 0.24-0.30x of
+ * its throughput); one 16-bit solve (iters = 1, ~1e-4) is the fast one.
+ * Fast paths: B 16-byte aligned with an even ldb is read in place, and the
+ * residuals come out of the fused FP64 product when X is 16-byte aligned
+ * with an even ldx too; other layouts take a copy of B and separate passes
+ * (same results). */
 int dplrx_solve_refined(dplrx_plan *plan, int n, const double *dB, int ldb, double *dX,
                         int ldx, int iters, double *res, void *stream);
 
