@@ -307,6 +307,88 @@ __global__ __launch_bounds__(256) void k_gj_inverse_w(const double *A, int n, si
     if (lane == 0) info[mat] = sinfo;
 }
 
+// The same elimination for few matrices: one work-group of NM/16 waves per
+// matrix, wave w holding columns 16w .. 16w+15 (lane i: row i).  The wave
+// that owns the step's column finds the pivot (same rule and positions as
+// k_gj_inverse_w) and publishes the column, the pivot's position and its
+// lane through LDS, double-buffered so that one barrier per step suffices;
+// every wave then updates its own columns.  Output and info as
+// k_gj_inverse_w.
+template <int NM>
+__global__ __launch_bounds__(NM * 4) void k_gj_inverse_q(const double *A, int n, size_t stride,
+                                                         double *Inv, size_t istride, int *info)
+{
+    __shared__ double colv[2][64];
+    __shared__ int pk[2][2]; // position of the pivot, its lane
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int mat = blockIdx.x;
+    const double *src = A + (size_t)mat * stride;
+    double a[16];
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int gc = 16 * wv + j;
+        a[j] = (lane < n && gc < n) ? src[(size_t)gc * n + lane] : (gc == lane ? 1.0 : 0.0);
+    }
+    int pos = lane, sinfo = 0;
+    const int nb = min(NM, (n + 15) & ~15) / 16;
+    for (int kb = 0; kb < nb; ++kb) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int k = 16 * kb + c, buf = k & 1;
+            if (wv == kb) {
+                // largest |a[c]| over positions >= k, ties to the smaller
+                // position (NaN ranks below every number)
+                double v = pos >= k ? fabs(a[c]) : -1.0;
+                if (v != v) v = -0.5;
+                int key = pos;
+#pragma unroll
+                for (int o = 1; o < 64; o <<= 1) {
+                    const double v2 = __shfl_xor(v, o, 64);
+                    const int k2 = __shfl_xor(key, o, 64);
+                    if (v2 > v || (v2 == v && k2 < key)) {
+                        v = v2;
+                        key = k2;
+                    }
+                }
+                const int pl = __ffsll((unsigned long long)__ballot(pos == key)) - 1;
+                colv[buf][lane] = a[c];
+                if (lane == 0) {
+                    pk[buf][0] = key;
+                    pk[buf][1] = pl;
+                }
+            }
+            __syncthreads();
+            const int key = pk[buf][0], pl = pk[buf][1];
+            const bool piv = pos == key;
+            if (pos == k)
+                pos = key;
+            else if (piv)
+                pos = k;
+            const double pv = colv[buf][pl];
+            if (pv == 0.0 && sinfo == 0) sinfo = k + 1;
+            const double inv = 1.0 / pv, f = colv[buf][lane];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                if (wv == kb && j == c) continue;
+                const double pj = gj_rdlane(a[j], pl) * inv;
+                a[j] = piv ? pj : a[j] - f * pj;
+            }
+            if (wv == kb) a[c] = piv ? inv : -f * inv;
+        }
+    }
+    // column col of inv(PA) is column perm[col] of inv(A), perm[col] the
+    // lane at position col
+    double *dst = Inv + (size_t)mat * istride;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int col = 16 * wv + j;
+        const int pc = __ffsll((unsigned long long)__ballot(pos == col)) - 1;
+        if (col < n && lane < n) dst[(size_t)pc * n + pos] = a[j];
+    }
+    if (threadIdx.x == 0) info[mat] = sinfo;
+}
+
 // Largest order the LDS Gauss-Jordan kernel takes on this device (0 if the
 // dynamic-LDS limit cannot be raised).
 static int gj_max_order()
@@ -337,6 +419,16 @@ static int batched_inverse(double *A, int n, int count, double *Inv, int *piv, i
                            hipStream_t st)
 {
     if (count <= 0 || n <= 0) return 0;
+    if (n <= 64 && count <= 256) { // few matrices: a work-group per matrix
+        if (n <= 32)
+            hipLaunchKernelGGL(k_gj_inverse_q<32>, dim3(count), dim3(128), 0, st, A, n,
+                               (size_t)n * n, Inv, (size_t)n * n, info);
+        else
+            hipLaunchKernelGGL(k_gj_inverse_q<64>, dim3(count), dim3(256), 0, st, A, n,
+                               (size_t)n * n, Inv, (size_t)n * n, info);
+        PLR_HIP(hipGetLastError());
+        return 0;
+    }
     if (n <= 64) {
         const dim3 grid((count + 3) / 4);
         if (n <= 32)
