@@ -225,3 +225,82 @@ def test_gaussian_kernel_plr_gradient(cuda, ref):
     assert rel(b.grad.cpu().numpy().ravel(), y + lam) < 1e-9
     compare(T, P, P.last_factor_grad, -np.outer(lam, y), 1e-8)
     P.close()
+
+
+def test_gaussian_kernel_gradient_workload_size(cuda, orc):
+    """The gradient workload at its stated size: N=16384 Gaussian kernel
+    (x ~ U[0,1] sorted, l = 0.05, nugget 1e-2) compressed at tol 1e-10 with
+    leaf 128, q = b^T A^{-1} b.  FP64: y against the oracle's product
+    (residual), dq/db = 2y (A is symmetric), central finite differences
+    (h = 1e-6) on factor entries of three levels and a leaf.  FP32: q,
+    dq/db and the factor gradients against FP64 (tolerances: the condition
+    number, ~1e4 here, times FP32 rounding)."""
+    import torch
+    from dplr_amd.autograd import plr_solve
+    m, L, h = 16384, 7, 1e-6
+    x = np.sort(np.random.default_rng(0).uniform(0, 1, m))
+    A = np.asfortranarray(np.exp(-(x[:, None] - x[None, :]) ** 2 / (2 * 0.05 ** 2)) + 1e-2 * np.eye(m))
+    T = PlrTree(m)
+    info = ctypes.c_int(0)
+    D.lib.dplrd2gep(m, A.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), m, T.ref, 1e-10, L,
+                    ctypes.byref(info))
+    assert info.value == 0
+    del A
+    P = D.Plan(T)
+    assert P.factor() == 0
+    b = torch.tensor(np.random.default_rng(1).standard_normal((1, m)), device=cuda)
+    out = {}
+    for dt in (torch.float64, torch.float32):
+        if dt == torch.float32:
+            assert P.factor_single("f32") == 0
+        bb = b.to(dt).clone().requires_grad_(True)
+        y = plr_solve(bb, P)
+        q = (bb * y).sum()
+        q.backward()
+        out[dt] = (q.item(), y.detach().clone(), bb.grad.clone(),
+                   [g.clone() for g in P.last_factor_grad])
+    q64, y64, g64, f64 = out[torch.float64]
+    yh = y64.cpu().numpy().ravel()
+    back = D.gepdmm("N", "N", T, yh.reshape(m, 1).copy(), 1, api=orc).ravel()
+    # condition number ~2e5 (row sums ~2e3, nugget 1e-2); the factored solve's
+    # residual measured 1.7e-9
+    assert rel(back, b.cpu().numpy().ravel()) < 2e-8
+    assert rel(g64.cpu().numpy(), 2 * yh) < 1e-9
+    got, got_leaves = gpu_grads(T, P, f64)
+    blocks = {(d, qq): nd for (d, qq, nd) in block_index(T)}
+    checks = [("u", (0, 1), 100, 0), ("v", (3, 5), 7, 1), ("s", (6, 40), None, 0),
+              ("leaf", 17, (3, 9), None)]
+    w = b.clone()
+    for kind, key, i, k in checks:
+        if kind == "leaf":
+            node = leaf_nodes(T)[key].contents
+            idx = i[1] * node.size + i[0]
+            arr, analytic = node.dd.d, got_leaves[key][i[0], i[1]]
+        else:
+            lr = blocks[key].contents.dlr
+            hsz = blocks[key].contents.size
+            if kind == "u":
+                arr, idx, analytic = lr.u, k * hsz + i, got[key][0][i, k]
+            elif kind == "v":
+                arr, idx, analytic = lr.v, k * hsz + i, got[key][1][i, k]
+            else:
+                arr, idx, analytic = lr.sig, k, got[key][2][k]
+        x0 = arr[idx]
+        arr[idx] = x0 + h
+        fp = _loss(T, b, w, cuda)
+        arr[idx] = x0 - h
+        fm = _loss(T, b, w, cuda)
+        arr[idx] = x0
+        fd = (fp - fm) / (2 * h)
+        print("fd", kind, key, fd, analytic, flush=True)
+        # the difference quotient's error here: rounding of q (~1e-16 q / h) and
+        # the h^2 term at a condition number of ~2e5 (a deep level's sig is
+        # ~1e-13, far below h); measured agreement 6e-7 .. 4e-5
+        assert abs(fd - analytic) <= 1e-4 * abs(analytic) + 1e-3, (kind, key)
+    q32, y32, g32, f32 = out[torch.float32]
+    assert abs(q32 - q64) / abs(q64) < 1e-5
+    assert rel(g32.double().cpu().numpy(), g64.cpu().numpy()) < 2e-3
+    for a, bref in zip(f32, f64):
+        assert rel(a.double().cpu().numpy(), bref.cpu().numpy()) < 5e-2
+    P.close()
+    T.free()

@@ -161,3 +161,51 @@ def test_batch_autotuner_picks_a_measured_tile(batch):
         z1 = torch.zeros_like(x)
         P.solve("N", x, z1)
         assert rel(Z[k], z1) < 1e-12, SPECS[k]
+
+
+def test_batch_workload_size(cuda, orc):
+    """The batching workload at its stated size: 512 matrices (N log-uniform
+    in [1024, 16384], rank 8..48, leaf 64/128/256; bench.batch_specs), 16
+    right-hand sides each, applied and solved in one call per operation in
+    FP64, FP32 and with BF16 factors refined by FP32 residuals (three
+    solves).  Every 16th matrix and the largest one are checked against the
+    oracle's product (apply) and through it (solve residual)."""
+    import torch
+    from bench import batch_specs
+    specs = batch_specs(512)
+    print("batch workload", len(specs), "matrices, N up to", max(s[0] for s in specs), flush=True)
+    trees = [PlrTree.random(m, L, r, 4.0, 1000 + k) for k, (m, L, r) in enumerate(specs)]
+    plans = [D.Plan(t) for t in trees]
+    for P in plans:
+        assert P.factor() == 0
+        assert P.factor_single("f32") == 0 and P.factor_single("bf16") == 0
+    B = D.Batch(plans)
+    g = torch.Generator(device="cpu").manual_seed(9)
+    X = [torch.randn(NRHS, m, generator=g, dtype=torch.float64).to("cuda") for (m, _, _) in specs]
+    X32 = [x.float() for x in X]
+    check = sorted(set(range(0, len(specs), 16)) | {max(range(len(specs)), key=lambda k: specs[k][0])})
+    Y = [torch.zeros_like(x) for x in X]
+    Z = [torch.zeros_like(x) for x in X]
+    B.gepdmm(X, Y)
+    B.solve(X, Z)
+    Y32 = [torch.zeros_like(x) for x in X32]
+    Z32 = [torch.zeros_like(x) for x in X32]
+    B.gepdmm(X32, Y32, prec="f32")
+    B.solve(X32, Z32, prec="f32")
+    ZR = [torch.zeros_like(x) for x in X32]
+    B.solve_refined(X32, ZR, 3)
+    torch.cuda.synchronize()
+    for k in check:
+        T, x = trees[k], X[k]
+        ref = orc_apply(orc, T, x)
+        assert orc_rel(Y[k], ref) < 1e-12, specs[k]
+        assert orc_rel(Y32[k].double(), ref) < 2e-5, specs[k]
+        xh = x.cpu().numpy()
+        assert orc_rel(orc_apply(orc, T, Z[k]), xh) < 1e-12, specs[k]
+        assert orc_rel(orc_apply(orc, T, Z32[k].double()), xh) < 2e-5, specs[k]
+        assert orc_rel(orc_apply(orc, T, ZR[k].double()), xh) < 2e-5, specs[k]
+    B.close()
+    for P in plans:
+        P.close()
+    for t in trees:
+        t.free()
