@@ -129,8 +129,9 @@ __global__ __launch_bounds__(256) void kb_finish(const MmTab *__restrict__ tab, 
 }
 
 // mode 0: apply Y = alpha*op(A) X + beta*Y; mode 1: leaves Y = op(Dinv) X;
-// mode 2: level Z += Q W; mode 3: leaves Y = op(Dinv) Z
-template <typename T>
+// mode 2: level Z += Q W; mode 3: leaves Y = op(Dinv) Z.  NT: column tiles
+// in registers (1 for at most 16 right-hand sides: more waves per SIMD)
+template <typename T, int NT>
 __global__ __launch_bounds__(256) void kb_expand(const MmTab *__restrict__ tab, int tsl, int slot,
                                                  const int *__restrict__ pre, int cnt,
                                                  const BIo *__restrict__ io,
@@ -154,9 +155,9 @@ __global__ __launch_bounds__(256) void kb_expand(const MmTab *__restrict__ tab, 
     const int ldy = mode == 2 ? o.ldz : o.ldy;
     const C al = mode == 0 ? alpha : (C)1, be = mode == 0 ? beta : mode == 2 ? (C)1 : (C)0;
     if constexpr (std::is_same<T, double>::value)
-        mm_expand_body(a, rb, cg, D, X, ldx, al, be, trans, w, wstride, Y, ldy);
+        mm_expand_body<NT>(a, rb, cg, D, X, ldx, al, be, trans, w, wstride, Y, ldy);
     else
-        sp_expand_body<T>(a, rb, cg, D, X, ldx, al, be, trans, w, wstride, Y, ldy);
+        sp_expand_body<T, NT>(a, rb, cg, D, X, ldx, al, be, trans, w, wstride, Y, ldy);
 }
 
 // capacitance step of level d, 16 columns per work-group (FP64 MFMA in every
@@ -499,6 +500,7 @@ static int launch_op(dplrh_batch *B, int prec, int op, int trans, int nrhs, cons
     const BWs<T> *ws = (const BWs<T> *)S.dws;
     auto pre = [&](int ph) { return S.dpre + S.pre_off[ph]; };
     auto grid = [&](int ph) { return dim3((unsigned)S.pre_tot[ph]); };
+    auto kexp = nrhs <= 16 ? kb_expand<T, 1> : kb_expand<T, 4>;
     if (op == 0) {
         const int slot = trans ? 2 : 0;
         if (S.pre_tot[0] > 0)
@@ -507,7 +509,7 @@ static int launch_op(dplrh_batch *B, int prec, int op, int trans, int nrhs, cons
         if (S.pre_tot[1] > 0)
             hipLaunchKernelGGL(kb_finish<T>, grid(1), dim3(256), 0, st, tab, tsl, slot, pre(1), cnt,
                                ws, 0, alpha);
-        hipLaunchKernelGGL(kb_expand<T>, grid(2), dim3(256), 0, st, tab, tsl, slot, pre(2), cnt, dio,
+        hipLaunchKernelGGL(kexp, grid(2), dim3(256), 0, st, tab, tsl, slot, pre(2), cnt, dio,
                            ws, 0, trans, alpha, beta);
         PLR_HIP(hipGetLastError());
         return 0;
@@ -524,18 +526,18 @@ static int launch_op(dplrh_batch *B, int prec, int op, int trans, int nrhs, cons
             hipLaunchKernelGGL(kb_cap<T>, grid(ph + 2), dim3(256), S.cap_lds, st, d, pre(ph + 2),
                                cnt, ws, nrhs, trans);
         if (S.pre_tot[ph + 3] > 0)
-            hipLaunchKernelGGL(kb_expand<T>, grid(ph + 3), dim3(256), 0, st, tab, tsl, slot,
+            hipLaunchKernelGGL(kexp, grid(ph + 3), dim3(256), 0, st, tab, tsl, slot,
                                pre(ph + 3), cnt, dio, ws, 2, 0, (C)1, (C)1);
     };
     if (!trans) {
-        hipLaunchKernelGGL(kb_expand<T>, grid(3), dim3(256), 0, st, tab, tsl, 1, pre(3), cnt, dio,
+        hipLaunchKernelGGL(kexp, grid(3), dim3(256), 0, st, tab, tsl, 1, pre(3), cnt, dio,
                            ws, 1, 0, (C)1, (C)0);
         for (int d = Lm - 1; d >= 0; --d) level(d);
     } else {
         hipLaunchKernelGGL(kb_axpby<C>, grid(4), dim3(256), 0, st, pre(4), cnt, dio, S.dnrow, nrhs,
                            0, 2, (C)1, (C)0);
         for (int d = 0; d < Lm; ++d) level(d);
-        hipLaunchKernelGGL(kb_expand<T>, grid(3), dim3(256), 0, st, tab, tsl, 1, pre(3), cnt, dio,
+        hipLaunchKernelGGL(kexp, grid(3), dim3(256), 0, st, tab, tsl, 1, pre(3), cnt, dio,
                            ws, 3, 1, (C)1, (C)0);
     }
     PLR_HIP(hipGetLastError());

@@ -95,7 +95,7 @@ __device__ __forceinline__ void mm_reduce_body(const MmTab &a, int task, const d
                 for (int h = 0; h < 4; ++h) ua[kt][h] = *(const double2 *)(pa[kt] + r + t0 + 2 * h);
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
-                if (ct >= nct) break; // uniform
+                if (ct >= nct) continue; // uniform
 #pragma unroll
                 for (int h = 0; h < 4; ++h) ub[ct][h] = *(const double2 *)(pb[ct] + r + t0 + 2 * h);
             }
@@ -616,6 +616,9 @@ __device__ __forceinline__ void mm_stage(double *sh, const double *src, size_t c
     }
 }
 
+// NT: column tiles held in registers (4: 64 columns; 1 for at most 16
+// right-hand sides, fewer registers and so more waves per SIMD).
+template <int NT = 4>
 __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
                                                const double *__restrict__ D,
                                                const double *__restrict__ X, int ldx, double alpha,
@@ -628,19 +631,19 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
     const int rb = bx * 64, r0 = rb + wv * 16;
     const int c0 = by * MM_CG;
     const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
-    const int nct = (nc + 15) >> 4; // live 16-column tiles (uniform)
+    const int nct = min(NT, (nc + 15) >> 4); // live 16-column tiles (uniform)
     // A operand (16 columns c, 4 k): lane holds sh[(16ct + m)*65 + k + g]
     // B operand (4 k x 16 rows of Q^T or D^T): lane holds [k + g][r0 + m]
     // Every global load of a chunk (its B operands, and Y at the start) is
     // issued before the LDS staging of the chunk's A operands, so one round
     // trip serves both; barriers keep the compiler from sinking them.
-    d4 acc[4];
+    d4 acc[NT];
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
+    for (int ct = 0; ct < NT; ++ct) acc[ct] = d4{0.0, 0.0, 0.0, 0.0};
     // lane holds Y^T[c0 + 16ct + g + 4v][r0 + m]
-    double yo[4][4];
+    double yo[NT][4];
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
+    for (int ct = 0; ct < NT; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = min(c0 + ct * 16 + g + 4 * v, a.ncols - 1);
@@ -665,15 +668,15 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
             __syncthreads();
             // rows past the chunk read zeros from sh (and real data from D)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                if (ct >= nct) break; // uniform
+            for (int ct = 0; ct < NT; ++ct) {
+                if (ct >= nct) continue; // uniform
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma64(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
             }
         }
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) acc[ct] *= alpha;
+        for (int ct = 0; ct < NT; ++ct) acc[ct] *= alpha;
     }
     for (int d = 0; d < a.L; ++d) {
         const int R = a.R[d];
@@ -697,8 +700,8 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
             __syncthreads();
             // ranks past the chunk read zeros from sh (and real data from Q)
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                if (ct >= nct) break; // uniform
+            for (int ct = 0; ct < NT; ++ct) {
+                if (ct >= nct) continue; // uniform
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma64(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
@@ -706,7 +709,7 @@ __device__ __forceinline__ void mm_expand_body(const MmTab &a, int bx, int by,
         }
     }
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
+    for (int ct = 0; ct < NT; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = c0 + ct * 16 + g + 4 * v;

@@ -221,7 +221,7 @@ __device__ __forceinline__ void sp_stage(float *sh, const float *src, size_t cst
 // W_d^T (sig .* Q_d^T) (sig only for apply tables, a.sx).  X and W are
 // staged in LDS per 64-wide chunk; D and Q come from HBM, 16 consecutive
 // rows per lane group.
-template <typename T>
+template <typename T, int NT = 4>
 __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
                                                const T *__restrict__ D,
                                                const float *__restrict__ X, int ldx, float alpha,
@@ -234,14 +234,14 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
     const int rb = bx * 64, r0 = rb + wv * 16;
     const int c0 = by * MM_CG;
     const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
-    const int nct = (nc + 15) >> 4; // live 16-column tiles (uniform)
-    sp_f4 acc[4];
+    const int nct = min(NT, (nc + 15) >> 4); // live 16-column tiles (uniform)
+    sp_f4 acc[NT];
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct) acc[ct] = sp_f4{0.f, 0.f, 0.f, 0.f};
+    for (int ct = 0; ct < NT; ++ct) acc[ct] = sp_f4{0.f, 0.f, 0.f, 0.f};
     // lane holds Y^T[c0 + 16ct + 4g + v][r0 + m]
-    float yo[4][4];
+    float yo[NT][4];
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
+    for (int ct = 0; ct < NT; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = min(c0 + ct * 16 + sp_drow(g, v), a.ncols - 1);
@@ -264,15 +264,15 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
             sp_stage(sh, xl + j0, ldx, nc, jn);
             __syncthreads();
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                if (ct >= nct) break; // uniform
+            for (int ct = 0; ct < NT; ++ct) {
+                if (ct >= nct) continue; // uniform
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma32(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
             }
         }
 #pragma unroll
-        for (int ct = 0; ct < 4; ++ct) acc[ct] *= alpha;
+        for (int ct = 0; ct < NT; ++ct) acc[ct] *= alpha;
     }
     for (int d = 0; d < a.L; ++d) {
         const int R = a.R[d];
@@ -294,8 +294,8 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
                      a.sx ? Sd + (size_t)(trans ? (s ^ 1) : s) * R + k0 : nullptr);
             __syncthreads();
 #pragma unroll
-            for (int ct = 0; ct < 4; ++ct) {
-                if (ct >= nct) break; // uniform
+            for (int ct = 0; ct < NT; ++ct) {
+                if (ct >= nct) continue; // uniform
 #pragma unroll
                 for (int u = 0; u < 8; ++u)
                     acc[ct] = mfma32(sh[(16 * ct + m) * 65 + 4 * u + g], bq[u], acc[ct]);
@@ -303,7 +303,7 @@ __device__ __forceinline__ void sp_expand_body(const MmTab &a, int bx, int by,
         }
     }
 #pragma unroll
-    for (int ct = 0; ct < 4; ++ct)
+    for (int ct = 0; ct < NT; ++ct)
 #pragma unroll
         for (int v = 0; v < 4; ++v) {
             const int c = c0 + ct * 16 + sp_drow(g, v);
