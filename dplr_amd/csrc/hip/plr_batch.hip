@@ -779,7 +779,7 @@ extern "C" int dplrh_sp_solve(dplrh_plan *p, int prec, int trans, int n, const f
 // timed with events) and keep the fastest.  ms[i] receives the time of
 // candidate i (apply + solve, per repetition) when not null.  The tables
 // and graphs are rebuilt for the chosen tile on the next call.
-static const int kTuneTiles[] = {128, 256, 512, 1024};
+static const int kTuneTiles[] = {128, 256, 512, 1024, 2048, 4096};
 
 extern "C" int dplrh_batch_tune_prec(dplrh_batch *B, int prec, int nrhs, const void *const *X,
                                      const int *ldx, void *const *Y, const int *ldy, int reps,
@@ -851,7 +851,7 @@ extern "C" int dplrh_batch_set_tile(dplrh_batch *B, int tmax)
     bool ok = false;
     for (int t : kTuneTiles) ok = ok || t == tmax;
     if (!ok) {
-        plr_set_error("tile %d is not one of 128, 256, 512, 1024", tmax);
+        plr_set_error("tile %d is not one of 128, 256, 512, 1024, 2048, 4096", tmax);
         return -1;
     }
     batch_drop_graphs(B);

@@ -256,7 +256,7 @@ class Batch:
               "dplrx_batch_solve_refined")
         return X
 
-    TILES = (128, 256, 512, 1024)
+    TILES = (128, 256, 512, 1024, 2048, 4096)
 
     def set_tile(self, tile):
         """Rows per reduction task (one of TILES), without timing."""

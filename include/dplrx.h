@@ -173,10 +173,10 @@ int dplrx_batch_solve(dplrx_batch *batch, int nrhs, const double *const *dB, con
                       double *const *dX, const int *ldx, void *stream);
 /* Autotuner: times the batch's apply + solve (nrhs columns, the caller's
  * buffers X and Y, `reps` repetitions after a warm-up) for each candidate
- * tile shape (rows per reduction task: 128, 256, 512, 1024) and keeps the
- * fastest for later calls.  *tile receives the choice; ms[0..3] (optional)
- * the time per repetition of each candidate.  No reference counterpart:
- * the reference has no batched path. */
+ * tile shape (rows per reduction task: 128, 256, 512, 1024, 2048, 4096) and
+ * keeps the fastest for later calls.  *tile receives the choice; ms[0..5]
+ * (optional) the time per repetition of each candidate.  No reference
+ * counterpart: the reference has no batched path. */
 int dplrx_batch_tune(dplrx_batch *batch, int nrhs, const double *const *dX, const int *ldx,
                      double *const *dY, const int *ldy, int reps, int *tile, double *ms,
                      void *stream);

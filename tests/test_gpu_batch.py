@@ -118,7 +118,7 @@ def test_batch_rebuilds_after_plan_workspace_changes(batch, orc):
         assert orc_rel(orc_apply(orc, T, z), x.cpu().numpy()) < 1e-12, SPECS[k]
 
 
-@pytest.mark.parametrize("tile", [128, 512, 1024])
+@pytest.mark.parametrize("tile", [128, 512, 1024, 4096])
 def test_batched_tiles_match_single(batch, tile):
     """Every tile shape the tuner can pick gives the single-plan results."""
     import torch
