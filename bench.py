@@ -728,7 +728,7 @@ def main():
                            "bytes_per_launch": lp_bytes, "us_per_launch": ms_one * 1e3},
         "roofline": {"bound": "hbm", "achieved": mv_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": mv_gbs / HBM_PEAK_GBS, "traffic": measured_traffic(),
-                     "kernel": "single-RHS FP64 matvec (k_mv_reduce2 + k_mv_finish + k_mv_expand2)",
+                     "kernel": "single-RHS FP64 matvec (k_mv_reduce3 + k_mv_expand4)",
                      "bytes_per_launch": mv_bytes, "us_per_launch": ms_mv * 1e3},
         "cpu_baseline": cpu,
         "batch": batch,

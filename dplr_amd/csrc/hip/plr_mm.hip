@@ -132,7 +132,12 @@ __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__rest
 
 
 // MFMA path needs every segment to be a whole number of 64-row blocks.
-int plr_mm_ok(const dplrh_plan *p) { return p->b % 64 == 0 && p->N >= 128; }
+// Leaf orders 64 * 2^k: every segment is 64 rows or a multiple of 128 (and
+// of 256 from 256 on), which the row groups of the MFMA kernels assume.
+int plr_mm_ok(const dplrh_plan *p)
+{
+    return p->b >= 64 && (p->b & (p->b - 1)) == 0 && p->N >= 128;
+}
 
 static int mm_part(dplrh_plan *p, size_t blocks)
 {

@@ -14,7 +14,9 @@
 //
 // Both kernels take a column count: blockIdx.y selects the column, so the
 // same code serves the multi-RHS product for any shape (each column reads
-// the factors once).
+// the factors once).  Leaf orders 64 * 2^k take the two-launch form below
+// (k_mv_reduce3 + k_mv_expand4/k_mv_expand3: no arrival counters, no fences).
+#include <cstdint>
 #include <cstring>
 #include <mutex>
 
@@ -183,305 +185,245 @@ __global__ __launch_bounds__(256) void k_mv_expand(LevelTab a, const double *__r
     }
 }
 
-// ---------------------------------------------------------------------------
-// Fast path for segments that are whole multiples of 64 rows (leaf order a
-// multiple of 64).  Reduction: one wavefront per (level, 16 ranks, block of
-// rows); lane l streams rows 4l..4l+3 of each 256-row slab (two 16-byte
-// loads per rank column, all issued before use).  The 64 lane sums of each
-// rank meet once in LDS; a lane quarter is 64 consecutive rows, so segments
-// of 64, 128 or >= 256 rows come out of the same reduction.  Tiles of long
-// segments write partials that k_mv_finish sums (no fences, no counters).
-
 typedef double d2v __attribute__((ext_vector_type(2)));
 
-struct Mv2Tab
+// ---------------------------------------------------------------------------
+// Matvec in two launches for leaf orders 64 * 2^k (every segment is 64 rows
+// or a multiple of 128).  The U (and V) factors of all levels are one
+// column-major N x sum(R_d) array, so a rank column of any level is one
+// pointer step from the next.
+//
+// k_mv_reduce3: a work-group of eight waves owns MK rank columns of one level
+// over 1024 rows (wave w: rows 128w..128w+127; lane l: rows 2l and 2l+1, one
+// 16-byte load per column, all MK loads issued at once).  A wave sums its
+// rows through an LDS transpose; segments of 64 and 128 rows are finished by
+// the wave, 256, 512 and 1024 by two, four or eight waves in a fixed order,
+// and longer segments leave one partial per 1024 rows.
+// k_mv_expand3: a work-group of MW waves owns 128 rows.  It first stages
+// alpha times the coefficients of its segments in LDS (summing the partials
+// of segments longer than 1024 rows in a fixed order and scaling by sig),
+// then each wave streams every MW-th column of [leaf | U of every level] in
+// batches of UB 16-byte loads, and the waves' sums meet in LDS.
+
+static constexpr int MV3_RB = 1024;   // rows of one reduction work-group
+static constexpr int MV3_MAXSR = 1536; // most rank columns over all levels
+static constexpr int MV3_MAXB = 1024;  // largest leaf order
+
+struct Mv3Tab
 {
-    int L, N, nb, nch; // reduction batches of KG ranks, expansion chunks of 32 ranks
-    int trans;
-    int R[PLR_MAXL], seg[PLR_MAXL], TW[PLR_MAXL], tps[PLR_MAXL], nkg[PLR_MAXL];
-    int task_off[PLR_MAXL + 1], part_off[PLR_MAXL + 1], fin_off[PLR_MAXL + 1];
-    int w_off[PLR_MAXL + 1];
-    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+    int L, N, nb, ng, SR, trans;
+    int R[PLR_MAXL], seg[PLR_MAXL], kb_off[PLR_MAXL + 1], r_off[PLR_MAXL + 1];
+    int w_off[PLR_MAXL + 1], p_off[PLR_MAXL + 1];
+    const double *P, *Q, *S[PLR_MAXL];
 };
 
-__device__ __forceinline__ double mv_coef(double t, int raw, double alpha, const double *S,
-                                          int sidx, int R, int k)
-{
-    return raw ? t : alpha * S[(size_t)sidx * R + k] * t;
-}
-
-// Reduction t = P|_s^T x|_s for every level at once: a work-group owns 128
-// rows (lane l holds rows 2l, 2l+1, so x is read once for all levels) and
-// its four waves take the level-wise batches of KG columns round-robin.  The
-// 64-lane sums go through a per-wave LDS transpose; segments of 64 or 128
-// rows are finished here, longer ones leave one partial per 128-row chunk
-// for k_mv_finish.  Every lane issues its KG 16-byte loads back to back.
-// The sum over segment s times the sig of its block goes to w at segment
-// s ^ 1 (the rows it is expanded into); the expansion applies alpha.
-__global__ __launch_bounds__(256) void k_mv_reduce2(Mv2Tab a, const double *__restrict__ x,
+template <int MK>
+__global__ __launch_bounds__(512) void k_mv_reduce3(Mv3Tab a, const double *__restrict__ x,
                                                     int ldx, double *__restrict__ part,
                                                     size_t pstride, double *__restrict__ w,
                                                     size_t wstride)
 {
-    __shared__ double red[4][KG][65];
+    __shared__ double red[8][MK][65];
+    __shared__ double sums[8][MK];
     const int lane = threadIdx.x & 63;
     const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col = blockIdx.y;
-    const int N = a.N, row0 = blockIdx.x * 128, i = row0 + 2 * lane;
+    const int grp = blockIdx.x % a.ng, bi = blockIdx.x / a.ng;
+    int d = 0;
+    while (bi >= a.kb_off[d + 1]) ++d;
+    const int kl = (bi - a.kb_off[d]) * MK;
+    const int N = a.N, R = a.R[d], kn = min(MK, R - kl), seg = a.seg[d];
+    const int row0 = grp * MV3_RB + g * 128;
+    const bool live = row0 < N;
     x += (size_t)col * ldx;
     part += (size_t)col * pstride;
     w += (size_t)col * wstride;
-    const d2v xv = *(const d2v *)(x + i);
-    // batch bi -> (level, first column) by a walk of the level table
-    auto locate = [&](int bi, int &dl, int &kl) {
-        int bd = 0;
-        dl = 0;
-        while (bi >= bd + a.nkg[dl]) bd += a.nkg[dl++];
-        kl = (bi - bd) * KG;
-    };
-    auto load = [&](d2v *v, int dl, int kl) {
-        const int kn = min(KG, a.R[dl] - kl);
-        const double *__restrict__ P = a.P[dl] + (size_t)kl * N + i;
+    if (live) {
+        const int i = row0 + 2 * lane;
+        const double *__restrict__ P = a.P + (size_t)(a.r_off[d] + kl) * N + i;
+        d2v v[MK];
 #pragma unroll
-        for (int kk = 0; kk < KG; ++kk)
+        for (int kk = 0; kk < MK; ++kk)
             v[kk] = __builtin_nontemporal_load((const d2v *)(P + (size_t)(kk < kn ? kk : kn - 1) * N));
-    };
-    auto reduce = [&](const d2v *v, int dl, int kl) {
-        const int R = a.R[dl], kn = min(KG, R - kl);
+        const d2v xv = *(const d2v *)(x + i);
 #pragma unroll
-        for (int kk = 0; kk < KG; ++kk) red[g][kk][lane] = v[kk].x * xv.x + v[kk].y * xv.y;
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        // lane (kk, quarter) sums the 16 lanes (32 rows) of one quarter
-        const int kk = lane & 15, qt = lane >> 4;
-        double sq = 0.0;
-#pragma unroll
-        for (int j = 0; j < 16; ++j) sq += red[g][kk][qt * 16 + j];
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-        const int kc = kl + kk, seg = a.seg[dl];
-        const bool ok = kk < kn;
-        sq += __shfl_xor(sq, 16, 64); // 64 rows
-        if (seg == 64) {
-            const int s = row0 / 64 + (qt >> 1);
-            if (ok && (qt & 1) == 0)
-                w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq * a.S[dl][(size_t)(a.trans ? s : s ^ 1) * R + kc];
-            return;
-        }
-        sq += __shfl_xor(sq, 32, 64); // 128 rows
-        if (!ok || qt != 0) return;
-        const int s = row0 / seg;
-        if (seg == 128) {
-            w[a.w_off[dl] + (size_t)(s ^ 1) * R + kc] = sq * a.S[dl][(size_t)(a.trans ? s : s ^ 1) * R + kc];
-        } else {
-            const int tile = (row0 - s * seg) / 128;
-            part[a.part_off[dl] + ((size_t)s * a.tps[dl] + tile) * R + kc] = sq;
-        }
-    };
-    // Software pipeline: the loads of the wave's next batch are issued before
-    // the current batch is reduced.  Two buffers alternate (a register copy
-    // would wait for the prefetch) and no load sits under a branch that
-    // rejoins before its use (the join would wait for every load).
-    const int cnt = g < a.nb ? (a.nb - g + 3) / 4 : 0;
-    if (cnt == 0) return;
-    d2v b0[KG], b1[KG];
-    int d0, k0, d1, k1;
-    locate(g, d0, k0);
-    load(b0, d0, k0);
-    for (int it = 0;; it += 2) {
-        if (it + 1 >= cnt) {
-            reduce(b0, d0, k0);
-            break;
-        }
-        locate(g + 4 * (it + 1), d1, k1);
-        load(b1, d1, k1);
-        reduce(b0, d0, k0);
-        if (it + 2 >= cnt) {
-            reduce(b1, d1, k1);
-            break;
-        }
-        locate(g + 4 * (it + 2), d0, k0);
-        load(b0, d0, k0);
-        reduce(b1, d1, k1);
+        for (int kk = 0; kk < MK; ++kk) red[g][kk][lane] = v[kk].x * xv.x + v[kk].y * xv.y;
     }
-}
-
-// Sum the tile partials of long segments (fixed order) and scale the sum by
-// the sig of its block: a work-group covers 32 consecutive (level, segment,
-// rank) entries with eight tile phases each (eight loads in flight per
-// thread), combined through LDS.
-__global__ __launch_bounds__(256) void k_mv_finish(Mv2Tab a, const double *__restrict__ part,
-                                                   size_t pstride, double *__restrict__ w,
-                                                   size_t wstride)
-{
-    __shared__ double red[8][32];
-    const int el = threadIdx.x & 31, ph = threadIdx.x >> 5;
-    const int e = blockIdx.x * 32 + el;
-    const int col = blockIdx.y;
-    const bool ok = e < a.fin_off[a.L];
-    int d = 0;
-    if (ok)
-        while (e >= a.fin_off[d + 1]) ++d;
-    const int loc = ok ? e - a.fin_off[d] : 0, R = a.R[d], tps = a.tps[d];
-    const int s = loc / (R ? R : 1), k = loc - s * R;
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-    if (ok) {
-        const double *pp = part + (size_t)col * pstride + a.part_off[d] + (size_t)s * tps * R + k;
-        for (int j0 = ph; j0 < tps; j0 += 64) {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    // lane (kk, pt) sums the MK lanes pt*MK .. pt*MK+MK-1 (2*MK rows)
+    const int kk = lane % MK, pt = lane / MK;
+    double sq = 0.0;
 #pragma unroll
-            for (int u = 0; u < 8; ++u) {
-                const int j = j0 + 8 * u;
-                const double v = pp[(size_t)min(j, tps - 1) * R];
-                acc[u] += j < tps ? v : 0.0;
+    for (int j = 0; j < MK; ++j) sq += red[g][kk][pt * MK + j];
+#pragma unroll
+    for (int off = MK; off < 32; off <<= 1) sq += __shfl_xor(sq, off, 64); // 64 rows
+    const bool ok = kk < kn;
+    const double *__restrict__ S = a.S[d];
+    const int trans = a.trans;
+    if (seg == 64) {
+        if (live && ok && (lane & 31) < MK) {
+            const int s = row0 / 64 + (lane >> 5);
+            w[a.w_off[d] + (size_t)(s ^ 1) * R + kl + kk] =
+                sq * S[(size_t)(trans ? s : s ^ 1) * R + kl + kk];
+        }
+        return;
+    }
+    sq += __shfl_xor(sq, 32, 64); // 128 rows
+    if (seg == 128) {
+        if (live && ok && lane < MK) {
+            const int s = row0 / 128;
+            w[a.w_off[d] + (size_t)(s ^ 1) * R + kl + kk] =
+                sq * S[(size_t)(trans ? s : s ^ 1) * R + kl + kk];
+        }
+        return;
+    }
+    if (lane < MK) sums[g][lane] = live ? sq : 0.0;
+    __syncthreads();
+    const int t = threadIdx.x, cps = seg / 128;
+    if (cps <= 8) {
+        // groups of cps waves, one segment each
+        const int gi = t / MK, k2 = t - gi * MK;
+        if (gi < 8 / cps) {
+            double v = 0.0;
+            for (int j = 0; j < cps; ++j) v += sums[gi * cps + j][k2];
+            const int r0 = grp * MV3_RB + gi * cps * 128;
+            if (r0 < N && k2 < kn) {
+                const int s = r0 / seg;
+                w[a.w_off[d] + (size_t)(s ^ 1) * R + kl + k2] =
+                    v * S[(size_t)(trans ? s : s ^ 1) * R + kl + k2];
             }
         }
+    } else if (t < kn) {
+        double v = 0.0;
+#pragma unroll
+        for (int j = 0; j < 8; ++j) v += sums[j][t];
+        const int r0 = grp * MV3_RB, s = r0 / seg, ngs = seg / MV3_RB;
+        const int gis = (r0 - s * seg) / MV3_RB;
+        part[a.p_off[d] + ((size_t)s * ngs + gis) * R + kl + t] = v;
     }
-    red[ph][el] = ((acc[0] + acc[1]) + (acc[2] + acc[3])) + ((acc[4] + acc[5]) + (acc[6] + acc[7]));
-    __syncthreads();
-    if (ph != 0 || !ok) return;
-    const double t = ((red[0][el] + red[1][el]) + (red[2][el] + red[3][el])) +
-                     ((red[4][el] + red[5][el]) + (red[6][el] + red[7][el]));
-    w[(size_t)col * wstride + a.w_off[d] + (size_t)(s ^ 1) * R + k] =
-        t * a.S[d][(size_t)(a.trans ? s : s ^ 1) * R + k];
 }
 
-// Expansion: work-group = 128 rows x 4 rank quarters; lane l holds rows
-// 2l, 2l+1 (16-byte loads), each rank quarter's loads issued in batches of 8.
-__global__ __launch_bounds__(256) void k_mv_expand2(Mv2Tab a, int b, const double *__restrict__ D,
-                                                    const double *__restrict__ x, int ldx,
-                                                    double alpha, double beta, int trans,
-                                                    const double *__restrict__ w, size_t wstride,
-                                                    double *__restrict__ y, int ldy)
+template <int MW, int UB>
+__global__ __launch_bounds__(MW * 64) void k_mv_expand3(Mv3Tab a, int b, const double *__restrict__ D,
+                                                        const double *__restrict__ x, int ldx,
+                                                        double alpha, double beta,
+                                                        const double *__restrict__ part,
+                                                        size_t pstride,
+                                                        const double *__restrict__ w,
+                                                        size_t wstride, double *__restrict__ y,
+                                                        int ldy)
 {
-    __shared__ double2 red[3][64];
+    __shared__ double coef[2][MV3_MAXSR]; // [row half][rank column]
+    __shared__ double xs[2 * 64 > MV3_MAXB ? 2 * 64 : MV3_MAXB];
+    __shared__ double2 red[MW - 1][64];
     const int lane = threadIdx.x & 63;
     const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int col = blockIdx.y;
     x += (size_t)col * ldx;
     y += (size_t)col * ldy;
+    part += (size_t)col * pstride;
     w += (size_t)col * wstride;
-    const int N = a.N;
-    const int row0 = blockIdx.x * 128;
-    const int i = row0 + 2 * lane;
-    double2 lr = {0.0, 0.0}; // sum over the levels, before alpha
-    // The levels are walked as chunks of 32 ranks (wave g takes ranks
-    // k0 + g + 4u), with the buffers in rotation (a register copy, or a load
-    // under a branch that rejoins, would wait for the prefetch).
-    auto locate = [&](int ci, int &dl, int &kl) {
-        int cb = 0;
-        dl = 0;
-        while (ci >= cb + (a.R[dl] + 31) / 32) cb += (a.R[dl++] + 31) / 32;
-        kl = (ci - cb) * 32;
-    };
-    // (w of segment s already carries the sig of the block whose U, V when
-    // transposed, lives on s; alpha is applied at the end)
-    auto load = [&](int dl, int kl, d2v *qv, double *wr) {
-        const int R = a.R[dl], seg = a.seg[dl];
-        const bool uni = (seg & 127) == 0;
-        const int s = uni ? __builtin_amdgcn_readfirstlane(row0 / seg) : i / seg;
-        const double *__restrict__ Q = a.Q[dl];
-        const double *__restrict__ wd = w + a.w_off[dl] + (size_t)s * R;
+    const int N = a.N, SR = a.SR, trans = a.trans;
+    const int row0 = blockIdx.x * 128, i = row0 + 2 * lane;
+    const int leaf0 = row0 / b, nx = b == 64 ? 128 : b;
+    // column c < b: leaf column c; c >= b: rank column c - b of every level.
+    // Wave g takes columns c0 + g + MW * (UB * it + u), batch it of UB.
+    const int h = lane >> 5; // row half (64-row segments, 64-row leaves)
+    const int il = i - leaf0 * b - h * (b == 64 ? 64 : 0);
+    const double *__restrict__ Dl = D + (size_t)(leaf0 + (b == 64 ? h : 0)) * b * b;
+    const double *__restrict__ xl = xs + (b == 64 ? h * 64 : 0);
+    const int ncol = (D ? b : 0) + SR, c0 = D ? 0 : b, cend = c0 + ncol;
+    const int nbat = (ncol - g + MW * UB - 1) / (MW * UB);
+    auto load = [&](d2v *v, int it) {
 #pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const int k = min(kl + g + 4 * u, R - 1);
-            qv[u] = __builtin_nontemporal_load((const d2v *)(Q + (size_t)k * N + i));
-            wr[u] = wd[k];
+        for (int u = 0; u < UB; ++u) {
+            const int c = min(c0 + g + MW * (UB * it + u), cend - 1);
+            const bool lf = c < b;
+            const double *src = lf ? (trans ? Dl + (size_t)il * b + c : Dl + (size_t)c * b + il)
+                                   : a.Q + (size_t)(c - b) * N + i;
+            if (lf && trans) {
+                v[u].x = src[0];
+                v[u].y = src[b];
+            } else {
+                v[u] = __builtin_nontemporal_load((const d2v *)src);
+            }
         }
     };
-    auto comp = [&](int dl, int kl, const d2v *qv, const double *wr) {
-        const int R = a.R[dl];
-#pragma unroll
-        for (int u = 0; u < 8; ++u) {
-            const double c = wr[u] * (double)(kl + g + 4 * u < R);
-            lr.x += qv[u].x * c;
-            lr.y += qv[u].y * c;
-        }
-    };
-    // three chunk buffers in rotation; the first two chunks' loads go out
-    // before the leaf product, so they are in flight while the leaf waits on
-    // its own (chunk 1 is clamped to chunk 0 when there is only one)
-    d2v q0[8], q1[8], q2[8];
-    double w0[8], w1[8], w2[8];
-    int d0 = 0, k0 = 0, d1 = 0, k1 = 0, d2, k2;
-    if (a.nch > 0) {
-        locate(0, d0, k0);
-        load(d0, k0, q0, w0);
-        locate(min(1, a.nch - 1), d1, k1);
-        load(d1, k1, q1, w1);
-    }
     double2 acc = {0.0, 0.0};
-    if (D) {
-        const int leaf = i / b, il = i - leaf * b;
-        const double *__restrict__ Dl = D + (size_t)leaf * b * b;
-        const double *__restrict__ xl = x + (size_t)leaf * b;
-        double2 sl = {0.0, 0.0};
-        if (!trans) {
-            // sixteen leaf columns per wave in flight at once
-            for (int j0 = g; j0 < b; j0 += 64) {
-                double2 dv[16];
-                double xv[16];
+    auto comp = [&](const d2v *v, int it) {
 #pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const int j = min(j0 + 4 * u, b - 1);
-                    dv[u] = *(const double2 *)(Dl + (size_t)j * b + il);
-                    xv[u] = xl[j];
-                }
-#pragma unroll
-                for (int u = 0; u < 16; ++u) {
-                    const double xu = j0 + 4 * u < b ? xv[u] : 0.0;
-                    sl.x += dv[u].x * xu;
-                    sl.y += dv[u].y * xu;
-                }
-            }
+        for (int u = 0; u < UB; ++u) {
+            const int c = c0 + g + MW * (UB * it + u);
+            const int cc = min(c, cend - 1);
+            const double m = cc < b ? xl[cc] : coef[h][cc - b];
+            const double mu = c < cend ? m : 0.0;
+            acc.x += v[u].x * mu;
+            acc.y += v[u].y * mu;
+        }
+    };
+    // the first batch is in flight while the coefficients are staged
+    d2v va[UB], vb[UB];
+    if (nbat > 0) load(va, 0);
+    // alpha * x of the block's leaves, and alpha * the coefficients
+    for (int t = threadIdx.x; t < nx; t += MW * 64) xs[t] = alpha * x[(size_t)leaf0 * b + t];
+    for (int q = threadIdx.x; q < SR; q += MW * 64) {
+        int d = 0;
+        while (q >= a.r_off[d + 1]) ++d;
+        const int k = q - a.r_off[d], R = a.R[d], seg = a.seg[d];
+        if (seg == 64) {
+            const int s = row0 / 64;
+            coef[0][q] = alpha * w[a.w_off[d] + (size_t)s * R + k];
+            coef[1][q] = alpha * w[a.w_off[d] + (size_t)(s + 1) * R + k];
         } else {
-            for (int j = g; j < b; j += 4) {
-                sl.x += Dl[(size_t)il * b + j] * xl[j];
-                sl.y += Dl[(size_t)(il + 1) * b + j] * xl[j];
+            const int s = row0 / seg;
+            double v;
+            if (seg > MV3_RB) {
+                const int sp = s ^ 1, ngs = seg / MV3_RB;
+                const double *__restrict__ pp = part + a.p_off[d] + (size_t)sp * ngs * R + k;
+                double v0 = 0.0, v1 = 0.0;
+                int j = 0;
+                for (; j + 1 < ngs; j += 2) {
+                    v0 += pp[(size_t)j * R];
+                    v1 += pp[(size_t)(j + 1) * R];
+                }
+                if (j < ngs) v0 += pp[(size_t)j * R];
+                v = (v0 + v1) * a.S[d][(size_t)(trans ? sp : s) * R + k];
+            } else {
+                v = w[a.w_off[d] + (size_t)s * R + k];
             }
-        }
-        acc.x = alpha * sl.x;
-        acc.y = alpha * sl.y;
-    }
-    // (the loads of chunk it + 2 are issued before chunk it is used; a
-    // branch either exits the loop or holds no load)
-    if (a.nch > 0) {
-        for (int it = 0;; it += 3) {
-            if (it + 2 >= a.nch) {
-                comp(d0, k0, q0, w0);
-                if (it + 1 < a.nch) comp(d1, k1, q1, w1);
-                break;
-            }
-            locate(it + 2, d2, k2);
-            load(d2, k2, q2, w2);
-            comp(d0, k0, q0, w0);
-            if (it + 3 >= a.nch) {
-                comp(d1, k1, q1, w1);
-                comp(d2, k2, q2, w2);
-                break;
-            }
-            locate(it + 3, d0, k0);
-            load(d0, k0, q0, w0);
-            comp(d1, k1, q1, w1);
-            if (it + 4 >= a.nch) {
-                comp(d2, k2, q2, w2);
-                comp(d0, k0, q0, w0);
-                break;
-            }
-            locate(it + 4, d1, k1);
-            load(d1, k1, q1, w1);
-            comp(d2, k2, q2, w2);
+            coef[0][q] = coef[1][q] = alpha * v;
         }
     }
-    acc.x += alpha * lr.x;
-    acc.y += alpha * lr.y;
+    __syncthreads();
+    // two buffers in rotation: batch it + 1 is issued before batch it is
+    // used, and every branch either leaves the loop or holds no load
+    if (nbat > 0) {
+        for (int it = 0;; it += 2) {
+            if (it + 1 >= nbat) {
+                comp(va, it);
+                break;
+            }
+            load(vb, it + 1);
+            comp(va, it);
+            if (it + 2 >= nbat) {
+                comp(vb, it + 1);
+                break;
+            }
+            load(va, it + 2);
+            comp(vb, it + 1);
+        }
+    }
     if (g > 0) red[g - 1][lane] = acc;
     __syncthreads();
     if (g == 0) {
-        double2 r;
-        r.x = acc.x + red[0][lane].x + red[1][lane].x + red[2][lane].x;
-        r.y = acc.y + red[0][lane].y + red[1][lane].y + red[2][lane].y;
+        double2 r = acc;
+#pragma unroll
+        for (int j = 0; j < MW - 1; ++j) {
+            r.x += red[j][lane].x;
+            r.y += red[j][lane].y;
+        }
         double2 *yp = (double2 *)(y + i);
         if (beta != 0.0) {
             const double2 y0 = *yp;
@@ -492,75 +434,227 @@ __global__ __launch_bounds__(256) void k_mv_expand2(Mv2Tab a, int b, const doubl
     }
 }
 
-static void mv2_tab(const dplrh_plan *p, int trans, Mv2Tab *t)
+// The expansion over 256 rows per work-group (N a multiple of 256): lane l
+// holds rows 2l, 2l+1 and 128 + 2l, 128 + 2l + 1 (two 16-byte loads per
+// column), so every column is read as one 2 KB stretch per work-group.
+template <int MW, int UB>
+__global__ __launch_bounds__(MW * 64) void k_mv_expand4(Mv3Tab a, int b, const double *__restrict__ D,
+                                                        const double *__restrict__ x, int ldx,
+                                                        double alpha, double beta,
+                                                        const double *__restrict__ part,
+                                                        size_t pstride,
+                                                        const double *__restrict__ w,
+                                                        size_t wstride, double *__restrict__ y,
+                                                        int ldy)
+{
+    __shared__ double coef[4][MV3_MAXSR]; // [64-row quarter][rank column]
+    __shared__ double xs[256 > MV3_MAXB ? 256 : MV3_MAXB];
+    __shared__ double2 red[MW - 1][2][64];
+    const int lane = threadIdx.x & 63;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int col = blockIdx.y;
+    x += (size_t)col * ldx;
+    y += (size_t)col * ldy;
+    part += (size_t)col * pstride;
+    w += (size_t)col * wstride;
+    const int N = a.N, SR = a.SR, trans = a.trans;
+    const int row0 = blockIdx.x * 256;
+    const int i0 = row0 + 2 * lane, i1 = i0 + 128;
+    const int h0 = lane >> 5, h1 = 2 + h0; // 64-row quarters of the two row pairs
+    const int xoff = b <= 256 ? row0 : row0 / b * b, nx = b <= 256 ? 256 : b;
+    const int lf0 = i0 / b, lf1 = i1 / b;
+    const int il0 = i0 - lf0 * b, il1 = i1 - lf1 * b;
+    const double *__restrict__ D0 = D + (size_t)lf0 * b * b;
+    const double *__restrict__ D1 = D + (size_t)lf1 * b * b;
+    const double *__restrict__ x0 = xs + (lf0 * b - xoff);
+    const double *__restrict__ x1 = xs + (lf1 * b - xoff);
+    // column c < b: leaf column c; c >= b: rank column c - b of every level.
+    // Wave g takes columns c0 + g + MW * (UB * it + u), batch it of UB.
+    const int ncol = (D ? b : 0) + SR, c0 = D ? 0 : b, cend = c0 + ncol;
+    const int nbat = (ncol - g + MW * UB - 1) / (MW * UB);
+    auto load = [&](d2v *v, int it) {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int c = min(c0 + g + MW * (UB * it + u), cend - 1);
+            const bool lf = c < b;
+            if (lf && trans) {
+                v[2 * u].x = D0[(size_t)il0 * b + c];
+                v[2 * u].y = D0[(size_t)(il0 + 1) * b + c];
+                v[2 * u + 1].x = D1[(size_t)il1 * b + c];
+                v[2 * u + 1].y = D1[(size_t)(il1 + 1) * b + c];
+            } else {
+                const double *s0 = lf ? D0 + (size_t)c * b + il0 : a.Q + (size_t)(c - b) * N + i0;
+                const double *s1 = lf ? D1 + (size_t)c * b + il1 : a.Q + (size_t)(c - b) * N + i1;
+                v[2 * u] = __builtin_nontemporal_load((const d2v *)s0);
+                v[2 * u + 1] = __builtin_nontemporal_load((const d2v *)s1);
+            }
+        }
+    };
+    double2 acc0 = {0.0, 0.0}, acc1 = {0.0, 0.0};
+    auto comp = [&](const d2v *v, int it) {
+#pragma unroll
+        for (int u = 0; u < UB; ++u) {
+            const int c = c0 + g + MW * (UB * it + u);
+            const int cc = min(c, cend - 1);
+            const bool lf = cc < b;
+            const double m0 = lf ? x0[cc] : coef[h0][cc - b];
+            const double m1 = lf ? x1[cc] : coef[h1][cc - b];
+            const bool in = c < cend;
+            const double mu0 = in ? m0 : 0.0, mu1 = in ? m1 : 0.0;
+            acc0.x += v[2 * u].x * mu0;
+            acc0.y += v[2 * u].y * mu0;
+            acc1.x += v[2 * u + 1].x * mu1;
+            acc1.y += v[2 * u + 1].y * mu1;
+        }
+    };
+    d2v va[2 * UB], vb[2 * UB];
+    if (nbat > 0) load(va, 0);
+    // alpha * x of the block's leaves, and alpha * the coefficients of the
+    // segments holding each 64-row quarter
+    for (int t = threadIdx.x; t < nx; t += MW * 64) xs[t] = alpha * x[(size_t)xoff + t];
+    for (int q = threadIdx.x; q < SR; q += MW * 64) {
+        int d = 0;
+        while (q >= a.r_off[d + 1]) ++d;
+        const int k = q - a.r_off[d], R = a.R[d], seg = a.seg[d];
+        const int nsg = seg >= 256 ? 1 : 256 / seg, per = 4 / nsg;
+        for (int j = 0; j < nsg; ++j) {
+            const int s = row0 / seg + j;
+            double v;
+            if (seg > MV3_RB) {
+                const int sp = s ^ 1, ngs = seg / MV3_RB;
+                const double *__restrict__ pp = part + a.p_off[d] + (size_t)sp * ngs * R + k;
+                double v0 = 0.0, v1 = 0.0;
+                int jj = 0;
+                for (; jj + 1 < ngs; jj += 2) {
+                    v0 += pp[(size_t)jj * R];
+                    v1 += pp[(size_t)(jj + 1) * R];
+                }
+                if (jj < ngs) v0 += pp[(size_t)jj * R];
+                v = (v0 + v1) * a.S[d][(size_t)(trans ? sp : s) * R + k];
+            } else {
+                v = w[a.w_off[d] + (size_t)s * R + k];
+            }
+            for (int hh = 0; hh < per; ++hh) coef[j * per + hh][q] = alpha * v;
+        }
+    }
+    __syncthreads();
+    if (nbat > 0) {
+        for (int it = 0;; it += 2) {
+            if (it + 1 >= nbat) {
+                comp(va, it);
+                break;
+            }
+            load(vb, it + 1);
+            comp(va, it);
+            if (it + 2 >= nbat) {
+                comp(vb, it + 1);
+                break;
+            }
+            load(va, it + 2);
+            comp(vb, it + 1);
+        }
+    }
+    if (g > 0) {
+        red[g - 1][0][lane] = acc0;
+        red[g - 1][1][lane] = acc1;
+    }
+    __syncthreads();
+    if (g == 0) {
+        double2 r0 = acc0, r1 = acc1;
+#pragma unroll
+        for (int j = 0; j < MW - 1; ++j) {
+            r0.x += red[j][0][lane].x;
+            r0.y += red[j][0][lane].y;
+            r1.x += red[j][1][lane].x;
+            r1.y += red[j][1][lane].y;
+        }
+        double2 *y0p = (double2 *)(y + i0), *y1p = (double2 *)(y + i1);
+        if (beta != 0.0) {
+            const double2 ya = *y0p, yb = *y1p;
+            r0.x += beta * ya.x;
+            r0.y += beta * ya.y;
+            r1.x += beta * yb.x;
+            r1.y += beta * yb.y;
+        }
+        *y0p = r0;
+        *y1p = r1;
+    }
+}
+
+// Shapes the two-launch matvec covers: leaf order 64 * 2^k up to MV3_MAXB,
+// at least one level, and at most MV3_MAXSR rank columns over all levels.
+static bool mv3_ok(const dplrh_plan *p)
+{
+    if (p->L < 1 || p->b < 64 || p->b > MV3_MAXB || (p->b & (p->b - 1)) != 0) return false;
+    return p->uoff[p->L] / (size_t)p->N <= (size_t)MV3_MAXSR;
+}
+
+static void mv3_tab(const dplrh_plan *p, int trans, int mk, Mv3Tab *t)
 {
     memset(t, 0, sizeof(*t));
     t->L = p->L;
     t->N = p->N;
     t->trans = trans;
-    int nb = 0, part = 0, fin = 0;
+    t->ng = (p->N + MV3_RB - 1) / MV3_RB;
+    int nb = 0, part = 0, ro = 0;
     for (int d = 0; d < p->L; ++d) {
-        const int seg = p->N >> (d + 1), nseg = 2 << d, R = p->R[d];
-        const int tps = seg <= 128 ? 1 : seg / 128;
-        const int nkg = (R + KG - 1) / KG;
+        const int seg = p->N >> (d + 1), R = p->R[d];
         t->R[d] = R;
         t->seg[d] = seg;
-        t->TW[d] = 128;
-        t->tps[d] = tps;
-        t->nkg[d] = nkg;
-        t->task_off[d] = nb;
-        nb += nkg;
-        t->part_off[d] = part;
-        if (tps > 1) part += nseg * tps * R;
-        t->fin_off[d] = fin;
-        if (tps > 1) fin += nseg * R;
+        t->kb_off[d] = nb;
+        nb += (R + mk - 1) / mk;
+        t->r_off[d] = ro;
+        ro += R;
         t->w_off[d] = (int)p->soff[d];
-        const double *Ud = p->U + p->uoff[d], *Vd = p->V + p->uoff[d];
-        t->P[d] = trans ? Ud : Vd;
-        t->Q[d] = trans ? Vd : Ud;
+        t->p_off[d] = part;
+        if (seg > MV3_RB) part += (2 << d) * (seg / MV3_RB) * R;
         t->S[d] = p->S + p->soff[d];
     }
     t->nb = nb;
-    t->nch = 0;
-    for (int d = 0; d < p->L; ++d) t->nch += (p->R[d] + 31) / 32;
-    t->task_off[p->L] = nb;
-    t->part_off[p->L] = part;
-    t->fin_off[p->L] = fin;
+    t->SR = ro;
+    t->kb_off[p->L] = nb;
+    t->r_off[p->L] = ro;
     t->w_off[p->L] = (int)p->soff[p->L];
+    t->p_off[p->L] = part;
+    t->P = trans ? p->U : p->V;
+    t->Q = trans ? p->V : p->U;
 }
 
-// Fast-path matvec for nc columns; the partial buffer p->part holds
-// part_per_col doubles per column (sized by plr_workspace for this layout).
-static int mv2_launch_ws(dplrh_plan *p, int trans, int nc, double alpha, double beta,
+// Launch shapes (measured at N=65536, L=10, leaf 64, rank 32: reduction
+// 28.4 us, expansion 37.0 us for 370.7 MB)
+static constexpr int MV3_MK = 16, MV4_MW = 16, MV4_UB = 2, MV3_MW = 16, MV3_UB = 4;
+
+static int mv3_launch_ws(dplrh_plan *p, int trans, int nc, double alpha, double beta,
                          const double *x, int ldx, double *y, int ldy, double *part,
                          size_t pstride, double *w, size_t wstride, hipStream_t st)
 {
-    Mv2Tab t;
-    mv2_tab(p, trans, &t);
-    if ((size_t)t.part_off[t.L] > pstride) {
+    Mv3Tab t;
+    mv3_tab(p, trans, MV3_MK, &t);
+    if ((size_t)t.p_off[t.L] > pstride) {
         plr_set_error("matvec workspace too small");
         return -9;
     }
     if (t.nb > 0) {
-        hipLaunchKernelGGL(k_mv_reduce2, dim3(p->N / 128, nc), dim3(256), 0, st, t, x, ldx, part,
-                           pstride, w, wstride);
+        hipLaunchKernelGGL(k_mv_reduce3<MV3_MK>, dim3(t.ng * t.nb, nc), dim3(512), 0, st, t, x,
+                           ldx, part, pstride, w, wstride);
         PLR_HIP(hipGetLastError());
     }
-    if (t.fin_off[t.L] > 0) {
-        hipLaunchKernelGGL(k_mv_finish, dim3((t.fin_off[t.L] + 31) / 32, nc), dim3(256), 0, st, t,
-                           part, pstride, w, wstride);
-        PLR_HIP(hipGetLastError());
-    }
-    hipLaunchKernelGGL(k_mv_expand2, dim3(p->N / 128, nc), dim3(256), 0, st, t, p->b, p->D, x, ldx,
-                       alpha, beta, trans, w, wstride, y, ldy);
+    if (p->N % 256 == 0)
+        hipLaunchKernelGGL((k_mv_expand4<MV4_MW, MV4_UB>), dim3(p->N / 256, nc),
+                           dim3(MV4_MW * 64), 0, st, t, p->b, p->D, x, ldx, alpha, beta, part,
+                           pstride, w, wstride, y, ldy);
+    else
+        hipLaunchKernelGGL((k_mv_expand3<MV3_MW, MV3_UB>), dim3(p->N / 128, nc),
+                           dim3(MV3_MW * 64), 0, st, t, p->b, p->D, x, ldx, alpha, beta, part,
+                           pstride, w, wstride, y, ldy);
     PLR_HIP(hipGetLastError());
     return 0;
 }
 
-static int mv2_launch(dplrh_plan *p, int trans, int nc, double alpha, double beta,
+static int mv3_launch(dplrh_plan *p, int trans, int nc, double alpha, double beta,
                       const double *x, int ldx, double *y, int ldy, hipStream_t st)
 {
-    return mv2_launch_ws(p, trans, nc, alpha, beta, x, ldx, y, ldy, p->part, p->part_per_col, p->w,
+    return mv3_launch_ws(p, trans, nc, alpha, beta, x, ldx, y, ldy, p->part, p->part_per_col, p->w,
                          p->w_per_col, st);
 }
 
@@ -599,11 +693,13 @@ int plr_mv_launch(dplrh_plan *p, int trans, int ncols, double alpha, double beta
     const int chunk = ncols < kColChunk ? ncols : kColChunk;
     int rc = plr_workspace(p, chunk);
     if (rc) return rc;
-    const bool fast = plr_mm_ok(p) && !p->force_valu && ldx % 2 == 0 && ldy % 2 == 0;
+    // the fast kernels read x and y two rows at a time (16-byte loads)
+    const bool fast = mv3_ok(p) && !p->force_valu && ldx % 2 == 0 && ldy % 2 == 0 &&
+                      (((uintptr_t)dx | (uintptr_t)dy) & 15) == 0;
     if (fast) {
         for (int c0 = 0; c0 < ncols; c0 += chunk) {
             const int nc = ncols - c0 < chunk ? ncols - c0 : chunk;
-            rc = mv2_launch(p, trans, nc, alpha, beta, dx + (size_t)c0 * ldx, ldx,
+            rc = mv3_launch(p, trans, nc, alpha, beta, dx + (size_t)c0 * ldx, ldx,
                             dy + (size_t)c0 * ldy, ldy, st);
             if (rc) return rc;
         }
@@ -726,6 +822,14 @@ extern "C" int dplrh_gemv_noalloc(dplrh_plan *p, int trans, double alpha, double
     if (alpha == 0.0) return dplrh_scale2(p->N, 1, beta, dy, p->N, st);
     size_t part, w, cnt;
     plr_ws_per_col(p, &part, &w, &cnt);
+    const bool two = mv3_ok(p) && !p->force_valu && (((uintptr_t)dx | (uintptr_t)dy) & 15) == 0;
+    if (two) {
+        // the two-launch kernels use no counters and fewer partials
+        Mv3Tab t;
+        mv3_tab(p, trans, MV3_MK, &t);
+        part = (size_t)t.p_off[t.L];
+        cnt = 0;
+    }
     // 256-byte aligned regions: partials, coefficients, counters (ints)
     const size_t part_r = (part + 31) & ~(size_t)31, w_r = (w + 31) & ~(size_t)31;
     const size_t cnt_d = ((cnt + 1) / 2 + 31) & ~(size_t)31;
@@ -740,8 +844,8 @@ extern "C" int dplrh_gemv_noalloc(dplrh_plan *p, int trans, double alpha, double
     if (rc) return rc;
     double *pp = base, *wp = base + part_r;
     int *cp = (int *)(wp + w_r);
-    if (plr_mm_ok(p) && !p->force_valu)
-        return mv2_launch_ws(p, trans, 1, alpha, beta, dx, p->N, dy, p->N, pp, part, wp, w, st);
+    if (two)
+        return mv3_launch_ws(p, trans, 1, alpha, beta, dx, p->N, dy, p->N, pp, part, wp, w, st);
     LevelTab t;
     plr_level_tab(p, trans, &t);
     const int ntask = t.task_off[t.L];

@@ -131,9 +131,10 @@ void plr_ws_per_col(const dplrh_plan *p, size_t *part_out, size_t *w_out, size_t
     size_t part = 0, part2 = 0;
     for (int d = 0; d < p->L; ++d) {
         if (t.tps[d] > 1) part += (size_t)(2 << d) * t.tps[d] * t.R[d];
-        // layout of the fast matvec path: one partial per 128-row chunk
+        // layout of the two-launch matvec: one partial per 1024 rows of the
+        // segments longer than 1024 rows
         const int seg = p->N >> (d + 1);
-        if (seg > 128) part2 += (size_t)(2 << d) * (seg / 128) * t.R[d];
+        if (seg > 1024) part2 += (size_t)(2 << d) * (seg / 1024) * t.R[d];
     }
     *part_out = part2 > part ? part2 : part;
     *w_out = p->soff[p->L];

@@ -24,6 +24,11 @@ SHAPES = [
     (96, 5, 2),      # leaf 3, non power-of-two segments
     (64, 0, 0),      # dense root
     (256, 3, 0),     # all ranks zero
+    (6144, 5, 16),   # leaf 192: segments of 192 rows, not a power-of-two leaf
+    (4096, 4, 24),   # leaf 256: one leaf per 256-row expansion group
+    (8192, 3, 16),   # leaf 1024, the largest leaf of the two-launch matvec
+    (128, 1, 4),     # N = 128: 128-row expansion groups
+    (16384, 5, 20),  # leaf 512, segments longer than 1024 rows (partials)
 ]
 
 
@@ -122,6 +127,7 @@ def test_full_size_matvec(cuda, orc):
     (8192, 7, 64, 40),    # rank 64: 28 rank tiles, two passes over the staged X
     (2048, 6, 16, 8),     # leaf 32: the general path
     (3072, 5, 16, 8),     # leaf 96: the general path
+    (6144, 5, 16, 8),     # leaf 192: segments of 192 rows
     (256, 2, 8, 5),       # no level spans a 256-row group: nothing staged
     (512, 3, 12, 7),      # one staged level of 12 ranks
 ])

@@ -48,18 +48,20 @@ def test_noalloc_matvec_matches_gepmv(cuda, trans, beta):
 
 
 def test_noalloc_matvec_rejects_large_plans(cuda):
-    """N = 65536 on 9 levels (leaf 128) with rank 128 needs 5.2 MB of
-    scratch, more than the 4 MB the entry point holds: a clear error and
+    """N = 262144 on 12 levels (leaf 64) with rank 64 needs 5.1 MB of
+    scratch (4.2 MB of coefficients, one per segment and rank, and 0.9 MB of
+    partials), more than the 4 MB the entry point holds: a clear error and
     nothing launched (y unchanged)."""
     import torch
-    A = PlrTree.random(65536, 9, 128, 4.0, seed=70)
+    m = 262144
+    A = PlrTree.random(m, 12, 64, 4.0, seed=70)
     P = D.Plan(A)
-    x = torch.ones(65536, dtype=torch.float64, device=cuda)
-    y = torch.full((65536,), 2.0, dtype=torch.float64, device=cuda)
+    A.free()
+    x = torch.ones(m, dtype=torch.float64, device=cuda)
+    y = torch.full((m,), 2.0, dtype=torch.float64, device=cuda)
     rc = call(lib.dplrx_gepmv_noalloc, P, "N", 1.0, x, 0.0, y, torch)
     torch.cuda.synchronize()
     assert rc == -7
     assert b"scratch" in lib.dplrx_last_error()
     assert torch.all(y == 2.0)
     P.close()
-    A.free()
