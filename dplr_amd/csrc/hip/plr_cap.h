@@ -11,7 +11,9 @@
 // chunk); its four waves take 16-row tiles of c; the right-hand sides sit in
 // LDS as rhs[j][column], inv(K) goes straight from HBM into the A operand in
 // batches of 16 loads.  With LP the outputs are FP32 and each (segment,
-// column) also gets the power-of-two scale of its largest magnitude.
+// column) also gets the power-of-two scale of its largest magnitude.  Output
+// element (row j, column c) goes to W[c * wstride + j * rs]: rs = 1 keeps
+// the layout of T, wstride = 1 and rs = the column count is rank-major.
 #pragma once
 
 #include "plr_common.h"
@@ -23,7 +25,7 @@ __device__ __forceinline__ void cap_body(int node, int cblk, const KT *__restric
                                          const VT *__restrict__ S, const VT *__restrict__ T,
                                          size_t tstride, VT *__restrict__ W, size_t wstride,
                                          int trans, int ncols, float *__restrict__ wscale,
-                                         size_t sstride)
+                                         size_t sstride, size_t rs = 1)
 {
     // rhs[j][column], j < n2p (rows n2..n2p-1 zero), row stride cw + 1
     extern __shared__ double rhs[];
@@ -105,7 +107,7 @@ __device__ __forceinline__ void cap_body(int node, int cblk, const KT *__restric
                 const int sg = (i < R) != (trans != 0) ? top : bot;
                 const double sv = (double)S[(size_t)sg * R + kk];
                 const VT out = (VT)(trans ? -c : -sv * c);
-                w[(size_t)sg * R + kk] = out;
+                w[((size_t)sg * R + kk) * rs] = out;
                 if (LP) atomicMax(&smax[cl][i < R ? 0 : 1], __float_as_uint(fabsf((float)out)));
             }
         }
@@ -128,10 +130,10 @@ __global__ __launch_bounds__(256) void k_cap_mm(const KT *__restrict__ Kinv, int
                                                 const VT *__restrict__ T, size_t tstride,
                                                 VT *__restrict__ W, size_t wstride, int trans,
                                                 int ncols, float *__restrict__ wscale,
-                                                size_t sstride)
+                                                size_t sstride, size_t rs)
 {
     cap_body<KT, VT, LP, NCT>(blockIdx.x, blockIdx.y, Kinv, R, S, T, tstride, W, wstride, trans,
-                              ncols, wscale, sstride);
+                              ncols, wscale, sstride, rs);
 }
 
 // Column tiles per work-group: at most four, with the staged right-hand
@@ -147,7 +149,7 @@ static inline int plr_cap_nct(int n2)
 template <typename KT, typename VT, bool LP>
 static int plr_cap_launch(const KT *Kinv, int d, int R, const VT *S, const VT *T, size_t tstride,
                           VT *W, size_t wstride, int trans, int ncols, float *wscale,
-                          size_t sstride, hipStream_t st)
+                          size_t sstride, hipStream_t st, size_t rs = 1)
 {
     const int n2 = 2 * R, n2p = (n2 + 63) & ~63;
     if (ncols <= 0 || R <= 0) return 0;
@@ -161,7 +163,7 @@ static int plr_cap_launch(const KT *Kinv, int d, int R, const VT *S, const VT *T
 #define PLR_CAP_CASE(n)                                                                            \
     case n:                                                                                        \
         hipLaunchKernelGGL((k_cap_mm<KT, VT, LP, n>), grid, dim3(256), lds, st, Kinv, R, S, T,     \
-                           tstride, W, wstride, trans, ncols, wscale, sstride);                    \
+                           tstride, W, wstride, trans, ncols, wscale, sstride, rs);                \
         break;
     switch (nct) {
         PLR_CAP_CASE(1)

@@ -407,6 +407,33 @@ int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W,
                                nullptr, st);
 }
 
+// One level through the third-form expansion: Y := beta*Y + Q_d|_s W[s]
+// with W rank-major and complete (w[(s*R_d + k) * ws + c], no sig), or with
+// d < 0 the leaf term Y := alpha*D X + beta*Y.  Returns 1, launching
+// nothing, when the shape or the operands do not suit that kernel.
+int plr_mm_expand_lvl(dplrh_plan *p, int d, const double *Q, const double *W, size_t ws,
+                      const double *D, const double *X, int ldx, double alpha, double beta,
+                      double *Y, int ldy, int ncols, hipStream_t st)
+{
+    const double *P = nullptr, *S = nullptr;
+    MmTab t;
+    if (d >= 0)
+        mm_tab(p, d, d + 1, &P, &Q, &S, ncols, 1, &t);
+    else
+        mm_tab(p, 0, 0, &P, &Q, &S, ncols, 1, &t);
+    t.nosig = 1;
+    const bool leaves_ok = p->b >= 64 && (p->b & (p->b - 1)) == 0;
+    if (p->N % ME3_RB != 0 || !leaves_ok || p->force_valu || ldy % 2 != 0 ||
+        ((uintptr_t)Y & 15) != 0)
+        return 1;
+    if (d < 0 && (!D || ldx % 2 != 0 || ((uintptr_t)X & 15) != 0)) return 1;
+    if (d >= 0 && ((size_t)t.ncg * MM_CG > ws || ((uintptr_t)W & 15) != 0)) return 1;
+    // the kernel reads one element of w even when no level is staged: with
+    // no level it gets X, a valid address whose value it does not use
+    return mm_expand_launch<0>(p, t, d < 0 ? D : nullptr, X, ldx, alpha, beta, d < 0 ? X : W,
+                               (int)ws, Y, ldy, Y, ldy, nullptr, st);
+}
+
 // T[c][s*R_d + k] = sum of the raw partial blocks in p->tmp that a caller
 // wrote in mm_reduce_body's layout with tiles of `tmax` rows (level d,
 // ncols columns).

@@ -507,8 +507,13 @@ __global__ __launch_bounds__(MW * 64) void k_mv_expand4(Mv3Tab a, int b, const d
             acc1.y += v[2 * u + 1].y * mu1;
         }
     };
+    // the first two batches are in flight while the coefficients are staged
+    // (a single-batch wave loads its batch twice)
     d2v va[2 * UB], vb[2 * UB];
-    if (nbat > 0) load(va, 0);
+    if (nbat > 0) {
+        load(va, 0);
+        load(vb, min(1, nbat - 1));
+    }
     // alpha * x of the block's leaves, and alpha * the coefficients of the
     // segments holding each 64-row quarter
     for (int t = threadIdx.x; t < nx; t += MW * 64) xs[t] = alpha * x[(size_t)xoff + t];
@@ -539,19 +544,24 @@ __global__ __launch_bounds__(MW * 64) void k_mv_expand4(Mv3Tab a, int b, const d
     }
     __syncthreads();
     if (nbat > 0) {
+        // batches it and it + 1 are in flight on entry; each buffer is
+        // refilled right after it is used, and every branch either leaves
+        // the loop or holds no load
         for (int it = 0;; it += 2) {
-            if (it + 1 >= nbat) {
-                comp(va, it);
-                break;
-            }
-            load(vb, it + 1);
-            comp(va, it);
             if (it + 2 >= nbat) {
-                comp(vb, it + 1);
+                comp(va, it);
+                if (it + 1 < nbat) comp(vb, it + 1);
                 break;
             }
+            comp(va, it);
             load(va, it + 2);
+            if (it + 3 >= nbat) {
+                comp(vb, it + 1);
+                comp(va, it + 2);
+                break;
+            }
             comp(vb, it + 1);
+            load(vb, it + 3);
         }
     }
     if (g > 0) {

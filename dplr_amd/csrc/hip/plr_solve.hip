@@ -632,11 +632,14 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
     if (ctot > 0) {
         LevelTab t0;
         plr_level_tab0(p, &t0);
-        int rc;
+        int rc = 1;
         if (plr_mm_ok(p) && !p->force_valu)
+            rc = plr_mm_expand_lvl(p, -1, nullptr, nullptr, 0, f.Dinv, p->U, N, 1.0, 0.0, f.Ut, N,
+                                   ctot, st);
+        if (rc == 1 && plr_mm_ok(p) && !p->force_valu)
             rc = plr_mm_expand1(p, -1, nullptr, nullptr, 0, f.Dinv, 0, p->U, N, 1.0, 0.0, f.Ut, N,
                                 ctot, st);
-        else
+        else if (rc == 1)
             rc = plr_launch_expand(t0, f.Dinv, p->U, N, 1.0, 0.0, 0, nullptr, 0, f.Ut, N, ctot, st);
         if (rc) return rc;
     }
@@ -647,7 +650,11 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         if (R == 0) continue;
         const int off = (int)col_off(p, d);
         const int ncol = off + R;
-        int rc = solve_ws(p, ncol);
+        // the ancestor update through the third-form expansion takes the
+        // capacitance output rank-major over whole 64-column groups
+        const bool fx = plr_mm_ok(p) && !p->force_valu && N % 256 == 0;
+        const int wtc = (off + 63) / 64 * 64;
+        int rc = solve_ws(p, fx && wtc > ncol ? wtc : ncol);
         if (rc) return rc;
         const double *Ud = f.Ut + p->uoff[d], *Vd = p->V + p->uoff[d], *Sd = p->S + p->soff[d];
         LevelTab t1;
@@ -666,7 +673,16 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         if ((rc = batched_inverse(work, n2, nodes, f.Kinv + f.koff[d], piv,
                                   f.dinfo + nleaf + (nodes - 1), st)))
             return rc;
-        if (off > 0) {
+        if (off > 0 && fx) {
+            // W rank-major: row s*R + k, column c at W[(s*R + k) * wtc + c]
+            if ((rc = plr_cap_launch<double, double, false>(f.Kinv + f.koff[d], d, R, Sd, p->sw.T,
+                                                            p->sw.t_per_col, p->sw.W, 1, 0, off,
+                                                            nullptr, 0, st, (size_t)wtc)))
+                return rc;
+            if ((rc = plr_mm_expand_lvl(p, d, Ud, p->sw.W, wtc, nullptr, nullptr, 0, 1.0, 1.0, f.Ut,
+                                        N, off, st)))
+                return rc < 0 ? rc : -3;
+        } else if (off > 0) {
             if ((rc = launch_cap(f.Kinv + f.koff[d], d, R, Sd, p->sw.T, p->sw.t_per_col, p->sw.W, off,
                                  0, st)))
                 return rc;

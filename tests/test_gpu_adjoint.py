@@ -142,12 +142,22 @@ def test_quadratic_form_gradient(cuda):
     T.free()
 
 
-def _loss(T, b, w, cuda):
+def _loss(T, b, w, cuda, refine=False):
+    """w . A^{-1} b in FP64; with refine, one step of iterative refinement
+    (residual through the FP64 product), which takes the solve's rounding
+    error, and so the finite differences' noise, down by the factor of one
+    more solve."""
     import torch
     P = D.Plan(T)
     assert P.factor() == 0
     y = torch.zeros_like(b)
     P.solve("N", b, y)
+    if refine:
+        r = b.clone()
+        P.gepdmm("N", -1.0, y, 1.0, r)
+        dy = torch.zeros_like(b)
+        P.solve("N", r, dy)
+        y += dy
     val = float((w * y).sum())
     P.close()
     return val
@@ -264,7 +274,9 @@ def test_gaussian_kernel_gradient_workload_size(cuda, orc):
     back = D.gepdmm("N", "N", T, yh.reshape(m, 1).copy(), 1, api=orc).ravel()
     # condition number ~2e5 (row sums ~2e3, nugget 1e-2); the factored solve's
     # residual measured 1.7e-9
-    assert rel(back, b.cpu().numpy().ravel()) < 2e-8
+    res = rel(back, b.cpu().numpy().ravel())
+    print(f"gradient workload: FP64 solve relative residual {res:.2e}", flush=True)
+    assert res < 2e-8
     assert rel(g64.cpu().numpy(), 2 * yh) < 1e-9
     got, got_leaves = gpu_grads(T, P, f64)
     blocks = {(d, qq): nd for (d, qq, nd) in block_index(T)}
@@ -287,15 +299,16 @@ def test_gaussian_kernel_gradient_workload_size(cuda, orc):
                 arr, idx, analytic = lr.sig, k, got[key][2][k]
         x0 = arr[idx]
         arr[idx] = x0 + h
-        fp = _loss(T, b, w, cuda)
+        fp = _loss(T, b, w, cuda, refine=True)
         arr[idx] = x0 - h
-        fm = _loss(T, b, w, cuda)
+        fm = _loss(T, b, w, cuda, refine=True)
         arr[idx] = x0
         fd = (fp - fm) / (2 * h)
         print("fd", kind, key, fd, analytic, flush=True)
-        # the difference quotient's error here: rounding of q (~1e-16 q / h) and
-        # the h^2 term at a condition number of ~2e5 (a deep level's sig is
-        # ~1e-13, far below h); measured agreement 6e-7 .. 4e-5
+        # the difference quotient's error here: rounding of q over 2h and the
+        # h^2 term at a condition number of ~2e5 (a deep level's sig is ~1e-13,
+        # far below h); each loss is refined once, as a single factored solve's
+        # rounding of q (relative residual ~1e-9) alone moves the quotient by ~1
         assert abs(fd - analytic) <= 1e-4 * abs(analytic) + 1e-3, (kind, key)
     q32, y32, g32, f32 = out[torch.float32]
     assert abs(q32 - q64) / abs(q64) < 1e-5
