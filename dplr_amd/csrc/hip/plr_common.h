@@ -208,6 +208,8 @@ int plr_up_ready(const dplrh_plan *p, int d0);
 int plr_up_reserve(dplrh_plan *p, int d0, int ncols);
 int plr_up_steps(dplrh_plan *p, int d0, int ncols, hipStream_t st);
 int plr_tail_d0(const dplrh_plan *p);
+int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, double *C,
+                    size_t cap, int *w_off, hipStream_t st);
 int plr_mm_expand_lvl(dplrh_plan *p, int d, const double *Q, const double *W, size_t ws,
                       const double *D, const double *X, int ldx, double alpha, double beta,
                       double *Y, int ldy, int ncols, hipStream_t st);

@@ -131,7 +131,6 @@ __global__ __launch_bounds__(256) void k_mm_expand(MmTab a, const double *__rest
 // host side
 
 
-// MFMA path needs every segment to be a whole number of 64-row blocks.
 // Leaf orders 64 * 2^k: every segment is 64 rows or a multiple of 128 (and
 // of 256 from 256 on), which the row groups of the MFMA kernels assume.
 int plr_mm_ok(const dplrh_plan *p)
@@ -214,6 +213,79 @@ int plr_mm_cross(dplrh_plan *p, int e, const double *X, int ldx, int ncols, doub
     if (fin > 0) {
         hipLaunchKernelGGL(k_mm_finish<false>, dim3(fin * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1,
                            p->tmp, C, (size_t)woff);
+        PLR_HIP(hipGetLastError());
+    }
+    return 0;
+}
+
+// As plr_mm_cross through the all-level reduction of the product (one pass
+// over X per 256-row group, every table entry from the staged rows), with the
+// output rank-major: C[(w_off[d] + s * R_d + k) * 64 + c] for c < ncols <= 64.
+// cap: doubles C holds.  Returns 1, launching nothing, when the shape or X
+// does not suit that kernel, or C is too small.
+int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, double *C,
+                    size_t cap, int *w_off, hipStream_t st)
+{
+    if (e <= 0 || ncols <= 0 || ncols > MM_CG) {
+        plr_set_error("plr_mm_cross_rm: level %d, %d columns", e, ncols);
+        return -5;
+    }
+    const bool leaves_ok = p->b >= 64 && (p->b & (p->b - 1)) == 0;
+    if (p->N % MMF_RB != 0 || !leaves_ok || ldx % 2 != 0 || ((uintptr_t)X & 15) != 0 ||
+        p->force_valu)
+        return 1;
+    MmTab t;
+    memset(&t, 0, sizeof(t));
+    const int N = p->N, seg = N >> (e + 1), nseg = 2 << e;
+    const int T = seg < MM_TMAX ? seg : MM_TMAX, tps = seg / T;
+    t.L = e;
+    t.N = N;
+    t.b = p->b;
+    t.ncols = ncols;
+    t.ncg = 1;
+    t.raw = 1;
+    int task = 0, part = 0, fin = 0, woff = 0;
+    for (int d = 0; d < e; ++d) {
+        const int R = p->R[d], nkg = (R + MM_KG - 1) / MM_KG;
+        t.R[d] = R;
+        t.seglen[d] = seg;
+        t.T[d] = T;
+        t.tps[d] = tps;
+        t.nkg[d] = nkg;
+        t.task_off[d] = task;
+        task += nkg * nseg * tps;
+        t.part_off[d] = part;
+        if (tps > 1) part += nkg * nseg * tps;
+        t.fin_off[d] = fin;
+        if (tps > 1) fin += nkg * nseg;
+        t.w_off[d] = woff;
+        w_off[d] = woff;
+        woff += nseg * R;
+        t.P[d] = p->V + p->uoff[d];
+    }
+    t.task_off[e] = task;
+    t.part_off[e] = part;
+    t.fin_off[e] = fin;
+    t.w_off[e] = woff;
+    w_off[e] = woff;
+    if ((size_t)woff * MM_CG > cap) return 1;
+    int rc = mm_part(p, (size_t)part);
+    if (rc) return rc;
+    if (task > 0) {
+        static bool attr = false;
+        if (!attr) {
+            PLR_HIP(hipFuncSetAttribute((const void *)k_mm_reduce_fused2,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            attr = true;
+        }
+        hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(N / MMF_RB, 1), dim3(64 * MMF2_NW),
+                           (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, 1.0, p->tmp,
+                           C, (size_t)MM_CG);
+        PLR_HIP(hipGetLastError());
+    }
+    if (fin > 0) {
+        hipLaunchKernelGGL(k_mm_finish<true>, dim3(fin * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1,
+                           p->tmp, C, (size_t)MM_CG);
         PLR_HIP(hipGetLastError());
     }
     return 0;

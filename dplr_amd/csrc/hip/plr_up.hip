@@ -229,6 +229,29 @@ __global__ void k_up_greshape(const double *__restrict__ C, size_t wstride, int 
     G[e] = C[(size_t)j * wstride + w_off + s * Rd + k];
 }
 
+// All coarser levels of one level e at once, from the rank-major output of
+// plr_mm_cross_rm: G_{d,e}[s][k + j R_d] = C[(w_off[d] + s R_d + k) * 64 + j].
+struct UpGTab
+{
+    int e, Re, nseg;
+    int Rd[PLR_MAXL], w_off[PLR_MAXL];
+    size_t first[PLR_MAXL + 1]; // first element of level d in this launch
+    double *G[PLR_MAXL];
+};
+
+__global__ void k_up_greshape_rm(const double *__restrict__ C, UpGTab t)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= t.first[t.e]) return;
+    int d = 0;
+    while (i >= t.first[d + 1]) ++d;
+    const int Rd = t.Rd[d];
+    const size_t r = i - t.first[d], blk = (size_t)Rd * t.Re;
+    const size_t s = r / blk, q = r - s * blk;
+    const int j = (int)(q / Rd), k = (int)(q - (size_t)j * Rd);
+    t.G[d][r] = C[((size_t)t.w_off[d] + s * Rd + k) * 64 + j];
+}
+
 void plr_up_release(dplrh_plan *p) { (void)p; }
 
 // After the factorization: the G blocks.  For each level e one reduction of
@@ -254,7 +277,29 @@ int plr_up_factor(dplrh_plan *p, hipStream_t st)
         const int Re = p->R[e], nseg = 2 << e;
         int w_off[PLR_MAXL + 1];
         size_t ws = 0;
-        int rc = plr_mm_cross(p, e, f.Ut + p->uoff[e], N, Re, C, w_off, &ws, st);
+        int rc = plr_mm_cross_rm(p, e, f.Ut + p->uoff[e], N, Re, C, (size_t)N * p->b, w_off, st);
+        if (rc < 0) return rc;
+        if (rc == 0) {
+            UpGTab g;
+            memset(&g, 0, sizeof(g));
+            g.e = e;
+            g.Re = Re;
+            g.nseg = nseg;
+            size_t n = 0;
+            for (int d = 0; d < e; ++d) {
+                g.Rd[d] = p->R[d];
+                g.w_off[d] = w_off[d];
+                g.first[d] = n;
+                g.G[d] = f.G + f.goff[d][e];
+                n += (size_t)nseg * p->R[d] * Re;
+            }
+            g.first[e] = n;
+            hipLaunchKernelGGL(k_up_greshape_rm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
+                               C, g);
+            PLR_HIP(hipGetLastError());
+            continue;
+        }
+        rc = plr_mm_cross(p, e, f.Ut + p->uoff[e], N, Re, C, w_off, &ws, st);
         if (rc) return rc;
         for (int d = 0; d < e; ++d) {
             const int Rd = p->R[d];
