@@ -235,6 +235,50 @@ __device__ __forceinline__ double gj_rdlane(double x, int l)
     return __hiloint2double(hi, lo);
 }
 
+// Pivot of step k: the position (uniform) of the largest |a| over the lanes
+// whose position is >= k, ties to the smaller position, NaN below every
+// number.  The order is that of one 64-bit key per lane (0: position < k,
+// 1: NaN, else the bits of |a| plus 2, which order as the values do); the
+// lanes of each 16-lane row meet through DPP moves (quad exchanges, then
+// rotations by 4 and 8), the four row winners through lane reads.
+template <int CTRL>
+__device__ __forceinline__ void gj_amax_dpp(unsigned &hi, unsigned &lo, int &p)
+{
+    const unsigned h2 = (unsigned)__builtin_amdgcn_update_dpp(0, (int)hi, CTRL, 0xF, 0xF, false);
+    const unsigned l2 = (unsigned)__builtin_amdgcn_update_dpp(0, (int)lo, CTRL, 0xF, 0xF, false);
+    const int p2 = __builtin_amdgcn_update_dpp(0, p, CTRL, 0xF, 0xF, false);
+    const bool take = h2 > hi || (h2 == hi && (l2 > lo || (l2 == lo && p2 < p)));
+    hi = take ? h2 : hi;
+    lo = take ? l2 : lo;
+    p = take ? p2 : p;
+}
+
+__device__ __forceinline__ int gj_pivot(double a, int pos, int k)
+{
+    unsigned long long key = 0ull;
+    if (pos >= k) key = a != a ? 1ull : (unsigned long long)__double_as_longlong(fabs(a)) + 2ull;
+    unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
+    int p = pos;
+    gj_amax_dpp<0xB1>(hi, lo, p);  // quad_perm 1,0,3,2
+    gj_amax_dpp<0x4E>(hi, lo, p);  // quad_perm 2,3,0,1
+    gj_amax_dpp<0x124>(hi, lo, p); // row_ror 4
+    gj_amax_dpp<0x128>(hi, lo, p); // row_ror 8
+    unsigned bh = __builtin_amdgcn_readlane(hi, 0), bl = __builtin_amdgcn_readlane(lo, 0);
+    int bp = __builtin_amdgcn_readlane(p, 0);
+#pragma unroll
+    for (int r = 1; r < 4; ++r) {
+        const unsigned h = __builtin_amdgcn_readlane(hi, 16 * r);
+        const unsigned l = __builtin_amdgcn_readlane(lo, 16 * r);
+        const int q = __builtin_amdgcn_readlane(p, 16 * r);
+        if (h > bh || (h == bh && (l > bl || (l == bl && q < bp)))) {
+            bh = h;
+            bl = l;
+            bp = q;
+        }
+    }
+    return bp;
+}
+
 template <int NM>
 __global__ __launch_bounds__(256) void k_gj_inverse_w(const double *A, int n, size_t stride,
                                                       double *Inv, size_t istride, int *info,
@@ -256,18 +300,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse_w(const double *A, int n, si
             const int k = k0 + c;
             // pivot: largest |a[c]| over positions >= k, ties to the smaller
             // position (NaN ranks below every number: one order for all lanes)
-            double v = pos >= k ? fabs(a[c]) : -1.0;
-            if (v != v) v = -0.5;
-            int key = pos;
-#pragma unroll
-            for (int o = 1; o < 64; o <<= 1) {
-                const double v2 = __shfl_xor(v, o, 64);
-                const int k2 = __shfl_xor(key, o, 64);
-                if (v2 > v || (v2 == v && k2 < key)) {
-                    v = v2;
-                    key = k2;
-                }
-            }
+            const int key = gj_pivot(a[c], pos, k);
             const bool piv = pos == key;
             const int pl = __ffsll((unsigned long long)__ballot(piv)) - 1;
             if (pos == k)
@@ -339,18 +372,7 @@ __global__ __launch_bounds__(NM * 4) void k_gj_inverse_q(const double *A, int n,
             if (wv == kb) {
                 // largest |a[c]| over positions >= k, ties to the smaller
                 // position (NaN ranks below every number)
-                double v = pos >= k ? fabs(a[c]) : -1.0;
-                if (v != v) v = -0.5;
-                int key = pos;
-#pragma unroll
-                for (int o = 1; o < 64; o <<= 1) {
-                    const double v2 = __shfl_xor(v, o, 64);
-                    const int k2 = __shfl_xor(key, o, 64);
-                    if (v2 > v || (v2 == v && k2 < key)) {
-                        v = v2;
-                        key = k2;
-                    }
-                }
+                const int key = gj_pivot(a[c], pos, k);
                 const int pl = __ffsll((unsigned long long)__ballot(pos == key)) - 1;
                 colv[buf][lane] = a[c];
                 if (lane == 0) {
