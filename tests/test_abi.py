@@ -25,9 +25,34 @@ def test_exports_every_declared_symbol():
     out = subprocess.run(["nm", "-D", "--defined-only", D.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
     exported = {line.split()[-1] for line in out.splitlines() if " T " in line}
-    for header in ("dplr.h", "dplrx.h"):
+    for header in ("dplr.h", "dplrx.h", "dplr_internal.h"):
         missing = declared(header) - exported
         assert not missing, f"{header}: not exported: {sorted(missing)}"
+
+
+def test_internal_header_forwards(tmp_path):
+    """Code written against the reference's private header builds and links
+    unchanged: dplr_internal.h brings the node types and the private
+    lifecycle routines (reference include/dplr_internal.h:99-115)."""
+    src = tmp_path / "uses_internal.c"
+    src.write_text(
+        "#include <dplr_internal.h>\n"
+        "int main(void) {\n"
+        "    DPlrMtx a, b, c;\n"
+        "    dplrinid(8, &a);\n"
+        "    dplrinilr(8, &b, 2);\n"
+        "    dplrresizelr(&b, 5);\n"
+        "    dplrinih(8, &c);\n"
+        "    int ok = a.type == PLR_DMTX && b.type == PLR_LRMTX && b.dlr.capacity >= 5 &&\n"
+        "             c.type == PLR_HMTX;\n"
+        "    dplrclr(&a); dplrclr(&b); dplrclr(&c);\n"
+        "    return ok ? 0 : 1;\n"
+        "}\n")
+    exe = tmp_path / "uses_internal"
+    libdir = os.path.dirname(D.LIB_PATH)
+    subprocess.run(["gcc", "-std=c99", "-I", os.path.join(ROOT, "include"), str(src), "-o", str(exe),
+                    "-L", libdir, "-ldplr_hip", f"-Wl,-rpath,{libdir}"], check=True)
+    assert subprocess.run([str(exe)]).returncode == 0
 
 
 def test_node_layout():
