@@ -575,7 +575,9 @@ def main():
     mv = lambda i: plans[i % len(plans)].gepmv("N", 1.0, x1, 0.0, y1)
     for i in range(6):
         mv(i)
-    ms_mv = event_ms(torch, mv, 60)
+    # median of three runs of 60 calls (the first run after the factorizations
+    # pays for the clocks coming back up)
+    ms_mv = float(np.median([event_ms(torch, mv, 60) for _ in range(3)]))
     mv_bytes = abytes + 2 * 8 * m
     mv_gbs = mv_bytes / (ms_mv * 1e-3) / 1e9
     log(f"matvec 1 RHS: {ms_mv * 1e3:.1f} us  {mv_gbs:.0f} GB/s ({mv_gbs / HBM_PEAK_GBS:.1%} of HBM)")
@@ -654,6 +656,30 @@ def main():
     log(f"step (apply + mixed solve): {ms_step * 1e3:.0f} us -> {rhs_s:.0f} RHS/s; "
         f"plain fp64 {plain_ms * 1e3:.0f} us -> {plain_rhs_s:.0f} RHS/s")
 
+    # --- the drop-in entry points on a host tree -----------------------------
+    # dplrgepmv / dplrgepdmm take the reference's host tree: every call
+    # flattens it, uploads the level arrays, runs the kernels and copies the
+    # result back (no plan survives the call)
+    tree0 = D.PlrTree.random(m, args.nlvl, args.rank, 4.0, 1)
+    xh = np.random.default_rng(2).standard_normal(m)
+    Bh = np.random.default_rng(3).standard_normal((m, nrhs))
+    D.gepmv("N", tree0, xh)
+    dropin = {}
+    for name, fn in (("gepmv_ms", lambda: D.gepmv("N", tree0, xh)),
+                     ("gepdmm_ms", lambda: D.gepdmm("N", "N", tree0, Bh, nrhs))):
+        ts = []
+        for _ in range(3):
+            t1 = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t1) * 1e3)
+        dropin[name] = float(np.median(ts))
+    tree0.free()
+    dropin["note"] = (f"host tree in, host result out, median of 3 calls at N={m}: flatten, "
+                      f"upload of {abytes / 1e6:.0f} MB, kernels, download; the device-resident "
+                      "plan (dplrx_*) is what the other figures time")
+    log(f"drop-in calls on the host tree: dplrgepmv {dropin['gepmv_ms']:.1f} ms, "
+        f"dplrgepdmm ({nrhs} RHS) {dropin['gepdmm_ms']:.1f} ms")
+
     batch = None
     if args.batch_count > 0:
         for P in plans:
@@ -693,6 +719,7 @@ def main():
         "config": {"workload": f"apply + solve, {nrhs} RHS, N={m}, L={args.nlvl}, leaf "
                                f"{m >> args.nlvl}, rank {args.rank}, {args.nmat} matrices rotated",
                    "global_batch": nrhs},
+        "dropin": dropin,
         "plain_fp64": {"rhs_s": plain_rhs_s, "ms_per_step": plain_ms,
                        "apply_ms": ms_apply, "solve_ms": ms_solve64},
         "mixed": {"rhs_s": rhs_s, "solve_ms": ms_mixed, "refine_solves": max(iters),
