@@ -2136,8 +2136,17 @@ __global__ __launch_bounds__(256) void k_lp_up_t0(LpUpTab a, const float *__rest
     const int per = a.nspan >> (d + 1); // spans per segment
     const float *p = up + a.upoff[d] + ((size_t)s * per * Rp + k) * n + c;
     const size_t step = (size_t)Rp * n;
+    // up to 32 loads in flight per thread (the top level sums 64 spans:
+    // two memory waits instead of sixteen)
     float acc[4] = {0.f, 0.f, 0.f, 0.f};
     int j = 0;
+    for (; j + 32 <= per; j += 32) {
+        float v[32];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) v[u] = p[(size_t)(j + u) * step];
+#pragma unroll
+        for (int u = 0; u < 32; ++u) acc[u & 3] += v[u];
+    }
     for (; j + 4 <= per; j += 4) {
         float v[4];
 #pragma unroll
