@@ -482,17 +482,34 @@ static int batched_inverse(double *A, int n, int count, double *Inv, int *piv, i
 __global__ void k_capf_build(const double *T, size_t tstride, int off, int R,
                              const double *S, double *K)
 {
-    const int node = blockIdx.x, n2 = 2 * R;
+    const int node = blockIdx.x, n2 = 2 * R, ne = n2 * n2;
     double *k = K + (size_t)node * n2 * n2;
     const int top = 2 * node, bot = 2 * node + 1;
-    for (int e = threadIdx.x; e < n2 * n2; e += blockDim.x) {
-        const int i = e % n2, j = e / n2;
-        double v = (i == j) ? 1.0 : 0.0;
-        if (i < R && j >= R)
-            v += T[(size_t)(off + j - R) * tstride + (size_t)bot * R + i] * S[(size_t)bot * R + j - R];
-        else if (i >= R && j < R)
-            v += T[(size_t)(off + j) * tstride + (size_t)top * R + i - R] * S[(size_t)top * R + j];
-        k[e] = v;
+    // sixteen elements per thread and round, every load of a round issued
+    // before the first is used (one memory wait per round)
+    for (int e0 = 0; e0 < ne; e0 += 16 * (int)blockDim.x) {
+        double tv[16], sv[16];
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = min(e0 + u * (int)blockDim.x + (int)threadIdx.x, ne - 1);
+            const int i = e % n2, j = e / n2;
+            // the off-diagonal blocks read T and S; the diagonal ones a valid
+            // dummy (the first entries), multiplied by zero below
+            const bool ur = i < R && j >= R, ll = i >= R && j < R;
+            const size_t ti = ur ? (size_t)(off + j - R) * tstride + (size_t)bot * R + i
+                                 : ll ? (size_t)(off + j) * tstride + (size_t)top * R + i - R : 0;
+            const size_t si = ur ? (size_t)bot * R + j - R : ll ? (size_t)top * R + j : 0;
+            tv[u] = T[ti];
+            sv[u] = S[si];
+        }
+#pragma unroll
+        for (int u = 0; u < 16; ++u) {
+            const int e = e0 + u * (int)blockDim.x + (int)threadIdx.x;
+            if (e >= ne) continue;
+            const int i = e % n2, j = e / n2;
+            const bool od = (i < R) != (j < R);
+            k[e] = (i == j ? 1.0 : 0.0) + (od ? tv[u] * sv[u] : 0.0);
+        }
     }
 }
 
