@@ -47,6 +47,9 @@ def _load():
         "dplrx_solve_refined": (c_int, [P, c_int, P, c_int, P, c_int, c_int, PD, P]),
         "dplrx_solve_refined_tol": (c_int, [P, c_int, P, c_int, P, c_int, c_double, c_int,
                                             POINTER(c_int), PD, P]),
+        "dplrx_apply_solve_refined_tol": (c_int, [P, c_int, P, c_int, P, c_int, P, c_int, P,
+                                                  c_int, c_double, c_int, POINTER(c_int), PD,
+                                                  P]),
         "dplrx_grad_outer": (c_int, [P, c_int, c_double, P, c_int, P, c_int, P, P, P, P, P]),
         "dplrx_batch_create": (c_int, [c_int, POINTER(P), POINTER(P)]),
         "dplrx_batch_destroy": (None, [P]),

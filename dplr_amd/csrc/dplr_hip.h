@@ -123,6 +123,11 @@ int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, double *dX, 
  * *iters and *err (optional) receive the solves made and the estimate. */
 int dplrh_lp_solve_tol(dplrh_plan *p, int n, const double *dB, int ldb, double *dX, int ldx,
                        double tol, int maxit, int *iters, double *err, void *stream);
+/* The same with an apply Ya = A Xa of n columns sharing the first residual
+ * product's pass over the factors. */
+int dplrh_lp_apply_solve_tol(dplrh_plan *p, int n, const double *dXa, int ldxa, double *dYa,
+                             int ldya, const double *dB, int ldb, double *dX, int ldx, double tol,
+                             int maxit, int *iters, double *err, void *stream);
 
 /* Explicit inverse of a factored plan, downloaded per level d' into
  * hBU[d'], hBV[d'] (N*Rt[d'] doubles, [segment][column][row]) and hBsig[d']

@@ -210,6 +210,9 @@ int plr_up_steps(dplrh_plan *p, int d0, int ncols, hipStream_t st);
 int plr_tail_d0(const dplrh_plan *p);
 int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, double *C,
                     size_t cap, int *w_off, hipStream_t st);
+int plr_mm_apply_split(dplrh_plan *p, int nc0, const double *X0, int ldx0, double *Y0, int ldy0,
+                       int nc1, const double *X1, int ldx1, const double *C1, int ldc1, double *Y1,
+                       int ldy1, double *bmax, hipStream_t st);
 int plr_mm_expand_lvl(dplrh_plan *p, int d, const double *Q, const double *W, size_t ws,
                       const double *D, const double *X, int ldx, double alpha, double beta,
                       double *Y, int ldy, int ncols, hipStream_t st);

@@ -1219,7 +1219,8 @@ static int lp_estimate(plr_lp *q, int nc, int k0, int k1, double *dprev, double 
 // max|b - A x| of column c after the last solve (one more FP64 residual).
 static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int ldb, double *dX,
                          int ldx, int iters, double tol, int *used, double *err, double *res,
-                         hipStream_t st)
+                         hipStream_t st, const double *dXa = nullptr, int ldxa = 0,
+                         double *dYa = nullptr, int ldya = 0)
 {
     const int N = p->N;
     if (used) *used = 0;
@@ -1246,10 +1247,30 @@ static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int 
         int made = 0, unread = 1; // first step whose sums are not read yet
         double est = INFINITY, dprev = 1.0;
         const size_t per = (size_t)LP_NRM_COPIES * 2 * nc;
+        // an apply Ya = A Xa of the same columns rides on the first residual
+        // product (one pass over the factors for both) when that product is
+        // the fused one; otherwise it is made apart
+        const double *Xa = dXa ? dXa + (size_t)c0 * ldxa : nullptr;
+        double *Ya = dYa ? dYa + (size_t)c0 * ldya : nullptr;
+        bool applied = !dXa;
         for (int it = 0; it < iters; ++it) {
             const double *r, *pm;
             int ldr, pm_n;
-            if ((rc = lp_residual(p, q, nc, B, ldb, X, ldx, it == 0, direct, &r, &ldr, &pm, &pm_n, st)))
+            rc = 1;
+            if (!applied && it == 1 && direct && nc <= 64) {
+                rc = plr_mm_apply_split(p, nc, Xa, ldxa, Ya, ldya, nc, X, ldx, B, ldb, q->r, N,
+                                        q->bmax, st);
+                if (rc < 0) return rc;
+                if (rc == 0) {
+                    applied = true;
+                    r = q->r;
+                    ldr = N;
+                    pm = q->bmax + (size_t)64 * (N / 128);
+                    pm_n = N / 128;
+                }
+            }
+            if (rc == 1 &&
+                (rc = lp_residual(p, q, nc, B, ldb, X, ldx, it == 0, direct, &r, &ldr, &pm, &pm_n, st)))
                 return rc;
             const bool track = adaptive && it >= 1;
             rc = lp_correct(p, q, nc, it == 0, X, ldx, r, ldr, pm, pm_n,
@@ -1261,6 +1282,8 @@ static int lp_solve_body(dplrh_plan *p, plr_lp *q, int n, const double *dB, int 
             unread = it + 1;
             if (est <= tol) break;
         }
+        if (!applied && (rc = plr_gemm_direct(p, 0, nc, 1.0, 0.0, Xa, ldxa, Ya, ldya, st)))
+            return rc < 0 ? rc : -1;
         if (made > used_max) used_max = made;
         if (adaptive && (est > est_max || est != est)) est_max = est;
         if (res) {
@@ -1300,6 +1323,32 @@ extern "C" int dplrh_lp_solve(dplrh_plan *p, int n, const double *dB, int ldb, d
     }
     return lp_solve_body(p, q, n, dB, ldb, dX, ldx, iters, -1.0, nullptr, nullptr, res,
                          (hipStream_t)stream);
+}
+
+// The apply Ya = A Xa and the refined solve X = A^{-1} B of n columns in one
+// call (the mixed-precision step): the apply shares the factor reads of the
+// first residual product.
+extern "C" int dplrh_lp_apply_solve_tol(dplrh_plan *p, int n, const double *dXa, int ldxa,
+                                        double *dYa, int ldya, const double *dB, int ldb,
+                                        double *dX, int ldx, double tol, int maxit, int *iters,
+                                        double *err, void *stream)
+{
+    plr_lp *q = lp_of(p);
+    if (!q) {
+        plr_set_error("solve before a successful mixed-precision factorization");
+        return -4;
+    }
+    if (!(tol >= 0.0) || maxit < 2 || maxit > LP_NRM_SLOTS) {
+        plr_set_error("refined solve: tol must be >= 0 and 2 <= maxit <= %d (got %g, %d)",
+                      LP_NRM_SLOTS, tol, maxit);
+        return -5;
+    }
+    if (!dXa || !dYa || ldxa < p->N || ldya < p->N) {
+        plr_set_error("apply and solve: the apply's operands are missing or too short");
+        return -5;
+    }
+    return lp_solve_body(p, q, n, dB, ldb, dX, ldx, maxit, tol, iters, err, nullptr,
+                         (hipStream_t)stream, dXa, ldxa, dYa, ldya);
 }
 
 extern "C" int dplrh_lp_solve_tol(dplrh_plan *p, int n, const double *dB, int ldb, double *dX,

@@ -1,6 +1,7 @@
 // FP64 MFMA building blocks of the multi-RHS products (device bodies shared
 // by the single-plan kernels and the batched kernels), and their tables.
 #pragma once
+#include <cstdint>
 
 #include <cstring>
 
@@ -31,6 +32,27 @@ struct MmTab
     int nosig; // second-form expansion: no sig factor (W is complete)
     const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
 };
+
+// Two operand sets in one multi-RHS product (the mixed step's apply and its
+// first residual share the factor reads): column groups [0, ncg0) take the
+// kernel's own X, C, Y, alpha, beta and nc0 columns; the groups from ncg0 on
+// take the second set, its column c at X1 + (c - 64 ncg0) * ldx1, and nc1
+// columns.  ncg0 = 0: one operand set.
+struct MmSplit
+{
+    int ncg0, nc0, nc1;
+    const double *X1, *C1;
+    double *Y1;
+    int ldx1, ldc1, ldy1;
+    double alpha1, beta1;
+};
+
+// the base that puts column 64 * cg0 of a split product's second part at p
+template <typename T>
+__device__ __forceinline__ T *mm_split_base(T *p, int cg0, int ld)
+{
+    return (T *)((uintptr_t)p - (uintptr_t)cg0 * MM_CG * (uintptr_t)ld * sizeof(double));
+}
 
 // ---------------------------------------------------------------------------
 
@@ -381,7 +403,7 @@ template <int NT>
 __device__ __forceinline__ void mm_fused2_pass(const MmTab &a, int rb0, int cg, int t0, int wv,
                                                double alpha, double *__restrict__ part,
                                                double *__restrict__ w, size_t wstride,
-                                               const double *xs)
+                                               const double *xs, int NC)
 {
     const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
     const int c0 = cg * MM_CG, N = a.N;
@@ -438,7 +460,7 @@ __device__ __forceinline__ void mm_fused2_pass(const MmTab &a, int rb0, int cg, 
 #pragma unroll
             for (int ct = 0; ct < 4; ++ct) {
                 const int c = c0 + ct * 16 + m;
-                if (c >= a.ncols) continue;
+                if (c >= NC) continue;
                 double *wc = w + ((size_t)a.w_off[d] + (size_t)sp * R) * wstride + c;
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
@@ -494,12 +516,13 @@ __device__ __forceinline__ void mm_reduce_fused2_body(const MmTab &a, int rbk, i
                                                       const double *__restrict__ X, int ldx,
                                                       double alpha, double *__restrict__ part,
                                                       double *__restrict__ w, size_t wstride,
-                                                      double *xs)
+                                                      double *xs, int NC = -1)
 {
     // wave-uniform (scalar) so that the per-tile table reads are scalar
     // loads: a vector load there would wait for the P loads in flight
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rb0 = rbk * MMF_RB, c0 = cg * MM_CG;
+    if (NC < 0) NC = a.ncols; // columns (a split product gives each part its own)
     int ntile = 0;
     for (int d = 0; d < a.L; ++d) ntile += (a.R[d] + 15) / 16;
     // X of all 256 rows, row-major in LDS (xs[row * 72 + col]: the B reads
@@ -513,13 +536,13 @@ __device__ __forceinline__ void mm_reduce_fused2_body(const MmTab &a, int rbk, i
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * 64 * MMF2_NW + (int)threadIdx.x;
             const int pr = 4 * (e >> 8) + (e & 3), c = 16 * ((e >> 6) & 3) + ((e >> 2) & 15);
-            xr[u] = *(const double2 *)(X + (size_t)min(c0 + c, a.ncols - 1) * ldx + rb0 + 2 * pr);
+            xr[u] = *(const double2 *)(X + (size_t)min(c0 + c, NC - 1) * ldx + rb0 + 2 * pr);
         }
 #pragma unroll
         for (int u = 0; u < 8; ++u) {
             const int e = e0 + u * 64 * MMF2_NW + (int)threadIdx.x;
             const int pr = 4 * (e >> 8) + (e & 3), c = 16 * ((e >> 6) & 3) + ((e >> 2) & 15);
-            const double on = c0 + c < a.ncols ? 1.0 : 0.0;
+            const double on = c0 + c < NC ? 1.0 : 0.0;
             xs[(2 * pr) * MMF2_XS + c] = xr[u].x * on;
             xs[(2 * pr + 1) * MMF2_XS + c] = xr[u].y * on;
         }
@@ -529,11 +552,11 @@ __device__ __forceinline__ void mm_reduce_fused2_body(const MmTab &a, int rbk, i
         const int rest = ntile - t0 - wv; // tiles t0 + wv + 8j exist for 8j < rest
         const int nt = rest <= 0 ? 0 : min(MMF2_TPW, (rest + MMF2_NW - 1) / MMF2_NW);
         if (nt == 3)
-            mm_fused2_pass<3>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs);
+            mm_fused2_pass<3>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs, NC);
         else if (nt == 2)
-            mm_fused2_pass<2>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs);
+            mm_fused2_pass<2>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs, NC);
         else if (nt == 1)
-            mm_fused2_pass<1>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs);
+            mm_fused2_pass<1>(a, rb0, cg, t0, wv, alpha, part, w, wstride, xs, NC);
     }
 }
 
@@ -546,7 +569,8 @@ static constexpr int MM_FCH = MM_BLK / 64; // chunks per block
 template <bool WT = false>
 __device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double alpha, int trans,
                                                int raw, const double *__restrict__ part,
-                                               double *__restrict__ w, size_t wstride)
+                                               double *__restrict__ w, size_t wstride,
+                                               int ncg0 = 0, double alpha1 = 0.0)
 {
     __shared__ double red[4][64];
     const int task = bx / MM_FCH, chunk = bx % MM_FCH;
@@ -555,6 +579,7 @@ __device__ __forceinline__ void mm_finish_body(const MmTab &a, int bx, double al
     const int loc = task - a.fin_off[d];
     const int nkg = a.nkg[d], ncg = a.ncg, tps = a.tps[d], R = a.R[d];
     const int kg = loc % nkg, cg = (loc / nkg) % ncg, s = loc / (nkg * ncg);
+    if (ncg0 > 0 && cg >= ncg0) alpha = alpha1; // the second part of a split product
     const int el = threadIdx.x & 63, ph = threadIdx.x >> 6;
     const int e = chunk * 64 + el;
     const double *pp = part + ((size_t)a.part_off[d] + ((size_t)s * tps * nkg + kg) * ncg + cg) * MM_BLK + e;
@@ -952,14 +977,15 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
                                                 double alpha, double beta,
                                                 const double *__restrict__ w, int wtc,
                                                 const double *C, int ldc, double *Y, int ldy,
-                                                double *__restrict__ bmax, double *lds)
+                                                double *__restrict__ bmax, double *lds, int NC = -1)
 {
     __shared__ double cmx[ME3_NW * MM_CG];
     const int lane = threadIdx.x & 63, m = lane & 15, g = lane >> 4;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int rb0 = bx * ME3_RB, r0 = rb0 + wv * 32;
     const int c0 = by * MM_CG;
-    const int N = a.N, b = a.b, nc = min(MM_CG, a.ncols - c0);
+    if (NC < 0) NC = a.ncols; // columns (a split product gives each part its own)
+    const int N = a.N, b = a.b, nc = min(MM_CG, NC - c0);
 
     // shared levels [0, dsh) and their staged rows
     int dsh = 0, rows = 0;
@@ -1203,7 +1229,7 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
             for (int ct = 0; ct < 4; ++ct)
 #pragma unroll
                 for (int v = 0; v < 4; ++v) {
-                    const int c = min(c0 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1), a.ncols - 1);
+                    const int c = min(c0 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1), NC - 1);
                     yo[rt][ct][v] = C[(size_t)c * ldc + r0 + 2 * m + rt];
                 }
     }
@@ -1215,7 +1241,7 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
             for (int v = 0; v < 4; ++v) {
                 const int c = c0 + 32 * (ct >> 1) + 2 * (g + 4 * v) + (ct & 1);
                 acc[rt][ct][v] += beta == 0.0 ? 0.0 : beta * yo[rt][ct][v];
-                if (c < a.ncols) Y[(size_t)c * ldy + r0 + 2 * m + rt] = acc[rt][ct][v];
+                if (c < NC) Y[(size_t)c * ldy + r0 + 2 * m + rt] = acc[rt][ct][v];
             }
     if (bmax) {
         // column maxima per 128-row block: the two row tiles, the 16 lanes of
@@ -1239,7 +1265,7 @@ __device__ __forceinline__ void mm_expand3_body(const MmTab &a, int bx, int by,
         __syncthreads();
         if (threadIdx.x < 2 * MM_CG) {
             const int c = threadIdx.x & 63, h = threadIdx.x >> 6;
-            if (c0 + c < a.ncols) {
+            if (c0 + c < NC) {
                 const double *q = cmx + h * 4 * 64 + c;
                 const double x = fmax(fmax(q[0], q[64]), fmax(q[128], q[192]));
                 bmax[(size_t)(c0 + c) * (N / ME_RB) + 2 * bx + h] = x;

@@ -41,18 +41,32 @@ __global__ __launch_bounds__(256) void k_mm_reduce_fused(MmTab a, const double *
 __global__ __launch_bounds__(64 * MMF2_NW) void k_mm_reduce_fused2(MmTab a, const double *__restrict__ X,
                                                                 int ldx, double alpha,
                                                                 double *__restrict__ part,
-                                                                double *__restrict__ w, size_t wstride)
+                                                                double *__restrict__ w, size_t wstride,
+                                                                MmSplit sp)
 {
     extern __shared__ double xs_dyn[];
-    mm_reduce_fused2_body(a, blockIdx.x, blockIdx.y, X, ldx, alpha, part, w, wstride, xs_dyn);
+    int nc = -1;
+    const int rg = blockIdx.x, cg = blockIdx.y;
+    if (sp.ncg0 > 0) {
+        if (cg >= sp.ncg0) {
+            X = mm_split_base(sp.X1, sp.ncg0, sp.ldx1);
+            ldx = sp.ldx1;
+            alpha = sp.alpha1;
+            nc = sp.ncg0 * MM_CG + sp.nc1;
+        } else {
+            nc = sp.nc0;
+        }
+    }
+    mm_reduce_fused2_body(a, rg, cg, X, ldx, alpha, part, w, wstride, xs_dyn, nc);
 }
 
 template <bool WT>
 __global__ __launch_bounds__(256) void k_mm_finish(MmTab a, double alpha, int trans, int raw,
                                                    const double *__restrict__ part,
-                                                   double *__restrict__ w, size_t wstride)
+                                                   double *__restrict__ w, size_t wstride,
+                                                   MmSplit sp)
 {
-    mm_finish_body<WT>(a, blockIdx.x, alpha, trans, raw, part, w, wstride);
+    mm_finish_body<WT>(a, blockIdx.x, alpha, trans, raw, part, w, wstride, sp.ncg0, sp.alpha1);
 }
 
 template <int TR>
@@ -69,11 +83,29 @@ __global__ __launch_bounds__(64 * ME3_NW) void k_mm_expand3(MmTab a, const doubl
                                                         double alpha, double beta,
                                                         const double *__restrict__ w, int wtc,
                                                         const double *C, int ldc, double *Y,
-                                                        int ldy, double *__restrict__ bmax)
+                                                        int ldy, double *__restrict__ bmax,
+                                                        MmSplit sp)
 {
     extern __shared__ double lds_dyn[];
-    mm_expand3_body<TR>(a, blockIdx.x, blockIdx.y, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy,
-                        bmax, lds_dyn);
+    int nc = -1;
+    const int rg = blockIdx.x, cg = blockIdx.y;
+    if (sp.ncg0 > 0) {
+        if (cg >= sp.ncg0) {
+            X = mm_split_base(sp.X1, sp.ncg0, sp.ldx1);
+            ldx = sp.ldx1;
+            C = mm_split_base(sp.C1, sp.ncg0, sp.ldc1);
+            ldc = sp.ldc1;
+            Y = mm_split_base(sp.Y1, sp.ncg0, sp.ldy1);
+            ldy = sp.ldy1;
+            alpha = sp.alpha1;
+            beta = sp.beta1;
+            nc = sp.ncg0 * MM_CG + sp.nc1;
+        } else {
+            nc = sp.nc0;
+        }
+    }
+    mm_expand3_body<TR>(a, rg, cg, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy, bmax, lds_dyn,
+                        nc);
 }
 
 // The expansion over the table's levels (and the leaf when D is given):
@@ -84,7 +116,7 @@ template <int TR>
 static int mm_expand_launch(dplrh_plan *p, const MmTab &t, const double *D, const double *X,
                             int ldx, double alpha, double beta, const double *w, int wtc,
                             const double *C, int ldc, double *Y, int ldy, double *bmax,
-                            hipStream_t st)
+                            hipStream_t st, const MmSplit &sp = MmSplit{})
 {
     int rows = 0;
     for (int d = 0; d < t.L && t.seglen[d] >= ME3_RB; ++d) rows += t.R[d];
@@ -97,8 +129,12 @@ static int mm_expand_launch(dplrh_plan *p, const MmTab &t, const double *D, cons
             attr = true;
         }
         hipLaunchKernelGGL(k_mm_expand3<TR>, dim3(p->N / ME3_RB, t.ncg), dim3(64 * ME3_NW), lds, st,
-                           t, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy, bmax);
+                           t, D, X, ldx, alpha, beta, w, wtc, C, ldc, Y, ldy, bmax, sp);
     } else {
+        if (sp.ncg0 > 0) {
+            plr_set_error("split product needs the third-form expansion");
+            return -3;
+        }
         hipLaunchKernelGGL(k_mm_expand2<TR>, dim3(p->N / ME_RB, t.ncg), dim3(256), 0, st, t, D, X,
                            ldx, alpha, beta, w, wtc, C, ldc, Y, ldy, bmax);
     }
@@ -212,7 +248,7 @@ int plr_mm_cross(dplrh_plan *p, int e, const double *X, int ldx, int ncols, doub
     }
     if (fin > 0) {
         hipLaunchKernelGGL(k_mm_finish<false>, dim3(fin * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1,
-                           p->tmp, C, (size_t)woff);
+                           p->tmp, C, (size_t)woff, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     return 0;
@@ -280,12 +316,12 @@ int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, d
         }
         hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(N / MMF_RB, 1), dim3(64 * MMF2_NW),
                            (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, 1.0, p->tmp,
-                           C, (size_t)MM_CG);
+                           C, (size_t)MM_CG, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     if (fin > 0) {
         hipLaunchKernelGGL(k_mm_finish<true>, dim3(fin * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1,
-                           p->tmp, C, (size_t)MM_CG);
+                           p->tmp, C, (size_t)MM_CG, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     return 0;
@@ -341,12 +377,12 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
             }
             hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF2_NW),
                                (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, alpha, p->tmp,
-                               w, (size_t)wtc);
+                               w, (size_t)wtc, MmSplit{});
             PLR_HIP(hipGetLastError());
         }
         if (t.fin_off[t.L] > 0) {
             hipLaunchKernelGGL(k_mm_finish<true>, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t,
-                               alpha, trans, 0, p->tmp, w, (size_t)wtc);
+                               alpha, trans, 0, p->tmp, w, (size_t)wtc, MmSplit{});
             PLR_HIP(hipGetLastError());
         }
         return trans ? mm_expand_launch<1>(p, t, p->D, X, ldx, alpha, beta, w, wtc, C ? C : Y,
@@ -361,7 +397,7 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
     }
     if (t.fin_off[t.L] > 0) {
         hipLaunchKernelGGL(k_mm_finish<false>, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t, alpha,
-                           trans, 0, p->tmp, w, wstride);
+                           trans, 0, p->tmp, w, wstride, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     dim3 grid((p->N + 63) / 64, t.ncg);
@@ -369,6 +405,65 @@ int plr_mm_apply_c(dplrh_plan *p, int trans, int ncols, double alpha, double bet
                        wstride, Y, ldy);
     PLR_HIP(hipGetLastError());
     return 0;
+}
+
+// The mixed step's two products in one pass over the factors:
+// Y0 := A X0 (nc0 <= 64 columns) and Y1 := C1 - A X1 (nc1 <= 64 columns, the
+// refinement's residual), with the maxima of |Y1| per (column, 128-row
+// block) in bmax from column 64 on (bmax holds 128 columns).  Returns 1,
+// launching nothing, when the shape or the operands do not suit the fused
+// kernels (the caller then makes the two products apart).
+int plr_mm_apply_split(dplrh_plan *p, int nc0, const double *X0, int ldx0, double *Y0, int ldy0,
+                       int nc1, const double *X1, int ldx1, const double *C1, int ldc1, double *Y1,
+                       int ldy1, double *bmax, hipStream_t st)
+{
+    if (nc0 < 1 || nc0 > MM_CG || nc1 < 1 || nc1 > MM_CG) return 1;
+    const bool leaves_ok = p->L == 0 || (p->b >= 64 && (p->b & (p->b - 1)) == 0);
+    const bool al = ((((uintptr_t)X0 | (uintptr_t)X1) & 15) == 0) && ldx0 % 2 == 0 && ldx1 % 2 == 0;
+    if (p->N % MMF_RB != 0 || p->N % ME3_RB != 0 || !leaves_ok || !al || p->force_valu) return 1;
+    int rc = plr_workspace(p, 2 * MM_CG);
+    if (rc) return rc;
+    const double *P[PLR_MAXL], *Q[PLR_MAXL], *S[PLR_MAXL];
+    for (int d = 0; d < p->L; ++d) {
+        P[d] = p->V + p->uoff[d];
+        Q[d] = p->U + p->uoff[d];
+        S[d] = p->S + p->soff[d];
+    }
+    MmTab t;
+    mm_tab(p, 0, p->L, P, Q, S, MM_CG + nc1, 0, &t);
+    if ((rc = mm_part(p, (size_t)t.part_off[t.L]))) return rc;
+    const int wtc = t.ncg * MM_CG;
+    MmSplit sp{};
+    sp.ncg0 = 1;
+    sp.nc0 = nc0;
+    sp.nc1 = nc1;
+    sp.X1 = X1;
+    sp.C1 = C1;
+    sp.Y1 = Y1;
+    sp.ldx1 = ldx1;
+    sp.ldc1 = ldc1;
+    sp.ldy1 = ldy1;
+    sp.alpha1 = -1.0;
+    sp.beta1 = 1.0;
+    if (t.task_off[t.L] > 0) {
+        static bool attr2 = false;
+        if (!attr2) {
+            PLR_HIP(hipFuncSetAttribute((const void *)k_mm_reduce_fused2,
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 150 * 1024));
+            attr2 = true;
+        }
+        hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF2_NW),
+                           (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X0, ldx0, 1.0, p->tmp,
+                           p->w, (size_t)wtc, sp);
+        PLR_HIP(hipGetLastError());
+    }
+    if (t.fin_off[t.L] > 0) {
+        hipLaunchKernelGGL(k_mm_finish<true>, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t,
+                           1.0, 0, 0, p->tmp, p->w, (size_t)wtc, sp);
+        PLR_HIP(hipGetLastError());
+    }
+    return mm_expand_launch<0>(p, t, p->D, X0, ldx0, 1.0, 0.0, p->w, wtc, Y0, ldy0, Y0, ldy0, bmax,
+                               st, sp);
 }
 
 // Raw reduction T = P_d^T X per segment of level d (T layout: column c at
@@ -392,7 +487,7 @@ int plr_mm_reduce1(dplrh_plan *p, int d, const double *P, const double *X, int l
     }
     if (t.fin_off[1] > 0) {
         hipLaunchKernelGGL(k_mm_finish<false>, dim3(t.fin_off[1] * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1, p->tmp,
-                           T, tstride);
+                           T, tstride, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     return 0;
@@ -448,12 +543,12 @@ int plr_mm_reduce_raw(dplrh_plan *p, int d1, const double *X, int ldx, int ncols
         }
         hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(p->N / MMF_RB, t.ncg), dim3(64 * MMF2_NW),
                            (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, 1.0, p->tmp, T,
-                           ws);
+                           ws, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     if (t.fin_off[t.L] > 0) {
         hipLaunchKernelGGL(k_mm_finish<true>, dim3(t.fin_off[t.L] * MM_FCH), dim3(256), 0, st, t, 1.0,
-                           0, 1, p->tmp, T, ws);
+                           0, 1, p->tmp, T, ws, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     return 0;
@@ -517,7 +612,7 @@ int plr_mm_finish_raw(dplrh_plan *p, int d, int ncols, int tmax, double *T, size
     mm_tab(p, d, d + 1, &P, &Q, &S, ncols, 1, &t, tmax);
     if (t.fin_off[1] > 0) {
         hipLaunchKernelGGL(k_mm_finish<false>, dim3(t.fin_off[1] * MM_FCH), dim3(256), 0, st, t, 1.0,
-                           0, 1, p->tmp, T, tstride);
+                           0, 1, p->tmp, T, tstride, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     return 0;

@@ -237,6 +237,20 @@ int dplrx_solve_refined_tol(dplrx_plan *plan, int n, const double *dB, int ldb, 
     return dplrh_lp_solve_tol(plan, n, dB, ldb, dX, ldx, tol, maxit, iters, err, stream);
 }
 
+int dplrx_apply_solve_refined_tol(dplrx_plan *plan, int n, const double *dXa, int ldxa,
+                                  double *dYa, int ldya, const double *dB, int ldb, double *dX,
+                                  int ldx, double tol, int maxit, int *iters, double *err,
+                                  void *stream)
+{
+    int N, L, b, R[DPLRH_MAXL];
+    dplrh_plan_dims(plan, &N, &L, &b, R);
+    if (n < 0 || maxit < 2 || !(tol >= 0.0) || ldb < N || ldx < N || ldxa < N || ldya < N ||
+        (n > 0 && (dB == dX || dXa == dYa || dYa == dX || dYa == dB)))
+        return -1;
+    return dplrh_lp_apply_solve_tol(plan, n, dXa, ldxa, dYa, ldya, dB, ldb, dX, ldx, tol, maxit,
+                                    iters, err, stream);
+}
+
 int dplrx_factor_single(dplrx_plan *plan, int type, int *info, void *stream)
 {
     return dplrh_sp_factor(plan, type, 1, info, stream);

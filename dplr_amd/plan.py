@@ -189,6 +189,21 @@ class Plan:
         return it.value, err.value
 
 
+    def apply_solve_refined_tol(self, Xa, Ya, B, X, tol=1e-10, maxit=8, stream=None):
+        """One mixed-precision step: Ya := A Xa (FP64) and X := A^{-1} B
+        refined to tol, the apply sharing the first residual product.
+        Returns (solves made, error estimate); synchronises."""
+        n, m = B.shape
+        if Xa.shape != B.shape or Ya.shape != B.shape:
+            raise ValueError("Xa, Ya and B must have the same shape")
+        it, err = c_int(0), ctypes.c_double(0.0)
+        check(lib.dplrx_apply_solve_refined_tol(self.p, n, _dptr(Xa), _ld(Xa), _dptr(Ya), _ld(Ya),
+                                                _dptr(B), _ld(B), _dptr(X), _ld(X), tol, maxit,
+                                                byref(it), byref(err), _stream(stream)),
+              "dplrx_apply_solve_refined_tol")
+        return it.value, err.value
+
+
 class Batch:
     """Independent plans applied or solved together: one kernel launch per
     phase covers every matrix, replayed from a HIP graph (dplrx_batch_*).

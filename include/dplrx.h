@@ -117,6 +117,24 @@ int dplrx_solve_refined_tol(dplrx_plan *plan, int n, const double *dB, int ldb, 
                             int ldx, double tol, int maxit, int *iters, double *err,
                             void *stream);
 
+/* The mixed-precision apply and solve of one step in one call:
+ * Ya := A Xa (FP64) and X := A^{-1} B refined to tol as
+ * dplrx_solve_refined_tol (same results, estimate and arguments).  No
+ * counterpart in the reference, whose apply and inverse are separate calls
+ * (src/dplrpdmm.c:454, src/dplrinv.c:93).  The apply rides on the first
+ * residual product, r = B - A X_1, when that product takes the fused path
+ * (n <= 64 per chunk, B and Xa 16-byte aligned with even leading
+ * dimensions, leaf orders 64 * 2^k, N a multiple of 256): one pass over the
+ * factors and one launch of each kernel serve both (measured: a 763-756 us
+ * step against 768-771 us as two calls at N=65536, 64 columns; the
+ * 128-column product is paced by its FP64 MFMA work, so the shared reads
+ * save little).  Otherwise the apply is a separate FP64 product.  Ya must
+ * not alias Xa, B or X. */
+int dplrx_apply_solve_refined_tol(dplrx_plan *plan, int n, const double *dXa, int ldxa,
+                                  double *dYa, int ldya, const double *dB, int ldb, double *dX,
+                                  int ldx, double tol, int maxit, int *iters, double *err,
+                                  void *stream);
+
 /* ---- single precision --------------------------------------------------- */
 
 /* Single-precision copies of the factors for FP32 products and solves:

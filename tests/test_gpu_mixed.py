@@ -256,3 +256,47 @@ def test_f16_pairs_refine_in_two_solves(cuda, orc, m, nlvl, rank, n):
         r = D.gepdmm("N", "N", A, X.cpu().numpy().T.copy(), n, api=orc) - B
         assert np.linalg.norm(r) / np.linalg.norm(B) < 1e-10
     P.close()
+
+
+@pytest.mark.parametrize("m,nlvl,rank,n,off", [
+    (8192, 7, 32, 64, 0),    # the step's shape class: the apply rides on the first residual
+    (4096, 6, 32, 16, 0),    # fewer columns than a group
+    (4096, 6, 32, 100, 0),   # a chunk above 64 columns: apply made apart
+    (4096, 6, 32, 64, 1),    # Xa one element off its 16-byte alignment: apply made apart
+])
+def test_apply_solve_refined(cuda, orc, m, nlvl, rank, n, off):
+    """dplrx_apply_solve_refined_tol against its two halves: Ya against the
+    FP64 product (and the oracle), X, the solve count and the estimate
+    against dplrx_solve_refined_tol on the same B (the same kernels in the
+    same order for the solve)."""
+    import torch
+    print(f"apply+solve m={m} L={nlvl} r={rank} n={n} off={off}", flush=True)
+    A = PlrTree.random(m, nlvl, rank, seed=44)
+    P = D.Plan(A)
+    assert P.factor() == 0
+    assert P.factor_mixed("f16x2") == 0
+    rng = np.random.default_rng(9)
+    Xh, Bh = rng.standard_normal((m, n)), rng.standard_normal((m, n))
+    big = torch.zeros(n * m + off, dtype=torch.float64, device=cuda)
+    Xa = big[off:].view(n, m)
+    Xa.copy_(torch.tensor(np.ascontiguousarray(Xh.T), device=cuda))
+    B = torch.tensor(np.ascontiguousarray(Bh.T), device=cuda)
+    Ya, X = torch.zeros_like(B), torch.zeros_like(B)
+    its, est = P.apply_solve_refined_tol(Xa, Ya, B, X, tol=1e-10, maxit=8)
+    Yref, Xref = torch.zeros_like(B), torch.zeros_like(B)
+    P.gepdmm("N", 1.0, Xa.contiguous(), 0.0, Yref)
+    its2, est2 = P.solve_refined_tol(B, Xref, tol=1e-10, maxit=8)
+    torch.cuda.synchronize()
+    assert rel(Ya.cpu().numpy(), Yref.cpu().numpy()) < 1e-14
+    yo = D.gepdmm("N", "N", A, Xh, n, api=orc)
+    assert rel(Ya.cpu().numpy().T, yo) < 1e-12
+    # (the correction norms behind the estimate are summed by atomics: the
+    # estimates agree to rounding, not bits)
+    assert its == its2 and abs(est - est2) <= 1e-9 * est2 and est <= 1e-10
+    assert rel(X.cpu().numpy(), Xref.cpu().numpy()) <= 1e-14
+    X64 = torch.zeros_like(B)
+    P.solve("N", B, X64)
+    torch.cuda.synchronize()
+    assert rel(X.cpu().numpy(), X64.cpu().numpy()) <= 1e-10
+    P.close()
+    A.free()
