@@ -217,6 +217,28 @@ struct Mv3Tab
     const double *P, *Q, *S[PLR_MAXL];
 };
 
+// Sum of the ngs partials pp[j * R] of one coefficient (the 1024-row pieces
+// of a long segment): even j into one sum and odd j into another, added at
+// the end, with the loads of MV_PSUM partials issued before any is used
+// (one L2 round trip per MV_PSUM partials, not per two; more would spill in
+// the expansion, whose first two column batches are live here).
+static constexpr int MV_PSUM = 8;
+__device__ __forceinline__ double mv_part_sum(const double *__restrict__ pp, int ngs, int R)
+{
+    double v0 = 0.0, v1 = 0.0;
+    for (int j0 = 0; j0 < ngs; j0 += MV_PSUM) {
+        double t[MV_PSUM];
+#pragma unroll
+        for (int j = 0; j < MV_PSUM; ++j) t[j] = j0 + j < ngs ? pp[(size_t)(j0 + j) * R] : 0.0;
+#pragma unroll
+        for (int j = 0; j < MV_PSUM; j += 2) {
+            v0 += t[j];
+            v1 += t[j + 1];
+        }
+    }
+    return v0 + v1;
+}
+
 template <int MK>
 __global__ __launch_bounds__(512) void k_mv_reduce3(Mv3Tab a, const double *__restrict__ x,
                                                     int ldx, double *__restrict__ part,
@@ -381,15 +403,8 @@ __global__ __launch_bounds__(MW * 64) void k_mv_expand3(Mv3Tab a, int b, const d
             double v;
             if (seg > MV3_RB) {
                 const int sp = s ^ 1, ngs = seg / MV3_RB;
-                const double *__restrict__ pp = part + a.p_off[d] + (size_t)sp * ngs * R + k;
-                double v0 = 0.0, v1 = 0.0;
-                int j = 0;
-                for (; j + 1 < ngs; j += 2) {
-                    v0 += pp[(size_t)j * R];
-                    v1 += pp[(size_t)(j + 1) * R];
-                }
-                if (j < ngs) v0 += pp[(size_t)j * R];
-                v = (v0 + v1) * a.S[d][(size_t)(trans ? sp : s) * R + k];
+                v = mv_part_sum(part + a.p_off[d] + (size_t)sp * ngs * R + k, ngs, R) *
+                    a.S[d][(size_t)(trans ? sp : s) * R + k];
             } else {
                 v = w[a.w_off[d] + (size_t)s * R + k];
             }
@@ -527,15 +542,8 @@ __global__ __launch_bounds__(MW * 64) void k_mv_expand4(Mv3Tab a, int b, const d
             double v;
             if (seg > MV3_RB) {
                 const int sp = s ^ 1, ngs = seg / MV3_RB;
-                const double *__restrict__ pp = part + a.p_off[d] + (size_t)sp * ngs * R + k;
-                double v0 = 0.0, v1 = 0.0;
-                int jj = 0;
-                for (; jj + 1 < ngs; jj += 2) {
-                    v0 += pp[(size_t)jj * R];
-                    v1 += pp[(size_t)(jj + 1) * R];
-                }
-                if (jj < ngs) v0 += pp[(size_t)jj * R];
-                v = (v0 + v1) * a.S[d][(size_t)(trans ? sp : s) * R + k];
+                v = mv_part_sum(part + a.p_off[d] + (size_t)sp * ngs * R + k, ngs, R) *
+                    a.S[d][(size_t)(trans ? sp : s) * R + k];
             } else {
                 v = w[a.w_off[d] + (size_t)s * R + k];
             }
@@ -631,7 +639,7 @@ static void mv3_tab(const dplrh_plan *p, int trans, int mk, Mv3Tab *t)
 }
 
 // Launch shapes (measured at N=65536, L=10, leaf 64, rank 32: reduction
-// 28.4 us, expansion 37.0 us for 370.7 MB)
+// 28.5-29.3 us, expansion 34.7-35.5 us for 370.7 MB)
 static constexpr int MV3_MK = 16, MV4_MW = 16, MV4_UB = 2, MV3_MW = 16, MV3_UB = 4;
 
 static int mv3_launch_ws(dplrh_plan *p, int trans, int nc, double alpha, double beta,
