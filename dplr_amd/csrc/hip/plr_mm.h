@@ -47,11 +47,15 @@ struct MmSplit
     double alpha1, beta1;
 };
 
-// the base that puts column 64 * cg0 of a split product's second part at p
+// the base that puts column 64 * cg0 of a split product's second part at p;
+// formed by integer arithmetic, which hides that it points to global
+// memory, so it is cast through a global pointer (else every load through
+// X, C or Y in the kernels that select it would become a flat load)
 template <typename T>
 __device__ __forceinline__ T *mm_split_base(T *p, int cg0, int ld)
 {
-    return (T *)((uintptr_t)p - (uintptr_t)cg0 * MM_CG * (uintptr_t)ld * sizeof(double));
+    typedef __attribute__((address_space(1))) T GT;
+    return (T *)(GT *)((uintptr_t)p - (uintptr_t)cg0 * MM_CG * (uintptr_t)ld * sizeof(double));
 }
 
 // ---------------------------------------------------------------------------
