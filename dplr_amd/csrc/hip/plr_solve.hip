@@ -192,10 +192,20 @@ __global__ __launch_bounds__(256) void k_gj_inverse(const double *A, int n, size
         for (int j = tid; j < n; j += nt)
             if (j != k) a[(size_t)j * n + k] *= inv;
         __syncthreads();
-        // eliminate column k from every other row
-        for (int e = tid; e < n * n; e += nt) {
-            const int i = e % n, j = e / n;
-            if (i != k && j != k) a[(size_t)j * n + i] -= a[(size_t)k * n + i] * a[(size_t)j * n + k];
+        // eliminate column k from every other row; element e = j*n + i,
+        // its row and column stepped along (no division per element)
+        {
+            const int di = nt % n, dj = nt / n;
+            int i = tid % n, j = tid / n;
+            for (int e = tid; e < n * n; e += nt) {
+                if (i != k && j != k) a[(size_t)j * n + i] -= a[(size_t)k * n + i] * a[(size_t)j * n + k];
+                i += di;
+                j += dj;
+                if (i >= n) {
+                    i -= n;
+                    ++j;
+                }
+            }
         }
         __syncthreads();
         for (int i = tid; i < n; i += nt)
@@ -419,11 +429,15 @@ static int gj_max_order()
     if (nmax < 0) {
         int dev = 0, lds = 0;
         (void)hipGetDevice(&dev);
-        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess)
-            lds = 65536;
-        int want = lds - 4096; // room for the static scratch
-        if (want > 65536 &&
-            hipFuncSetAttribute((const void *)k_gj_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
+        // the most a kernel may opt in to (the default per-block limit is
+        // 64 KB; this device's LDS holds 160 KB)
+        if (hipDeviceGetAttribute(&lds, hipDeviceAttributeSharedMemPerBlockOptin, dev) != hipSuccess ||
+            lds <= 65536) {
+            (void)hipGetLastError();
+            lds = 160 * 1024;
+        }
+        int want = lds - 8192; // room for the static scratch (4100 bytes)
+        if (hipFuncSetAttribute((const void *)k_gj_inverse, hipFuncAttributeMaxDynamicSharedMemorySize,
                                 want) != hipSuccess) {
             (void)hipGetLastError();
             want = 65536 - 4096;
