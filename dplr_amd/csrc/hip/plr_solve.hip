@@ -263,10 +263,18 @@ __device__ __forceinline__ void gj_amax_dpp(unsigned &hi, unsigned &lo, int &p)
     p = take ? p2 : p;
 }
 
-__device__ __forceinline__ int gj_pivot(double a, int pos, int k)
+// ok: the row may pivot step k -- a row of the matrix for k < n, a row of
+// the identity padding for k >= n (else an all-NaN column would take a
+// padding row, which would then leave the n x n block)
+__device__ __forceinline__ unsigned long long gj_key(double a, int pos, int k, bool ok)
 {
-    unsigned long long key = 0ull;
-    if (pos >= k) key = a != a ? 1ull : (unsigned long long)__double_as_longlong(fabs(a)) + 2ull;
+    if (pos < k || !ok) return 0ull;
+    return a != a ? 1ull : (unsigned long long)__double_as_longlong(fabs(a)) + 2ull;
+}
+
+// the position of the wave's largest key (ties: the smaller position)
+__device__ __forceinline__ int gj_pivot_key(unsigned long long key, int pos)
+{
     unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
     int p = pos;
     gj_amax_dpp<0xB1>(hi, lo, p);  // quad_perm 1,0,3,2
@@ -287,6 +295,20 @@ __device__ __forceinline__ int gj_pivot(double a, int pos, int k)
         }
     }
     return bp;
+}
+
+__device__ __forceinline__ int gj_pivot(double a, int pos, int k, bool ok)
+{
+    return gj_pivot_key(gj_key(a, pos, k, ok), pos);
+}
+
+// the same over two rows per lane: the lane's better candidate first
+__device__ __forceinline__ int gj_pivot2(double a0, int pos0, bool ok0, double a1, int pos1,
+                                         bool ok1, int k)
+{
+    const unsigned long long k0 = gj_key(a0, pos0, k, ok0), k1 = gj_key(a1, pos1, k, ok1);
+    const bool second = k1 > k0 || (k1 == k0 && pos1 < pos0);
+    return gj_pivot_key(second ? k1 : k0, second ? pos1 : pos0);
 }
 
 template <int NM>
@@ -310,7 +332,7 @@ __global__ __launch_bounds__(256) void k_gj_inverse_w(const double *A, int n, si
             const int k = k0 + c;
             // pivot: largest |a[c]| over positions >= k, ties to the smaller
             // position (NaN ranks below every number: one order for all lanes)
-            const int key = gj_pivot(a[c], pos, k);
+            const int key = gj_pivot(a[c], pos, k, (lane < n) == (k < n));
             const bool piv = pos == key;
             const int pl = __ffsll((unsigned long long)__ballot(piv)) - 1;
             if (pos == k)
@@ -382,7 +404,7 @@ __global__ __launch_bounds__(NM * 4) void k_gj_inverse_q(const double *A, int n,
             if (wv == kb) {
                 // largest |a[c]| over positions >= k, ties to the smaller
                 // position (NaN ranks below every number)
-                const int key = gj_pivot(a[c], pos, k);
+                const int key = gj_pivot(a[c], pos, k, (lane < n) == (k < n));
                 const int pl = __ffsll((unsigned long long)__ballot(pos == key)) - 1;
                 colv[buf][lane] = a[c];
                 if (lane == 0) {
@@ -417,6 +439,90 @@ __global__ __launch_bounds__(NM * 4) void k_gj_inverse_q(const double *A, int n,
         const int col = 16 * wv + j;
         const int pc = __ffsll((unsigned long long)__ballot(pos == col)) - 1;
         if (col < n && lane < n) dst[(size_t)pc * n + pos] = a[j];
+    }
+    if (threadIdx.x == 0) info[mat] = sinfo;
+}
+
+// The elimination of k_gj_inverse_q for orders up to 128: one work-group
+// of eight waves per matrix, wave w holding columns 16w .. 16w+15 of rows
+// l and 64 + l in lane l (two registers per column).  The owner of the
+// step's column finds the pivot over both rows of every lane and publishes
+// the column and the pivot's position through LDS (double-buffered, one
+// barrier per step); the pivot row's entries come from lane reads of the
+// half that holds it.  Pivot rule, arithmetic per element, output and info
+// as k_gj_inverse_w.
+__global__ __launch_bounds__(512) void k_gj_inverse_r(const double *A, int n, size_t stride,
+                                                      double *Inv, size_t istride, int *info)
+{
+    __shared__ double colv[2][128];
+    __shared__ int pk[2];
+    const int lane = threadIdx.x & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int mat = blockIdx.x;
+    const double *src = A + (size_t)mat * stride;
+    double a[16][2];
+#pragma unroll
+    for (int j = 0; j < 16; ++j)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+            const int gc = 16 * wv + j, row = 64 * h + lane;
+            a[j][h] = (row < n && gc < n) ? src[(size_t)gc * n + row] : (gc == row ? 1.0 : 0.0);
+        }
+    int pos0 = lane, pos1 = 64 + lane, sinfo = 0;
+    const int nb = min(128, (n + 15) & ~15) / 16;
+    for (int kb = 0; kb < nb; ++kb) {
+#pragma unroll
+        for (int c = 0; c < 16; ++c) {
+            const int k = 16 * kb + c, buf = k & 1;
+            if (wv == kb) {
+                const int key = gj_pivot2(a[c][0], pos0, (lane < n) == (k < n), a[c][1], pos1,
+                                          (64 + lane < n) == (k < n), k);
+                colv[buf][lane] = a[c][0];
+                colv[buf][64 + lane] = a[c][1];
+                if (lane == 0) pk[buf] = key;
+            }
+            __syncthreads();
+            const int key = pk[buf];
+            const bool piv0 = pos0 == key, piv1 = pos1 == key;
+            const unsigned long long b0 = __ballot(piv0), b1 = __ballot(piv1);
+            const int ph = b0 ? 0 : 1; // the half that holds the pivot row (uniform)
+            const int pl = __ffsll(ph ? b1 : b0) - 1;
+            if (pos0 == k)
+                pos0 = key;
+            else if (piv0)
+                pos0 = k;
+            if (pos1 == k)
+                pos1 = key;
+            else if (piv1)
+                pos1 = k;
+            const double pv = colv[buf][64 * ph + pl];
+            if (pv == 0.0 && sinfo == 0) sinfo = k + 1;
+            const double inv = 1.0 / pv, f0 = colv[buf][lane], f1 = colv[buf][64 + lane];
+#pragma unroll
+            for (int j = 0; j < 16; ++j) {
+                if (wv == kb && j == c) continue;
+                const double pj = gj_rdlane(ph ? a[j][1] : a[j][0], pl) * inv;
+                a[j][0] = piv0 ? pj : a[j][0] - f0 * pj;
+                a[j][1] = piv1 ? pj : a[j][1] - f1 * pj;
+            }
+            if (wv == kb) {
+                a[c][0] = piv0 ? inv : -f0 * inv;
+                a[c][1] = piv1 ? inv : -f1 * inv;
+            }
+        }
+    }
+    // column col of inv(PA) is column perm[col] of inv(A), perm[col] the
+    // row at position col
+    double *dst = Inv + (size_t)mat * istride;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+        const int col = 16 * wv + j;
+        const unsigned long long b0 = __ballot(pos0 == col), b1 = __ballot(pos1 == col);
+        const int pc = b0 ? __ffsll(b0) - 1 : 64 + __ffsll(b1) - 1;
+        if (col < n) {
+            if (lane < n) dst[(size_t)pc * n + pos0] = a[j][0];
+            if (64 + lane < n) dst[(size_t)pc * n + pos1] = a[j][1];
+        }
     }
     if (threadIdx.x == 0) info[mat] = sinfo;
 }
@@ -473,6 +579,12 @@ static int batched_inverse(double *A, int n, int count, double *Inv, int *piv, i
         else
             hipLaunchKernelGGL(k_gj_inverse_w<64>, grid, dim3(256), 0, st, A, n, (size_t)n * n, Inv,
                                (size_t)n * n, info, count);
+        PLR_HIP(hipGetLastError());
+        return 0;
+    }
+    if (n <= 128) {
+        hipLaunchKernelGGL(k_gj_inverse_r, dim3(count), dim3(512), 0, st, A, n, (size_t)n * n, Inv,
+                           (size_t)n * n, info);
         PLR_HIP(hipGetLastError());
         return 0;
     }
