@@ -36,7 +36,12 @@ def gpu_solve(A, B, trans, torch, dev):
     return X
 
 
-SHAPES = [(2048, 5, 32), (2048, 6, 16), (1024, 2, 40), (96, 5, 2), (64, 0, 0), (256, 3, 0)]
+# (1024, 2, 40): leaves of 256, capacitances of order 80; (1024, 3, 56):
+# leaves of 128, capacitances of order 112; (1536, 4, 12): leaves of 96 --
+# the orders 65..128 of the two-rows-per-lane Gauss-Jordan, with and without
+# identity padding
+SHAPES = [(2048, 5, 32), (2048, 6, 16), (1024, 2, 40), (1024, 3, 56), (1536, 4, 12), (96, 5, 2),
+          (64, 0, 0), (256, 3, 0)]
 
 
 @pytest.mark.parametrize("m,nlvl,rank", SHAPES)
