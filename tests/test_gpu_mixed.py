@@ -262,6 +262,7 @@ def test_f16_pairs_refine_in_two_solves(cuda, orc, m, nlvl, rank, n):
     (8192, 7, 32, 64, 0),    # the step's shape class: the apply rides on the first residual
     (4096, 6, 32, 16, 0),    # fewer columns than a group
     (4096, 6, 32, 100, 0),   # a chunk above 64 columns: apply made apart
+    (4096, 6, 32, 130, 0),   # chunks of 128 and 2: apart, then riding on the residual
     (4096, 6, 32, 64, 1),    # Xa one element off its 16-byte alignment: apply made apart
 ])
 def test_apply_solve_refined(cuda, orc, m, nlvl, rank, n, off):
