@@ -691,8 +691,23 @@ def main():
     Bh = np.random.default_rng(3).standard_normal((m, nrhs))
     D.gepmv("N", tree0, xh)
     dropin = {}
+
+    def geinv_once():
+        Binv, info = D.geinv(tree0, 0.0)
+        Binv.free()
+
+    def cpu_sample_once():
+        # the CPU baseline's sample through the drop-in: apply, explicit
+        # inverse, inverse applied
+        D.gepdmm("N", "N", tree0, Bh, nrhs)
+        Binv, info = D.geinv(tree0, 0.0)
+        D.gepdmm("N", "N", Binv, Bh, nrhs)
+        Binv.free()
+
     for name, fn in (("gepmv_ms", lambda: D.gepmv("N", tree0, xh)),
-                     ("gepdmm_ms", lambda: D.gepdmm("N", "N", tree0, Bh, nrhs))):
+                     ("gepdmm_ms", lambda: D.gepdmm("N", "N", tree0, Bh, nrhs)),
+                     ("geinv_ms", geinv_once),
+                     ("cpu_sample_ms", cpu_sample_once)):
         ts = []
         for _ in range(3):
             t1 = time.perf_counter()
@@ -702,9 +717,12 @@ def main():
     tree0.free()
     dropin["note"] = (f"host tree in, host result out, median of 3 calls at N={m}: flatten, "
                       f"upload of {abytes / 1e6:.0f} MB, kernels, download; the device-resident "
-                      "plan (dplrx_*) is what the other figures time")
+                      "plan (dplrx_*) is what the other figures time; cpu_sample_ms is the CPU "
+                      "baseline's sample (dplrgepdmm, dplrgeinv at eps 0, dplrgepdmm of the "
+                      "inverse) through the same entry points")
     log(f"drop-in calls on the host tree: dplrgepmv {dropin['gepmv_ms']:.1f} ms, "
-        f"dplrgepdmm ({nrhs} RHS) {dropin['gepdmm_ms']:.1f} ms")
+        f"dplrgepdmm ({nrhs} RHS) {dropin['gepdmm_ms']:.1f} ms, dplrgeinv {dropin['geinv_ms']:.1f} ms, "
+        f"the CPU baseline's sample {dropin['cpu_sample_ms']:.1f} ms")
 
     batch = None
     if args.batch_count > 0:

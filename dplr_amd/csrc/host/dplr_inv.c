@@ -29,12 +29,37 @@ typedef struct
 static int rank_at(const plr_flat *f, int d, int s) { return f->rank[(2 << d) - 2 + s]; }
 
 /* dplrgeprk's structural effect: k more columns in every block of the subtree */
+/* The output tree is shaped as the reference shapes it (each node created
+ * with its own rank, then every ancestor's correction columns appended to
+ * its subtrees: at least doubling capacities, src/dplrresizelr.c:54), but
+ * on the sizes alone: a node that this sequence would create or grow has
+ * its buffers released here and is allocated once, at its final capacity,
+ * by alloc_shaped.  The end state is the reference's: the final
+ * capacities, every slot below the rank written by fill_inverse, and NaN
+ * above it in the buffers this call allocates (a node whose buffers suffice
+ * keeps them, slots above the rank untouched). */
+static void lr_release(DPlrMtx *X)
+{
+    free(X->dlr.u);
+    free(X->dlr.v);
+    free(X->dlr.sig);
+    X->dlr.u = X->dlr.v = X->dlr.sig = NULL;
+}
+
+/* dplrresizelr on the sizes: the new capacity, the buffers released */
+static void lr_grow(DPlrMtx *X, int need)
+{
+    if (need <= X->dlr.capacity) return;
+    X->dlr.capacity = need > 2 * X->dlr.capacity ? need : 2 * X->dlr.capacity;
+    lr_release(X);
+}
+
 static void grow_subtree(DPlrMtx *X, int k)
 {
     if (k == 0 || X->type != PLR_HMTX) return;
     DPlrMtx *lr[2] = {X->dh.ne, X->dh.sw};
     for (int i = 0; i < 2; ++i) {
-        if (lr[i]->dlr.capacity < lr[i]->dlr.rank + k) dplrresizelr(lr[i], lr[i]->dlr.rank + k);
+        lr_grow(lr[i], lr[i]->dlr.rank + k);
         lr[i]->dlr.rank += k;
     }
     grow_subtree(X->dh.nw, k);
@@ -60,10 +85,14 @@ static void shape_inverse(const DPlrMtx *A, DPlrMtx *B, int m, int d)
     const DPlrMtx *Ax[2] = {A->dh.ne, A->dh.sw};
     for (int i = 0; i < 2; ++i) {
         if (Bx[i]->type != PLR_LRMTX || Bx[i]->size != m / 2) {
+            /* dplrinilr on the sizes */
             dplrclr(Bx[i]);
-            dplrinilr(m / 2, Bx[i], Ax[i]->dlr.rank);
+            Bx[i]->type = PLR_LRMTX;
+            Bx[i]->size = m / 2;
+            Bx[i]->dlr.rank = 0;
+            Bx[i]->dlr.capacity = Ax[i]->dlr.rank;
         }
-        if (Bx[i]->dlr.capacity < Ax[i]->dlr.rank) dplrresizelr(Bx[i], Ax[i]->dlr.rank);
+        lr_grow(Bx[i], Ax[i]->dlr.rank);
     }
     const int r1 = A->dh.ne->dlr.rank, r2 = A->dh.sw->dlr.rank;
     if (r1 == 0 && r2 == 0) return;
@@ -103,17 +132,70 @@ static void fill_block(DPlrMtx *X, const inv_ctx *c, int dp, int q)
         fprintf(stderr, "dplrgeinv: internal rank mismatch (%d vs %d)\n", pos, X->dlr.rank);
 }
 
-static void fill_inverse(DPlrMtx *B, const inv_ctx *c, int d, int n)
+/* The output's blocks in a flat list: leaves (level -1) and low-rank
+ * blocks with their level and segment, for the parallel passes below. */
+typedef struct
+{
+    DPlrMtx *X;
+    int d, q;
+} inv_blk;
+
+static size_t list_blocks(DPlrMtx *B, int d, int n, inv_blk *out, size_t k)
 {
     if (B->type == PLR_DMTX) {
-        const size_t bb = (size_t)c->f->b * c->f->b;
-        if (bb) memcpy(B->dd.d, c->BD + (size_t)n * bb, bb * sizeof(double));
-        return;
+        if (out) out[k] = (inv_blk){B, -1, n};
+        return k + 1;
     }
-    fill_block(B->dh.ne, c, d, 2 * n);
-    fill_block(B->dh.sw, c, d, 2 * n + 1);
-    fill_inverse(B->dh.nw, c, d + 1, 2 * n);
-    fill_inverse(B->dh.se, c, d + 1, 2 * n + 1);
+    if (B->type != PLR_HMTX) return k;
+    if (out) {
+        out[k] = (inv_blk){B->dh.ne, d, 2 * n};
+        out[k + 1] = (inv_blk){B->dh.sw, d, 2 * n + 1};
+    }
+    k = list_blocks(B->dh.nw, d + 1, 2 * n, out, k + 2);
+    return list_blocks(B->dh.se, d + 1, 2 * n + 1, out, k);
+}
+
+static void nan_tail(double *p, size_t from, size_t to)
+{
+    for (size_t i = from; i < to; ++i) p[i] = NAN;
+}
+
+/* Buffers of the low-rank blocks the shaping released or created, at their
+ * final capacity; NaN above the rank (below it fill_inverse writes). */
+static int alloc_shaped(inv_blk *bl, size_t nb)
+{
+    int fail = 0;
+#pragma omp parallel for schedule(dynamic, 8) reduction(| : fail)
+    for (size_t i = 0; i < nb; ++i) {
+        DPlrMtx *X = bl[i].X;
+        if (bl[i].d < 0 || X->dlr.u || X->dlr.capacity == 0) continue;
+        const size_t len = (size_t)X->size, cap = (size_t)X->dlr.capacity, r = (size_t)X->dlr.rank;
+        X->dlr.u = malloc(len * cap * sizeof(double));
+        X->dlr.v = malloc(len * cap * sizeof(double));
+        X->dlr.sig = malloc(cap * sizeof(double));
+        if (!X->dlr.u || !X->dlr.v || !X->dlr.sig) {
+            lr_release(X);
+            fail = 1;
+            continue;
+        }
+        nan_tail(X->dlr.u, r * len, cap * len);
+        nan_tail(X->dlr.v, r * len, cap * len);
+        nan_tail(X->dlr.sig, r, cap);
+    }
+    return fail;
+}
+
+static void fill_inverse(inv_blk *bl, size_t nb, const inv_ctx *c)
+{
+    const size_t bb = (size_t)c->f->b * c->f->b;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (size_t i = 0; i < nb; ++i) {
+        if (bl[i].d < 0) {
+            if (bb) memcpy(bl[i].X->dd.d, c->BD + (size_t)bl[i].q * bb, bb * sizeof(double));
+        } else {
+            fill_block(bl[i].X, c, bl[i].d, bl[i].q);
+        }
+    }
 }
 
 static void nan_fill(DPlrMtx *X)
@@ -152,6 +234,13 @@ static void nan_spread(DPlrMtx *B, const plr_flat *f, const int *flag)
                 break;
             }
         }
+}
+
+static void touch_pages(double *p, size_t n)
+{
+    const size_t step = 4096 / sizeof(double);
+#pragma omp parallel for schedule(static)
+    for (size_t i = 0; i < n; i += step) p[i] = 0.0;
 }
 
 /* reference src/dplrinv.c:93 */
@@ -210,6 +299,13 @@ void dplrgeinv(int m, DPlrMtx *A, DPlrMtx *B, double eps, int *info)
         fprintf(stderr, "%s: host memory allocation failed\n", __func__);
         goto out;
     }
+    /* the pages of the staging buffers mapped by all threads before the
+     * download writes them (a first touch inside the copy is one thread's) */
+    for (int dp = 0; dp < f.L; ++dp) {
+        touch_pages(c.BU[dp], (size_t)f.N * c.Rt[dp]);
+        touch_pages(c.BV[dp], (size_t)f.N * c.Rt[dp]);
+    }
+    touch_pages(c.BD, (size_t)f.N * f.b);
     if (m > 0) {
         rc = dplrh_geinv(p, eps, c.Rt, c.coff, c.BU, c.BV, c.Bsig, c.BD, nanflag, NULL);
         if (rc) {
@@ -218,9 +314,24 @@ void dplrgeinv(int m, DPlrMtx *A, DPlrMtx *B, double eps, int *info)
         }
     }
     shape_inverse(A, B, m, 0);
-    if (m > 0) {
-        fill_inverse(B, &c, 0, 0);
-        nan_spread(B, &f, nanflag);
+    {
+        const size_t nb = list_blocks(B, 0, 0, NULL, 0);
+        inv_blk *bl = malloc((nb ? nb : 1) * sizeof(inv_blk));
+        if (!bl) {
+            fprintf(stderr, "%s: host memory allocation failed\n", __func__);
+            goto out;
+        }
+        list_blocks(B, 0, 0, bl, 0);
+        if (alloc_shaped(bl, nb)) {
+            fprintf(stderr, "%s: memory allocation failed\n", __func__);
+            free(bl);
+            goto out;
+        }
+        if (m > 0) {
+            fill_inverse(bl, nb, &c);
+            nan_spread(B, &f, nanflag);
+        }
+        free(bl);
     }
     *info = 0;
 out:

@@ -6,7 +6,8 @@ import numpy as np
 import pytest
 
 import dplr_amd as D
-from dplr_amd.tree import PlrTree
+from dplr_amd.abi import PLR_HMTX, PLR_LRMTX
+from dplr_amd.tree import PlrTree, _fn, nodes
 
 pytestmark = pytest.mark.gpu
 
@@ -88,6 +89,48 @@ def test_geinv_reuses_existing_b(cuda, orc):
     D.lib.dplrgeinv(512, A.ref, B.ref, 0.0, ctypes.byref(info))   # second call into the same B
     Bo, _ = D.geinv(A, 0.0, api=orc)
     assert info.value == 0 and rel(B.dense(), Bo.dense(api=orc)) < 1e-12
+
+
+def _lr_buffers(tree):
+    """(u, v, sig) of every low-rank block over its whole capacity, in
+    pre-order."""
+    out = []
+
+    def walk(p):
+        c = p.contents
+        if c.type == PLR_LRMTX:
+            n, cap = c.size * c.dlr.capacity, c.dlr.capacity
+            get = (lambda q, k: np.ctypeslib.as_array(q, shape=(k,)).copy() if k else np.zeros(0))
+            out.append((get(c.dlr.u, n), get(c.dlr.v, n), get(c.dlr.sig, cap)))
+        elif c.type == PLR_HMTX:
+            for w in ("nw", "ne", "se", "sw"):
+                walk(getattr(c.dh, w))
+    walk(ctypes.pointer(tree.root))
+    return out
+
+
+@pytest.mark.parametrize("reuse", [False, True])
+def test_geinv_output_buffers_match_reference(cuda, ref, reuse):
+    """The output tree as the reference shapes it: every node's type, size,
+    rank and capacity (capacities at least double as correction columns are
+    appended, src/dplrresizelr.c:54), and NaN exactly where the reference's
+    buffers hold NaN (the slots above the rank of the buffers the call
+    allocates) -- for a new B, and for a B shaped by an earlier call."""
+    A = PlrTree.from_blocks(1024, 4, lambda d, n, w: 1 + (3 * d + n + (w == "sw") * 4) % 9,
+                            np.random.default_rng(5))
+    B, info = D.geinv(A, 0.0)
+    Br, info_r = D.geinv(A, 0.0, api=ref)
+    if reuse:
+        i1, i2 = ctypes.c_int(9), ctypes.c_int(9)
+        D.lib.dplrgeinv(1024, A.ref, B.ref, 0.0, ctypes.byref(i1))
+        _fn(ref, "dplrgeinv")(1024, A.ref, Br.ref, 0.0, ctypes.byref(i2))
+        info, info_r = i1.value, i2.value
+    assert info == info_r == 0
+    assert nodes(B) == nodes(Br)
+    for (u, v, s), (ur, vr, sr) in zip(_lr_buffers(B), _lr_buffers(Br)):
+        for x, y in ((u, ur), (v, vr), (s, sr)):
+            assert np.array_equal(np.isnan(x), np.isnan(y))
+    assert rel(B.dense(), Br.dense(api=ref)) < 1e-12
 
 
 def test_geinv_singular(cuda, orc):
