@@ -105,6 +105,28 @@ static void fill_lr(const DPlrMtx *B, plr_flat *f, int d, int useg, int vseg)
     f->rank[(2 << d) - 2 + useg] = r;
 }
 
+/* one block to copy: a leaf (vseg < 0, useg its index) or a low-rank block
+ * of level d with the segments of its U and V */
+typedef struct
+{
+    const DPlrMtx *X;
+    int d, useg, vseg;
+} flat_item;
+
+static size_t list_items(const DPlrMtx *A, int d, int n, flat_item *out, size_t k)
+{
+    if (A->type == PLR_DMTX) {
+        if (out) out[k] = (flat_item){A, d, n, -1};
+        return k + 1;
+    }
+    if (out) {
+        out[k] = (flat_item){A->dh.ne, d, 2 * n, 2 * n + 1};
+        out[k + 1] = (flat_item){A->dh.sw, d, 2 * n + 1, 2 * n};
+    }
+    k = list_items(A->dh.nw, d + 1, 2 * n, out, k + 2);
+    return list_items(A->dh.se, d + 1, 2 * n + 1, out, k);
+}
+
 static void fill_walk(const DPlrMtx *A, plr_flat *f, int d, int n)
 {
     if (A->type == PLR_DMTX) {
@@ -133,7 +155,25 @@ int plr_flat_fill(const void *Av, plr_flat *f)
         plr_flat_free(f);
         return -3;
     }
-    fill_walk((const DPlrMtx *)Av, f, 0, 0);
+    /* the blocks copied in parallel (they fill disjoint parts of the level
+     * arrays); the walk alone when the list cannot be allocated */
+    const size_t ni = list_items((const DPlrMtx *)Av, 0, 0, NULL, 0);
+    flat_item *it = malloc(ni * sizeof(flat_item));
+    if (!it) {
+        fill_walk((const DPlrMtx *)Av, f, 0, 0);
+        return 0;
+    }
+    list_items((const DPlrMtx *)Av, 0, 0, it, 0);
+    const size_t bb = (size_t)f->b * (size_t)f->b;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (size_t i = 0; i < ni; ++i) {
+        if (it[i].vseg < 0) {
+            if (bb) memcpy(f->D + (size_t)it[i].useg * bb, it[i].X->dd.d, bb * sizeof(double));
+        } else {
+            fill_lr(it[i].X, f, it[i].d, it[i].useg, it[i].vseg);
+        }
+    }
+    free(it);
     return 0;
 }
 
