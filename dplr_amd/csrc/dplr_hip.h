@@ -144,6 +144,12 @@ int dplrh_geinv(dplrh_plan *p, double eps, const int *Rt, int coff[][DPLRH_MAXL]
 int dplrh_geprk(int N, int L, int b, int k, double alpha, double eps, const double *hu,
                 const double *hv, const double *hs, double *hD, double **hUn, double **hVn,
                 double **hsig, void *stream);
+/* on != 0: dplrh_geprk calls of this thread keep their device buffers for
+ * the next call (grown as needed); 0: release them and allocate per call. */
+void dplrh_prk_pool(int on);
+/* The same for the host buffers of dplrgeprk, and dplrh_prk_pool with it
+ * (host layer, dplr_inv.c): dplrgeppmm turns it on around its corrections. */
+void plr_prk_pool(int on);
 
 /* Level products of dplrgeppmm: for every level d with R_d > 0,
  * PU[d] := alpha * op_ta(A_{>d}) * (tb ? V_d(B) : U_d(B))   (N x R_d(B)) and

@@ -431,6 +431,37 @@ __global__ void k_prk_leaves(int N, int b, int k, double alpha, const double *u,
 // hu, hv: N x k column-major (ld N); hs: k scales; hD: leaves (in/out).
 // Per level d: hUn[d], hVn[d] receive N*k doubles laid out [q][c][row] and
 // hsig[d] nseg*k doubles, ready to be appended to each block.
+// Device buffers of dplrh_geprk, kept from call to call while the calling
+// thread's pool is on (the rank-k corrections of dplrgeppmm: a thousand
+// calls at N=65536, each of which would otherwise allocate and free its
+// eleven buffers); grown when a call needs more.
+namespace {
+struct PrkPool
+{
+    bool on;
+    int dev;
+    void *p[11];
+    size_t cap[11];
+};
+thread_local PrkPool g_prk;
+
+void prk_pool_free()
+{
+    for (int i = 0; i < 11; ++i) {
+        if (g_prk.p[i]) (void)hipFree(g_prk.p[i]);
+        g_prk.p[i] = nullptr;
+        g_prk.cap[i] = 0;
+    }
+}
+} // namespace
+
+extern "C" void dplrh_prk_pool(int on)
+{
+    if (!on) prk_pool_free();
+    g_prk.on = on != 0;
+    if (on) (void)hipGetDevice(&g_prk.dev);
+}
+
 extern "C" int dplrh_geprk(int N, int L, int b, int k, double alpha, double eps, const double *hu,
                            const double *hv, const double *hs, double *hD, double **hUn,
                            double **hVn, double **hsig, void *stream)
@@ -443,15 +474,36 @@ extern "C" int dplrh_geprk(int N, int L, int b, int k, double alpha, double eps,
     const double **tab = nullptr;
     int rc = 0;
     const size_t maxseg = L > 0 ? ((size_t)2 << (L - 1)) : 1;
+    const size_t Lc = L > 0 ? (size_t)L : 1; // every level's factors and sig stay until one wait
+    bool pooled = g_prk.on;
+    if (pooled) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        if (dev != g_prk.dev) pooled = false;
+    }
+    int slot = 0;
     auto A = [&](void **ptr, size_t bytes) -> int {
-        PLR_HIP(hipMalloc(ptr, bytes ? bytes : 8));
+        bytes = bytes ? bytes : 8;
+        if (pooled) {
+            const int i = slot++;
+            if (g_prk.cap[i] < bytes) {
+                if (g_prk.p[i]) (void)hipFree(g_prk.p[i]);
+                g_prk.p[i] = nullptr;
+                g_prk.cap[i] = 0;
+                PLR_HIP(hipMalloc(&g_prk.p[i], bytes));
+                g_prk.cap[i] = bytes;
+            }
+            *ptr = g_prk.p[i];
+            return 0;
+        }
+        PLR_HIP(hipMalloc(ptr, bytes));
         return 0;
     };
     if ((rc = A((void **)&u, nk * 8)) || (rc = A((void **)&v, nk * 8)) || (rc = A((void **)&s, k * 8)) ||
         (rc = A((void **)&D, nd * 8)) || (rc = A((void **)&sq, 2 * (size_t)k * nleaf * 8)) ||
         (rc = A((void **)&gu, maxseg * k * 8)) || (rc = A((void **)&gv, maxseg * k * 8)) ||
-        (rc = A((void **)&sg, maxseg * k * 8)) || (rc = A((void **)&BU, nk * 8)) ||
-        (rc = A((void **)&BV, nk * 8)) || (rc = A((void **)&tab, 2 * (size_t)k * sizeof(double *))))
+        (rc = A((void **)&sg, Lc * maxseg * k * 8)) || (rc = A((void **)&BU, Lc * nk * 8)) ||
+        (rc = A((void **)&BV, Lc * nk * 8)) || (rc = A((void **)&tab, 2 * (size_t)k * sizeof(double *))))
         goto done;
     PLR_HIP(hipMemcpyAsync(u, hu, nk * 8, hipMemcpyHostToDevice, st));
     PLR_HIP(hipMemcpyAsync(v, hv, nk * 8, hipMemcpyHostToDevice, st));
@@ -470,17 +522,18 @@ extern "C" int dplrh_geprk(int N, int L, int b, int k, double alpha, double eps,
         }
         PLR_HIP(hipMemcpyAsync(tab, htab.data(), 2 * (size_t)k * sizeof(double *),
                                hipMemcpyHostToDevice, st));
+        // each level into its own buffers, one wait for all of them
         for (int d = 0; d < L; ++d) {
             const int nseg = 2 << d, ns = nseg * k;
+            double *sgd = sg + (size_t)d * maxseg * k, *BUd = BU + (size_t)d * nk, *BVd = BV + (size_t)d * nk;
             hipLaunchKernelGGL(k_prk_scales, dim3((ns + 255) / 256), dim3(256), 0, st, L, nleaf, k,
-                               d, alpha, s, sq, sq + ne, eps2_of(d, L, eps), gu, gv, sg);
+                               d, alpha, s, sq, sq + ne, eps2_of(d, L, eps), gu, gv, sgd);
             hipLaunchKernelGGL(k_inv_gather, dim3((nk + 255) / 256), dim3(256), 0, st, N, nseg, k,
-                               tab, tab + k, gu, gv, BU, BV);
+                               tab, tab + k, gu, gv, BUd, BVd);
             PLR_HIP(hipGetLastError());
-            PLR_HIP(hipMemcpyAsync(hUn[d], BU, nk * 8, hipMemcpyDeviceToHost, st));
-            PLR_HIP(hipMemcpyAsync(hVn[d], BV, nk * 8, hipMemcpyDeviceToHost, st));
-            PLR_HIP(hipMemcpyAsync(hsig[d], sg, (size_t)ns * 8, hipMemcpyDeviceToHost, st));
-            PLR_HIP(hipStreamSynchronize(st));
+            PLR_HIP(hipMemcpyAsync(hUn[d], BUd, nk * 8, hipMemcpyDeviceToHost, st));
+            PLR_HIP(hipMemcpyAsync(hVn[d], BVd, nk * 8, hipMemcpyDeviceToHost, st));
+            PLR_HIP(hipMemcpyAsync(hsig[d], sgd, (size_t)ns * 8, hipMemcpyDeviceToHost, st));
         }
         hipLaunchKernelGGL(k_prk_leaves, dim3((nd + 255) / 256), dim3(256), 0, st, N, b, k, alpha,
                            u, v, s, D);
@@ -489,6 +542,11 @@ extern "C" int dplrh_geprk(int N, int L, int b, int k, double alpha, double eps,
         PLR_HIP(hipStreamSynchronize(st));
     }
 done:
+    if (pooled) {
+        // the next call on this thread reuses them: nothing may be in flight
+        if (rc) (void)hipStreamSynchronize(st);
+        return rc;
+    }
     void *bufs[] = {u, v, s, D, sq, gu, gv, sg, BU, BV, (void *)tab};
     for (void *x : bufs)
         if (x) (void)hipFree(x);

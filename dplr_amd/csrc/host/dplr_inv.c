@@ -395,6 +395,46 @@ static void append_walk(DPlrMtx *A, const prk_ctx *c, int d, int n)
     append_walk(A->dh.se, c, d + 1, 2 * n + 1);
 }
 
+/* Host buffers of dplrgeprk, kept from call to call while the thread's
+ * pool is on (as the device ones, dplrh_prk_pool): the pages a call's
+ * downloads write are then mapped already, not faulted in afresh. */
+enum { PRK_NBUF = 4 + 3 * DPLRH_MAXL };
+typedef struct
+{
+    int on;
+    double *p[PRK_NBUF];
+    size_t cap[PRK_NBUF];
+} prk_hpool;
+static _Thread_local prk_hpool g_hp;
+
+static double *prk_alloc(int i, size_t n)
+{
+    if (!g_hp.on) return malloc((n ? n : 1) * sizeof(double));
+    if (g_hp.cap[i] < n) {
+        free(g_hp.p[i]);
+        g_hp.p[i] = malloc((n ? n : 1) * sizeof(double));
+        g_hp.cap[i] = g_hp.p[i] ? n : 0;
+    }
+    return g_hp.p[i];
+}
+
+static void prk_release(double *p)
+{
+    if (!g_hp.on) free(p);
+}
+
+void plr_prk_pool(int on)
+{
+    if (!on)
+        for (int i = 0; i < PRK_NBUF; ++i) {
+            free(g_hp.p[i]);
+            g_hp.p[i] = NULL;
+            g_hp.cap[i] = 0;
+        }
+    g_hp.on = on != 0;
+    dplrh_prk_pool(on);
+}
+
 /* reference src/dplrprk.c:21 */
 void dplrgeprk(int m, int k, double *alpha, double *u, int ldu, double *v, int ldv, double *s,
                int incs, DPlrMtx *A, double eps)
@@ -425,15 +465,15 @@ void dplrgeprk(int m, int k, double *alpha, double *u, int ldu, double *v, int l
     const int N = f.N, L = f.L, b = f.b;
     if (N == 0) return;
     const size_t nk = (size_t)N * k;
-    double *hu = malloc(nk * sizeof(double)), *hv = malloc(nk * sizeof(double));
-    double *hs = malloc((size_t)k * sizeof(double));
-    double *hD = malloc((size_t)N * b * sizeof(double));
+    double *hu = prk_alloc(0, nk), *hv = prk_alloc(1, nk);
+    double *hs = prk_alloc(2, (size_t)k);
+    double *hD = prk_alloc(3, (size_t)N * b);
     double *Un[DPLRH_MAXL] = {0}, *Vn[DPLRH_MAXL] = {0}, *sig[DPLRH_MAXL] = {0};
     int ok = hu && hv && hs && hD;
     for (int d = 0; d < L && ok; ++d) {
-        Un[d] = malloc(nk * sizeof(double));
-        Vn[d] = malloc(nk * sizeof(double));
-        sig[d] = malloc(((size_t)2 << d) * k * sizeof(double));
+        Un[d] = prk_alloc(4 + 3 * d, nk);
+        Vn[d] = prk_alloc(5 + 3 * d, nk);
+        sig[d] = prk_alloc(6 + 3 * d, ((size_t)2 << d) * k);
         ok = Un[d] && Vn[d] && sig[d];
     }
     if (!ok) {
@@ -450,17 +490,29 @@ void dplrgeprk(int m, int k, double *alpha, double *u, int ldu, double *v, int l
             fprintf(stderr, "%s: GPU execution failed (%d): %s\n", __func__, rc, dplrh_last_error());
         } else {
             prk_ctx c = {k, Un, Vn, sig};
-            append_walk(A, &c, 0, 0);
+            /* the blocks take their columns in parallel (each writes its own
+             * buffers); the walk alone when the list cannot be allocated */
+            const size_t nb = list_blocks(A, 0, 0, NULL, 0);
+            inv_blk *bl = malloc((nb ? nb : 1) * sizeof(inv_blk));
+            if (bl) {
+                list_blocks(A, 0, 0, bl, 0);
+#pragma omp parallel for schedule(dynamic, 8)
+                for (size_t i = 0; i < nb; ++i)
+                    if (bl[i].d >= 0) append_block(bl[i].X, &c, bl[i].d, bl[i].q);
+                free(bl);
+            } else {
+                append_walk(A, &c, 0, 0);
+            }
             scatter_leaves(A, b, 0, hD);
         }
     }
     for (int d = 0; d < DPLRH_MAXL; ++d) {
-        free(Un[d]);
-        free(Vn[d]);
-        free(sig[d]);
+        prk_release(Un[d]);
+        prk_release(Vn[d]);
+        prk_release(sig[d]);
     }
-    free(hu);
-    free(hv);
-    free(hs);
-    free(hD);
+    prk_release(hu);
+    prk_release(hv);
+    prk_release(hs);
+    prk_release(hD);
 }

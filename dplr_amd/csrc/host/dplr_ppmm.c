@@ -351,7 +351,12 @@ void dplrgeppmm(char transa, char transb, int m, double *alpha, DPlrMtx *A, DPlr
         if (rc) fprintf(stderr, "%s: GPU execution failed (%d): %s\n", __func__, rc,
                         dplrh_last_error());
     }
-    if (!rc) ppmm_rec(&cx, 0, 0, m, a, b, *beta, C, eps);
+    if (!rc) {
+        /* the corrections reuse their buffers from one node to the next */
+        plr_prk_pool(1);
+        ppmm_rec(&cx, 0, 0, m, a, b, *beta, C, eps);
+        plr_prk_pool(0);
+    }
     for (int d = 0; d < DPLRH_MAXL; ++d) {
         free(PU[d]);
         free(PV[d]);
