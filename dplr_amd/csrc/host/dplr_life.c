@@ -17,6 +17,7 @@
 static void fill_nan(double *p, size_t n)
 {
     const double q = nan("");
+#pragma omp parallel for schedule(static) if (n >= ((size_t)1 << 18))
     for (size_t i = 0; i < n; ++i) p[i] = q;
 }
 

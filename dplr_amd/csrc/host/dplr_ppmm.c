@@ -81,8 +81,12 @@ static void lr_scale(DPlrMtx *C, int m, double beta)
 /* Columns just written (k of length m): each is normalised against the
  * threshold, sig = |col| * s[j], or sig = 0 with the column left as it is
  * when that is below eps (src/dplrppmm.c:246-258). */
+/* Work (m * columns) below which the host loops here stay on one thread. */
+#define PPMM_PAR ((size_t)1 << 15)
+
 static void lr_normalise(double *cols, int m, int k, const double *s, double *sig, double eps)
 {
+#pragma omp parallel for if ((size_t)m * k >= PPMM_PAR)
     for (int j = 0; j < k; ++j) {
         double *c = cols + (size_t)j * m;
         double ss = 0.0;
@@ -154,6 +158,7 @@ static int llmm(int m, lrv a, lrv b, double alpha, double *u3, double *s3, doubl
         /* temp (r2 x r1) = s2 .* (u2^T v1); u3 = u1, v3 = alpha v2 temp */
         double *temp = malloc((size_t)r1 * r2 * sizeof(double));
         if (!temp) return 0;
+#pragma omp parallel for collapse(2) if ((size_t)m * r1 * r2 >= PPMM_PAR)
         for (int j = 0; j < r1; ++j)
             for (int i = 0; i < r2; ++i) {
                 const double *x = b.u + (size_t)i * m, *y = a.v + (size_t)j * m;
@@ -162,6 +167,7 @@ static int llmm(int m, lrv a, lrv b, double alpha, double *u3, double *s3, doubl
                 temp[i + (size_t)j * r2] = b.sig[i] * s;
             }
         memcpy(u3, a.u, (size_t)m * r1 * sizeof(double));
+#pragma omp parallel for if ((size_t)m * r1 * r2 >= PPMM_PAR)
         for (int j = 0; j < r1; ++j) {
             double *c = v3 + (size_t)j * m;
             for (int l = 0; l < m; ++l) c[l] = 0.0;
@@ -178,6 +184,7 @@ static int llmm(int m, lrv a, lrv b, double alpha, double *u3, double *s3, doubl
     /* temp (r1 x r2) = s1 .* (v1^T u2); v3 = v2, u3 = alpha u1 temp */
     double *temp = malloc((size_t)r1 * r2 * sizeof(double));
     if (!temp) return 0;
+#pragma omp parallel for collapse(2) if ((size_t)m * r1 * r2 >= PPMM_PAR)
     for (int j = 0; j < r2; ++j)
         for (int i = 0; i < r1; ++i) {
             const double *x = a.v + (size_t)i * m, *y = b.u + (size_t)j * m;
@@ -186,6 +193,7 @@ static int llmm(int m, lrv a, lrv b, double alpha, double *u3, double *s3, doubl
             temp[i + (size_t)j * r1] = a.sig[i] * s;
         }
     memcpy(v3, b.v, (size_t)m * r2 * sizeof(double));
+#pragma omp parallel for if ((size_t)m * r1 * r2 >= PPMM_PAR)
     for (int j = 0; j < r2; ++j) {
         double *c = u3 + (size_t)j * m;
         for (int l = 0; l < m; ++l) c[l] = 0.0;
