@@ -272,10 +272,27 @@ __device__ __forceinline__ unsigned long long gj_key(double a, int pos, int k, b
     return a != a ? 1ull : (unsigned long long)__double_as_longlong(fabs(a)) + 2ull;
 }
 
-// the position of the wave's largest key (ties: the smaller position)
+// the largest of the wave's 32-bit values: the lanes of each 16-lane row
+// meet through DPP moves, the four row maxima through lane reads
+__device__ __forceinline__ unsigned gj_wave_max(unsigned v)
+{
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));
+    v = max(v, (unsigned)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));
+    return max(max((unsigned)__builtin_amdgcn_readlane(v, 0), (unsigned)__builtin_amdgcn_readlane(v, 16)),
+               max((unsigned)__builtin_amdgcn_readlane(v, 32), (unsigned)__builtin_amdgcn_readlane(v, 48)));
+}
+
+// the position of the wave's largest key (ties: the smaller position).
+// When one lane alone holds the largest upper 32 bits, its key is the
+// largest; otherwise (equal exponents and leading 20 mantissa bits, or no
+// number above zero) the full 64-bit keys and positions are compared.
 __device__ __forceinline__ int gj_pivot_key(unsigned long long key, int pos)
 {
     unsigned hi = (unsigned)(key >> 32), lo = (unsigned)key;
+    const unsigned long long top = __ballot(hi == gj_wave_max(hi));
+    if (__popcll(top) == 1) return __builtin_amdgcn_readlane(pos, __ffsll(top) - 1);
     int p = pos;
     gj_amax_dpp<0xB1>(hi, lo, p);  // quad_perm 1,0,3,2
     gj_amax_dpp<0x4E>(hi, lo, p);  // quad_perm 2,3,0,1
@@ -372,46 +389,56 @@ __global__ __launch_bounds__(256) void k_gj_inverse_w(const double *A, int n, si
     if (lane == 0) info[mat] = sinfo;
 }
 
-// The same elimination for few matrices: one work-group of NM/16 waves per
-// matrix, wave w holding columns 16w .. 16w+15 (lane i: row i).  The wave
-// that owns the step's column finds the pivot (same rule and positions as
-// k_gj_inverse_w) and publishes the column, the pivot's position and its
-// lane through LDS, double-buffered so that one barrier per step suffices;
-// every wave then updates its own columns.  Output and info as
-// k_gj_inverse_w.
-template <int NM>
-__global__ __launch_bounds__(NM * 4) void k_gj_inverse_q(const double *A, int n, size_t stride,
+// The same elimination for few matrices: one work-group of NM/CW waves per
+// matrix, wave w holding columns CW*w .. CW*w + CW - 1 (lane i: row i).  The pivot
+// of step k + 1 is found one step early: after step k's pivot is known,
+// the wave that owns column k + 1 updates that column first, finds its
+// pivot (same rule and positions as k_gj_inverse_w) and publishes the
+// column, the pivot's position, lane and value through LDS, double-buffered,
+// before it updates its other columns; so one barrier per step suffices
+// and the owner's other updates are off the step's critical path.  Output
+// and info as k_gj_inverse_w.
+// columns per wave of k_gj_inverse_q: at order 64 eight waves of 8 columns
+// beat four of 16 and sixteen of 4 (less update work for the wave that
+// finds the pivot, against one more barrier participant per halving)
+static constexpr int GJQ_CW = 8;
+template <int NM, int CW>
+__global__ __launch_bounds__(NM / CW * 64) void k_gj_inverse_q(const double *A, int n, size_t stride,
                                                          double *Inv, size_t istride, int *info)
 {
     __shared__ double colv[2][64];
-    __shared__ int pk[2][2]; // position of the pivot, its lane
+    __shared__ double pval[2]; // the pivot
+    __shared__ int pk[2][2];   // position of the pivot, its lane
     const int lane = threadIdx.x & 63;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int mat = blockIdx.x;
     const double *src = A + (size_t)mat * stride;
-    double a[16];
+    double a[CW];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int gc = 16 * wv + j;
+    for (int j = 0; j < CW; ++j) {
+        const int gc = CW * wv + j;
         a[j] = (lane < n && gc < n) ? src[(size_t)gc * n + lane] : (gc == lane ? 1.0 : 0.0);
     }
     int pos = lane, sinfo = 0;
-    const int nb = min(NM, (n + 15) & ~15) / 16;
+    const int nb = min(NM, (n + CW - 1) / CW * CW) / CW, steps = CW * nb;
+    // publish the pivot of step k on column value x (positions >= k, ties
+    // to the smaller position, NaN ranks below every number)
+    auto publish = [&](int k, double x) {
+        const int key = gj_pivot(x, pos, k, (lane < n) == (k < n));
+        const int pl = __ffsll((unsigned long long)__ballot(pos == key)) - 1;
+        const double pv = gj_rdlane(x, pl);
+        colv[k & 1][lane] = x;
+        if (lane == 0) {
+            pk[k & 1][0] = key;
+            pk[k & 1][1] = pl;
+            pval[k & 1] = pv;
+        }
+    };
+    if (wv == 0) publish(0, a[0]);
     for (int kb = 0; kb < nb; ++kb) {
 #pragma unroll
-        for (int c = 0; c < 16; ++c) {
-            const int k = 16 * kb + c, buf = k & 1;
-            if (wv == kb) {
-                // largest |a[c]| over positions >= k, ties to the smaller
-                // position (NaN ranks below every number)
-                const int key = gj_pivot(a[c], pos, k, (lane < n) == (k < n));
-                const int pl = __ffsll((unsigned long long)__ballot(pos == key)) - 1;
-                colv[buf][lane] = a[c];
-                if (lane == 0) {
-                    pk[buf][0] = key;
-                    pk[buf][1] = pl;
-                }
-            }
+        for (int c = 0; c < CW; ++c) {
+            const int k = CW * kb + c, buf = k & 1;
             __syncthreads();
             const int key = pk[buf][0], pl = pk[buf][1];
             const bool piv = pos == key;
@@ -419,12 +446,20 @@ __global__ __launch_bounds__(NM * 4) void k_gj_inverse_q(const double *A, int n,
                 pos = key;
             else if (piv)
                 pos = k;
-            const double pv = colv[buf][pl];
+            const double pv = pval[buf];
             if (pv == 0.0 && sinfo == 0) sinfo = k + 1;
             const double inv = 1.0 / pv, f = colv[buf][lane];
+            // column k + 1: register c1 of wave kb (c < CW - 1) or kb + 1
+            const int c1 = (c + 1) % CW;
+            const bool next = k + 1 < steps && wv == kb + (c == CW - 1);
+            if (next) {
+                const double pj = gj_rdlane(a[c1], pl) * inv;
+                a[c1] = piv ? pj : a[c1] - f * pj;
+                publish(k + 1, a[c1]);
+            }
 #pragma unroll
-            for (int j = 0; j < 16; ++j) {
-                if (wv == kb && j == c) continue;
+            for (int j = 0; j < CW; ++j) {
+                if ((wv == kb && j == c) || (next && j == c1)) continue;
                 const double pj = gj_rdlane(a[j], pl) * inv;
                 a[j] = piv ? pj : a[j] - f * pj;
             }
@@ -435,8 +470,8 @@ __global__ __launch_bounds__(NM * 4) void k_gj_inverse_q(const double *A, int n,
     // lane at position col
     double *dst = Inv + (size_t)mat * istride;
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int col = 16 * wv + j;
+    for (int j = 0; j < CW; ++j) {
+        const int col = CW * wv + j;
         const int pc = __ffsll((unsigned long long)__ballot(pos == col)) - 1;
         if (col < n && lane < n) dst[(size_t)pc * n + pos] = a[j];
     }
@@ -561,12 +596,12 @@ static int batched_inverse(double *A, int n, int count, double *Inv, int *piv, i
                            hipStream_t st)
 {
     if (count <= 0 || n <= 0) return 0;
-    if (n <= 64 && count <= 256) { // few matrices: a work-group per matrix
+    if (n <= 64 && count <= 1024) { // a work-group per matrix (one wave per matrix beyond)
         if (n <= 32)
-            hipLaunchKernelGGL(k_gj_inverse_q<32>, dim3(count), dim3(128), 0, st, A, n,
+            hipLaunchKernelGGL((k_gj_inverse_q<32, GJQ_CW>), dim3(count), dim3(32 / GJQ_CW * 64), 0, st, A, n,
                                (size_t)n * n, Inv, (size_t)n * n, info);
         else
-            hipLaunchKernelGGL(k_gj_inverse_q<64>, dim3(count), dim3(256), 0, st, A, n,
+            hipLaunchKernelGGL((k_gj_inverse_q<64, GJQ_CW>), dim3(count), dim3(64 / GJQ_CW * 64), 0, st, A, n,
                                (size_t)n * n, Inv, (size_t)n * n, info);
         PLR_HIP(hipGetLastError());
         return 0;
