@@ -48,6 +48,13 @@ struct plr_fact
     double *G;
     int up_d0;
     size_t goff[PLR_MAXL][PLR_MAXL];
+    // G built beside the level loop: its own stream (default priority),
+    // reduction output and partial sums, so it shares no buffer with the loop
+    double *gC, *gpart; // gC sized once for the shape
+    size_t gpart_cap;
+    hipStream_t gst;
+    hipEvent_t gev, gdone;
+    int gd0;
 };
 
 // Single-precision copies of the factors (plr_sp.hip): type 1 FP32, type 2
@@ -202,6 +209,9 @@ int plr_mm_expand_w(dplrh_plan *p, int d1, const double *Qbase, const double *W,
                     double *Y, int ldy, int ncols, hipStream_t st);
 // Reassociated upper levels of the FP64 solve (plr_up.hip).
 int plr_up_factor(dplrh_plan *p, hipStream_t st);
+int plr_up_begin(dplrh_plan *p);
+int plr_up_level(dplrh_plan *p, int e, hipStream_t st);
+int plr_up_end(dplrh_plan *p, hipStream_t st);
 void plr_up_release(dplrh_plan *p);
 int plr_up_solve(dplrh_plan *p, int d0, double *X, int ldx, int ncols, hipStream_t st);
 int plr_up_ready(const dplrh_plan *p, int d0);
@@ -209,7 +219,9 @@ int plr_up_reserve(dplrh_plan *p, int d0, int ncols);
 int plr_up_steps(dplrh_plan *p, int d0, int ncols, hipStream_t st);
 int plr_tail_d0(const dplrh_plan *p);
 int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, double *C,
-                    size_t cap, int *w_off, hipStream_t st);
+                    size_t cap, int *w_off, hipStream_t st, double *part = nullptr,
+                    size_t part_cap = 0);
+int plr_mm_cross_rm_size(const dplrh_plan *p, int e, size_t *c_doubles, size_t *part_doubles);
 int plr_mm_apply_split(dplrh_plan *p, int nc0, const double *X0, int ldx0, double *Y0, int ldy0,
                        int nc1, const double *X1, int ldx1, const double *C1, int ldc1, double *Y1,
                        int ldy1, double *bmax, hipStream_t st);

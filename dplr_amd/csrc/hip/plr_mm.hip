@@ -259,8 +259,29 @@ int plr_mm_cross(dplrh_plan *p, int e, const double *X, int ldx, int ncols, doub
 // output rank-major: C[(w_off[d] + s * R_d + k) * 64 + c] for c < ncols <= 64.
 // cap: doubles C holds.  Returns 1, launching nothing, when the shape or X
 // does not suit that kernel, or C is too small.
+// Doubles of plr_mm_cross_rm's output and of its partial sums for level e
+// (64 columns); 1 when the shape takes plr_mm_cross instead.
+int plr_mm_cross_rm_size(const dplrh_plan *p, int e, size_t *c_doubles, size_t *part_doubles)
+{
+    const bool leaves_ok = p->b >= 64 && (p->b & (p->b - 1)) == 0;
+    if (e <= 0 || p->N % MMF_RB != 0 || !leaves_ok || p->force_valu) return 1;
+    const int seg = p->N >> (e + 1), nseg = 2 << e;
+    const int T = seg < MM_TMAX ? seg : MM_TMAX, tps = seg / T;
+    size_t part = 0, woff = 0;
+    for (int d = 0; d < e; ++d) {
+        const int R = p->R[d], nkg = (R + MM_KG - 1) / MM_KG;
+        if (tps > 1) part += (size_t)nkg * nseg * tps;
+        woff += (size_t)nseg * R;
+    }
+    *c_doubles = woff * MM_CG;
+    *part_doubles = part * MM_BLK;
+    return 0;
+}
+
+// With `pbuf` the partial sums go there (1 returned, nothing launched, when
+// they need more than pcap doubles), else to the plan's shared buffer.
 int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, double *C,
-                    size_t cap, int *w_off, hipStream_t st)
+                    size_t cap, int *w_off, hipStream_t st, double *pbuf, size_t pcap)
 {
     if (e <= 0 || ncols <= 0 || ncols > MM_CG) {
         plr_set_error("plr_mm_cross_rm: level %d, %d columns", e, ncols);
@@ -305,8 +326,13 @@ int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, d
     t.w_off[e] = woff;
     w_off[e] = woff;
     if ((size_t)woff * MM_CG > cap) return 1;
-    int rc = mm_part(p, (size_t)part);
-    if (rc) return rc;
+    if (pbuf) {
+        if ((size_t)part * MM_BLK > pcap) return 1;
+    } else {
+        int rc = mm_part(p, (size_t)part);
+        if (rc) return rc;
+        pbuf = p->tmp;
+    }
     if (task > 0) {
         static bool attr = false;
         if (!attr) {
@@ -315,13 +341,13 @@ int plr_mm_cross_rm(dplrh_plan *p, int e, const double *X, int ldx, int ncols, d
             attr = true;
         }
         hipLaunchKernelGGL(k_mm_reduce_fused2, dim3(N / MMF_RB, 1), dim3(64 * MMF2_NW),
-                           (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, 1.0, p->tmp,
+                           (size_t)MMF_RB * MMF2_XS * sizeof(double), st, t, X, ldx, 1.0, pbuf,
                            C, (size_t)MM_CG, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     if (fin > 0) {
         hipLaunchKernelGGL(k_mm_finish<true>, dim3(fin * MM_FCH), dim3(256), 0, st, t, 1.0, 0, 1,
-                           p->tmp, C, (size_t)MM_CG, MmSplit{});
+                           pbuf, C, (size_t)MM_CG, MmSplit{});
         PLR_HIP(hipGetLastError());
     }
     return 0;

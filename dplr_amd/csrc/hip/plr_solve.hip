@@ -691,11 +691,16 @@ static size_t col_off(const dplrh_plan *p, int d) { return p->N ? p->uoff[d] / p
 void plr_free_fact(dplrh_plan *p)
 {
     ++p->ws_gen;
+    if (p->f.gst) (void)hipStreamSynchronize(p->f.gst);
     void *bufs[] = {p->f.Dinv, p->f.Ut, p->f.Kinv, p->f.dinfo, p->f.work, p->f.G,
+                    p->f.gC, p->f.gpart,
                     p->sw.T, p->sw.W, p->sw.part, p->sw.tmp, p->sw.cnt,
                     p->sw.uT0, p->sw.uW, p->sw.uA[0], p->sw.uA[1]};
     for (void *b : bufs)
         if (b) (void)hipFree(b);
+    if (p->f.gev) (void)hipEventDestroy(p->f.gev);
+    if (p->f.gdone) (void)hipEventDestroy(p->f.gdone);
+    if (p->f.gst) (void)hipStreamDestroy(p->f.gst);
     memset(&p->f, 0, sizeof(p->f));
     memset(&p->sw, 0, sizeof(p->sw));
 }
@@ -779,6 +784,8 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
     hipStream_t st = (hipStream_t)stream;
     *info = 0;
     plr_fact &f = p->f;
+    // G work of an earlier call that stopped on an error may still read Ut
+    if (f.gst) PLR_HIP(hipStreamSynchronize(f.gst));
     const int N = p->N, L = p->L, b = p->b;
     const int nleaf = b ? N / b : 0;
     if (N == 0) {
@@ -844,7 +851,10 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         if (rc) return rc;
     }
 
-    // 3. levels, bottom-up
+    // 3. levels, bottom-up; the G blocks of the reassociated upper solve
+    // beside them on their own stream when the shape allows
+    const int gd0 = plr_up_begin(p);
+    if (gd0 < 0) return gd0;
     for (int d = L - 1; d >= 0; --d) {
         const int R = p->R[d];
         if (R == 0) continue;
@@ -870,6 +880,8 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         hipLaunchKernelGGL(k_capf_build, dim3(nodes), dim3(256), 0, st, p->sw.T, p->sw.t_per_col,
                            off, R, Sd, work);
         PLR_HIP(hipGetLastError());
+        // G of level d beside this level's inverses (few work-groups)
+        if (gd0 > 0 && d >= 1 && d < gd0 && (rc = plr_up_level(p, d, st))) return rc;
         if ((rc = batched_inverse(work, n2, nodes, f.Kinv + f.koff[d], piv,
                                   f.dinfo + nleaf + (nodes - 1), st)))
             return rc;
@@ -895,6 +907,10 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
             if (rc) return rc;
         }
     }
+    if (gd0 > 0) {
+        const int rc = plr_up_end(p, st);
+        if (rc) return rc;
+    }
     // every LU status at once: leaves, then the nodes of each level with a rank
     if (read_info(nleaf + (1 << L) - 1, f.dinfo, h, st)) return -10;
     for (int l = 0; l < nleaf; ++l)
@@ -907,7 +923,9 @@ extern "C" int dplrh_factor(dplrh_plan *p, int *info, void *stream)
         }
     }
     // the cross-level blocks of the reassociated upper solve
-    if (fails.empty()) {
+    if (gd0 > 0) {
+        f.up_d0 = fails.empty() ? gd0 : 0;
+    } else if (fails.empty()) {
         const int rc = plr_up_factor(p, st);
         if (rc) return rc;
     }

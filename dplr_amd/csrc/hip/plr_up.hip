@@ -254,17 +254,11 @@ __global__ void k_up_greshape_rm(const double *__restrict__ C, UpGTab t)
 
 void plr_up_release(dplrh_plan *p) { (void)p; }
 
-// After the factorization: the G blocks.  For each level e one reduction of
-// Ut_e's columns against the V factors of every coarser level, cut into
-// level e's segments (the FP64 MFMA reduction kernels with a table of those
-// levels, tiles of 256 rows summed in a fixed order), then reshaped.
-int plr_up_factor(dplrh_plan *p, hipStream_t st)
+// Offsets of the G blocks of levels below d0 and their buffer (sizes
+// depend on the shape only, so it is allocated once).
+static int up_alloc_g(dplrh_plan *p, int d0)
 {
     plr_fact &f = p->f;
-    const int d0 = up_shape(p);
-    f.up_d0 = 0;
-    if (!d0) return 0;
-    const int N = p->N;
     size_t tot = 0;
     for (int e = 1; e < d0; ++e)
         for (int d = 0; d < e; ++d) {
@@ -272,31 +266,56 @@ int plr_up_factor(dplrh_plan *p, hipStream_t st)
             tot += ((size_t)2 << e) * p->R[d] * p->R[e];
         }
     if (!f.G) PLR_HIP(hipMalloc(&f.G, tot * sizeof(double)));
+    return 0;
+}
+
+// G_{d,e} of every d < e from plr_mm_cross_rm's rank-major output C.
+static int up_greshape(dplrh_plan *p, int e, const double *C, const int *w_off, hipStream_t st)
+{
+    plr_fact &f = p->f;
+    const int Re = p->R[e], nseg = 2 << e;
+    UpGTab g;
+    memset(&g, 0, sizeof(g));
+    g.e = e;
+    g.Re = Re;
+    g.nseg = nseg;
+    size_t n = 0;
+    for (int d = 0; d < e; ++d) {
+        g.Rd[d] = p->R[d];
+        g.w_off[d] = w_off[d];
+        g.first[d] = n;
+        g.G[d] = f.G + f.goff[d][e];
+        n += (size_t)nseg * p->R[d] * Re;
+    }
+    g.first[e] = n;
+    hipLaunchKernelGGL(k_up_greshape_rm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, C, g);
+    PLR_HIP(hipGetLastError());
+    return 0;
+}
+
+// After the factorization: the G blocks.  For each level e one reduction of
+// Ut_e's columns against the V factors of every coarser level, cut into
+// level e's segments (the FP64 MFMA reduction kernels with a table of those
+// levels, tiles of 256 rows summed in a fixed order), then reshaped.  The
+// shapes plr_up_begin does not take.
+int plr_up_factor(dplrh_plan *p, hipStream_t st)
+{
+    plr_fact &f = p->f;
+    const int d0 = up_shape(p);
+    f.up_d0 = 0;
+    if (!d0) return 0;
+    const int N = p->N;
+    int rc = up_alloc_g(p, d0);
+    if (rc) return rc;
     double *C = f.work; // free once the factorization is done; it holds N * b doubles
     for (int e = 1; e < d0; ++e) {
         const int Re = p->R[e], nseg = 2 << e;
         int w_off[PLR_MAXL + 1];
         size_t ws = 0;
-        int rc = plr_mm_cross_rm(p, e, f.Ut + p->uoff[e], N, Re, C, (size_t)N * p->b, w_off, st);
+        rc = plr_mm_cross_rm(p, e, f.Ut + p->uoff[e], N, Re, C, (size_t)N * p->b, w_off, st);
         if (rc < 0) return rc;
         if (rc == 0) {
-            UpGTab g;
-            memset(&g, 0, sizeof(g));
-            g.e = e;
-            g.Re = Re;
-            g.nseg = nseg;
-            size_t n = 0;
-            for (int d = 0; d < e; ++d) {
-                g.Rd[d] = p->R[d];
-                g.w_off[d] = w_off[d];
-                g.first[d] = n;
-                g.G[d] = f.G + f.goff[d][e];
-                n += (size_t)nseg * p->R[d] * Re;
-            }
-            g.first[e] = n;
-            hipLaunchKernelGGL(k_up_greshape_rm, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st,
-                               C, g);
-            PLR_HIP(hipGetLastError());
+            if ((rc = up_greshape(p, e, C, w_off, st))) return rc;
             continue;
         }
         rc = plr_mm_cross(p, e, f.Ut + p->uoff[e], N, Re, C, w_off, &ws, st);
@@ -310,6 +329,70 @@ int plr_up_factor(dplrh_plan *p, hipStream_t st)
         }
     }
     f.up_d0 = d0;
+    return 0;
+}
+
+// The G blocks beside the factorization's level loop.  Ut_e is final once
+// the loop has finished level e + 1 (only finer levels update it), and the
+// loop's later levels read it without writing it; so G of level e goes out
+// on a second stream of default priority, after an event on the loop's
+// stream, as level e's capacitance inverses start (a few work-groups each),
+// with its own reduction output and partial sums.  plr_up_begin
+// (before the loop) returns up_d0, or 0 when the shape takes plr_up_factor
+// after the loop instead; plr_up_end makes the loop's stream wait for G.
+int plr_up_begin(dplrh_plan *p)
+{
+    plr_fact &f = p->f;
+    f.gd0 = 0;
+    const int d0 = up_shape(p);
+    if (!d0) return 0;
+    size_t cmax = 0, pmax = 1;
+    for (int e = 1; e < d0; ++e) {
+        size_t c = 0, q = 0;
+        if (plr_mm_cross_rm_size(p, e, &c, &q)) return 0;
+        cmax = c > cmax ? c : cmax;
+        pmax = q > pmax ? q : pmax;
+    }
+    int rc = up_alloc_g(p, d0);
+    if (rc) return rc;
+    if (!f.gC) PLR_HIP(hipMalloc(&f.gC, cmax * sizeof(double)));
+    if (pmax > f.gpart_cap) {
+        if (f.gpart) PLR_HIP(hipFree(f.gpart));
+        f.gpart = nullptr;
+        f.gpart_cap = 0;
+        PLR_HIP(hipMalloc(&f.gpart, pmax * sizeof(double)));
+        f.gpart_cap = pmax;
+    }
+    if (!f.gst) {
+        PLR_HIP(hipStreamCreateWithFlags(&f.gst, hipStreamNonBlocking));
+        PLR_HIP(hipEventCreateWithFlags(&f.gev, hipEventDisableTiming));
+        PLR_HIP(hipEventCreateWithFlags(&f.gdone, hipEventDisableTiming));
+    }
+    f.gd0 = d0;
+    return d0;
+}
+
+int plr_up_level(dplrh_plan *p, int e, hipStream_t st)
+{
+    plr_fact &f = p->f;
+    if (!f.gd0 || e < 1 || e >= f.gd0) return 0;
+    size_t c = 0, q = 0;
+    if (plr_mm_cross_rm_size(p, e, &c, &q)) return -3;
+    PLR_HIP(hipEventRecord(f.gev, st));
+    PLR_HIP(hipStreamWaitEvent(f.gst, f.gev, 0));
+    int w_off[PLR_MAXL + 1];
+    const int rc = plr_mm_cross_rm(p, e, f.Ut + p->uoff[e], p->N, p->R[e], f.gC, c, w_off, f.gst,
+                                   f.gpart, f.gpart_cap);
+    if (rc) return rc < 0 ? rc : -3;
+    return up_greshape(p, e, f.gC, w_off, f.gst);
+}
+
+int plr_up_end(dplrh_plan *p, hipStream_t st)
+{
+    plr_fact &f = p->f;
+    if (!f.gd0) return 0;
+    PLR_HIP(hipEventRecord(f.gdone, f.gst));
+    PLR_HIP(hipStreamWaitEvent(st, f.gdone, 0));
     return 0;
 }
 
