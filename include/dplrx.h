@@ -61,7 +61,10 @@ int dplrx_gepdmm(dplrx_plan *plan, char transa, int n, double alpha, const doubl
 
 /* Factor A in HBM for solves: FP64 leaf LU and inverses, Dinv-applied U
  * factors and one inverted capacitance matrix per node.  *info as in
- * dplrgeinv (> 0: singular leaf or capacitance). */
+ * dplrgeinv (> 0: singular leaf or capacitance).  Returns after the
+ * factorization is complete; part of it runs on a second stream of the
+ * plan (default priority), ordered after `stream` and joined back to it by
+ * events, so the call is not for graph capture. */
 int dplrx_factor(dplrx_plan *plan, int *info, void *stream);
 
 /* X := op(A)^{-1} B for n device columns (ldb, ldx >= m), after
